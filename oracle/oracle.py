@@ -1,0 +1,449 @@
+"""ctypes front end for the CPU checkers.  TEST INFRASTRUCTURE ONLY.
+
+Two libraries:
+  * ``liboracle.so``       -- the restatement (oracle/dmpm_oracle.cpp), float32
+                              and float64;
+  * ``_ref/libdmpm_ref.so`` -- the reference's own DiffMPMLib3D sources compiled
+                              with ref_capi.cpp (float32 only; present only
+                              where it was built from /root/reference).
+
+Only tests/, the smoke check and bench.py's CPU-baseline leg import this
+module.  The shipped library never does.
+
+Array conventions: x, v (N,3); F, C, dFc (N,3,3) row-major; grid node
+(i,j,k) at flat index i*ny*nz + j*nz + k (Grid.h:89-94).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libdmpm_ref.so")
+
+
+class _CParams(C.Structure):
+    _fields_ = [
+        ("dims", C.c_int * 3),
+        ("dx", C.c_double),
+        ("grid_min", C.c_double * 3),
+        ("dt", C.c_double),
+        ("drag", C.c_double),
+        ("smoothing", C.c_double),
+        ("zeta", C.c_double),
+        ("jmin", C.c_double),
+        ("f_ext", C.c_double * 3),
+        ("order", C.c_int),
+        ("model", C.c_int),
+        ("trust_region", C.c_int),
+        ("wall_cells", C.c_int),
+        ("adjoint", C.c_int),
+    ]
+
+
+@dataclass
+class Params:
+    """Step parameters.  Defaults are the reference's own physics
+    (ForwardSimulation.cpp:13, Interpolation.h, Elasticity.cpp:45-55)."""
+
+    dims: tuple = (16, 16, 16)
+    dx: float = 1.0 / 15
+    grid_min: tuple = (0.0, 0.0, 0.0)
+    dt: float = 1e-3
+    drag: float = 0.0
+    smoothing: float = 0.0
+    zeta: float = 0.25
+    jmin: float = 0.6
+    f_ext: tuple = (0.0, 0.0, 0.0)
+    order: int = 3
+    model: int = 0
+    trust_region: int = 1
+    wall_cells: int = 0
+    adjoint: int = 1
+
+    def c(self) -> _CParams:
+        p = _CParams()
+        p.dims[:] = [int(d) for d in self.dims]
+        p.dx = float(self.dx)
+        p.grid_min[:] = [float(a) for a in self.grid_min]
+        p.dt, p.drag, p.smoothing = float(self.dt), float(self.drag), float(self.smoothing)
+        p.zeta, p.jmin = float(self.zeta), float(self.jmin)
+        p.f_ext[:] = [float(a) for a in self.f_ext]
+        p.order, p.model = int(self.order), int(self.model)
+        p.trust_region, p.wall_cells, p.adjoint = int(self.trust_region), int(self.wall_cells), int(self.adjoint)
+        return p
+
+    @property
+    def G(self) -> int:
+        return int(np.prod(self.dims))
+
+
+def reference_params(**kw) -> Params:
+    """The reference's physics: cubic, fixed-corotated, zeta=0.25, J_min=0.6,
+    reference adjoint."""
+    base = dict(order=3, model=0, trust_region=1, zeta=0.25, jmin=0.6, wall_cells=0, adjoint=1)
+    base.update(kw)
+    return Params(**base)
+
+
+def plain_params(**kw) -> Params:
+    """The configs' plain method: quadratic, Neo-Hookean, no bulk viscosity,
+    no trust region, no smoothing or drag, sticky walls at 3 cells, exact adjoint."""
+    base = dict(order=2, model=1, trust_region=0, zeta=0.0, smoothing=0.0, drag=0.0,
+                wall_cells=3, adjoint=0)
+    base.update(kw)
+    return Params(**base)
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(C.c_void_p)
+
+
+_FIELDS = ("x", "v", "F", "C", "dFc", "m", "vol", "lam", "mu")
+
+
+def _state_args(s: dict, dt):
+    out = []
+    for k in _FIELDS:
+        a = s.get(k)
+        if a is None:
+            out.append(None)
+            continue
+        out.append(np.ascontiguousarray(a, dtype=dt))
+    return out
+
+
+class Oracle:
+    """The restatement, in float32 ("f32") or float64 ("f64")."""
+
+    _lib = None
+
+    def __init__(self, precision: str = "f64"):
+        if Oracle._lib is None:
+            if not os.path.exists(ORACLE_SO):
+                raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run make -C oracle)")
+            Oracle._lib = C.CDLL(ORACLE_SO)
+            for s in ("f32", "f64"):
+                getattr(Oracle._lib, f"orc_mass_loss_{s}").restype = C.c_double
+                getattr(Oracle._lib, f"orc_dLdF_norm_{s}").restype = C.c_double
+        self.p = precision
+        self.dt = np.float32 if precision == "f32" else np.float64
+
+    def _f(self, name):
+        return getattr(self._lib, f"orc_{name}_{self.p}")
+
+    def forward_step(self, prm: Params, s: dict) -> dict:
+        n = s["x"].shape[0]
+        args = _state_args(s, self.dt)
+        o = {k: np.zeros((n, 3), self.dt) for k in ("x", "v")}
+        o.update({k: np.zeros((n, 3, 3), self.dt) for k in ("F", "C", "P")})
+        gm = np.zeros(prm.G, self.dt)
+        gv = np.zeros((prm.G, 3), self.dt)
+        cp = prm.c()
+        self._f("forward_step")(C.byref(cp), n, *[_ptr(a) for a in args],
+                                _ptr(o["x"]), _ptr(o["v"]), _ptr(o["F"]), _ptr(o["C"]), _ptr(o["P"]),
+                                _ptr(gm), _ptr(gv))
+        o["grid_m"], o["grid_v"] = gm, gv
+        return o
+
+    def backward_step(self, prm: Params, s: dict, adj1: dict) -> dict:
+        n = s["x"].shape[0]
+        args = _state_args(s, self.dt)
+        a1 = [np.ascontiguousarray(adj1[k], dtype=self.dt) for k in ("x", "v", "F", "C")]
+        o = {k: np.zeros((n, 3), self.dt) for k in ("x", "v")}
+        o.update({k: np.zeros((n, 3, 3), self.dt) for k in ("F", "C", "dFc", "P")})
+        cp = prm.c()
+        self._f("backward_step")(C.byref(cp), n, *[_ptr(a) for a in args], *[_ptr(a) for a in a1],
+                                 _ptr(o["x"]), _ptr(o["v"]), _ptr(o["F"]), _ptr(o["C"]), _ptr(o["dFc"]),
+                                 _ptr(o["P"]))
+        return o
+
+    def pk1(self, model, F, lam, mu):
+        F = np.ascontiguousarray(F, self.dt)
+        n = F.shape[0]
+        lam = np.ascontiguousarray(np.broadcast_to(lam, (n,)), self.dt)
+        mu = np.ascontiguousarray(np.broadcast_to(mu, (n,)), self.dt)
+        P = np.zeros_like(F)
+        self._f("pk1")(int(model), n, _ptr(F), _ptr(lam), _ptr(mu), _ptr(P))
+        return P
+
+    def pk1_hvp(self, model, F, lam, mu, dF):
+        F = np.ascontiguousarray(F, self.dt)
+        dF = np.ascontiguousarray(dF, self.dt)
+        n = F.shape[0]
+        lam = np.ascontiguousarray(np.broadcast_to(lam, (n,)), self.dt)
+        mu = np.ascontiguousarray(np.broadcast_to(mu, (n,)), self.dt)
+        out = np.zeros_like(F)
+        self._f("pk1_hvp")(int(model), n, _ptr(F), _ptr(lam), _ptr(mu), _ptr(dF), _ptr(out))
+        return out
+
+    def energy(self, model, F, lam, mu):
+        F = np.ascontiguousarray(F, self.dt)
+        n = F.shape[0]
+        lam = np.ascontiguousarray(np.broadcast_to(lam, (n,)), self.dt)
+        mu = np.ascontiguousarray(np.broadcast_to(mu, (n,)), self.dt)
+        psi = np.zeros(n, self.dt)
+        self._f("energy")(int(model), n, _ptr(F), _ptr(lam), _ptr(mu), _ptr(psi))
+        return psi
+
+    def svd(self, F):
+        F = np.ascontiguousarray(F, self.dt)
+        n = F.shape[0]
+        U, V = np.zeros_like(F), np.zeros_like(F)
+        s = np.zeros((n, 3), self.dt)
+        self._f("svd")(n, _ptr(F), _ptr(U), _ptr(s), _ptr(V))
+        return U, s, V
+
+    def polar(self, F):
+        F = np.ascontiguousarray(F, self.dt)
+        R, S = np.zeros_like(F), np.zeros_like(F)
+        self._f("polar")(F.shape[0], _ptr(F), _ptr(R), _ptr(S))
+        return R, S
+
+    def p2g_mass(self, prm: Params, x, m):
+        x = np.ascontiguousarray(x, self.dt)
+        m = np.ascontiguousarray(m, self.dt)
+        gm = np.zeros(prm.G, self.dt)
+        cp = prm.c()
+        self._f("p2g_mass")(C.byref(cp), x.shape[0], _ptr(x), _ptr(m), _ptr(gm))
+        return gm
+
+    def volumes(self, prm: Params, x, m):
+        x = np.ascontiguousarray(x, self.dt)
+        m = np.ascontiguousarray(m, self.dt)
+        vol = np.zeros(x.shape[0], self.dt)
+        gm = np.zeros(prm.G, self.dt)
+        cp = prm.c()
+        self._f("volumes")(C.byref(cp), x.shape[0], _ptr(x), _ptr(m), _ptr(vol), _ptr(gm))
+        return vol, gm
+
+    def mass_loss(self, prm: Params, x, m, target_m):
+        x = np.ascontiguousarray(x, self.dt)
+        m = np.ascontiguousarray(m, self.dt)
+        tm = np.ascontiguousarray(target_m, self.dt)
+        gx = np.zeros_like(x)
+        gm = np.zeros(prm.G, self.dt)
+        gdm = np.zeros(prm.G, self.dt)
+        cp = prm.c()
+        loss = self._f("mass_loss")(C.byref(cp), x.shape[0], _ptr(x), _ptr(m), _ptr(tm), _ptr(gx),
+                                    _ptr(gm), _ptr(gdm))
+        return loss, gx, gm, gdm
+
+    def adam(self, dLdF, dFc, mom, vec, vmax, alpha, grad_norm, beta1=0.9, beta2=0.999, eps=1e-3,
+             timestep=1):
+        arrs = [np.array(a, dtype=self.dt, order="C", copy=True) for a in (dFc, mom, vec, vmax)]
+        g = np.ascontiguousarray(dLdF, self.dt)
+        self._f("adam")(g.shape[0], _ptr(g), *[_ptr(a) for a in arrs], C.c_double(alpha),
+                        C.c_double(grad_norm), C.c_double(beta1), C.c_double(beta2), C.c_double(eps),
+                        int(timestep))
+        return tuple(arrs)
+
+    def dLdF_norm(self, dLdF):
+        g = np.ascontiguousarray(dLdF, self.dt)
+        return self._f("dLdF_norm")(g.shape[0], _ptr(g))
+
+    def bspline(self, order, u):
+        u = np.ascontiguousarray(u, self.dt)
+        w, dw = np.zeros_like(u), np.zeros_like(u)
+        self._f("bspline")(int(order), u.size, _ptr(u), _ptr(w), _ptr(dw))
+        return w, dw
+
+    # -- multi-step helpers (small sizes) -------------------------------------
+    def rollout(self, prm: Params, s0: dict, steps: int, controls: Optional[list] = None) -> list:
+        """Forward `steps` steps; returns the list of layer states (steps+1).
+        controls[t] (N,3,3) is dFc of layer t (zero/None otherwise)."""
+        layers = [dict(s0)]
+        for t in range(steps):
+            s = dict(layers[-1])
+            s["dFc"] = None if controls is None else controls[t]
+            o = self.forward_step(prm, s)
+            nxt = {k: o[k] for k in ("x", "v", "F", "C")}
+            for k in ("m", "vol", "lam", "mu"):
+                nxt[k] = s0[k]
+            layers[-1] = s
+            layers.append(nxt)
+        return layers
+
+    def rollout_grad(self, prm: Params, layers: list, gT: dict, stop: int = 0) -> tuple:
+        """Backpropagate adjoint gT (keys x, v, F, C) of the last layer to layer
+        `stop`.  Returns (adjoint of layer stop, list of dFc gradients per layer)."""
+        adj = {k: np.asarray(gT[k], self.dt) for k in ("x", "v", "F", "C")}
+        gctl = [None] * (len(layers) - 1)
+        for t in range(len(layers) - 2, stop - 1, -1):
+            o = self.backward_step(prm, layers[t], adj)
+            gctl[t] = o["dFc"]
+            adj = {k: o[k] for k in ("x", "v", "F", "C")}
+        return adj, gctl
+
+
+class Ref:
+    """The reference's own code (float32), through oracle/ref_capi.cpp."""
+
+    _lib = None
+
+    @staticmethod
+    def available() -> bool:
+        return os.path.exists(REF_SO)
+
+    def __init__(self, threads: int = 1):
+        if Ref._lib is None:
+            if not os.path.exists(REF_SO):
+                raise RuntimeError(f"reference library missing: {REF_SO} (run make -C oracle ref)")
+            Ref._lib = C.CDLL(REF_SO)
+            for f in ("ref_episode", "ref_mass_loss", "ref_dLdF_norm"):
+                getattr(Ref._lib, f).restype = C.c_float
+            Ref._lib.ref_max_threads.restype = C.c_int
+        self.set_threads(threads)
+
+    def set_threads(self, n: int):
+        self._lib.ref_set_threads(int(n))
+
+    def max_threads(self) -> int:
+        return self._lib.ref_max_threads()
+
+    @staticmethod
+    def _grid(prm: Params):
+        dims = (C.c_int * 3)(*[int(d) for d in prm.dims])
+        mn = np.ascontiguousarray(prm.grid_min, np.float32)
+        return dims, mn
+
+    @staticmethod
+    def _f3(a):
+        return np.ascontiguousarray(a, np.float32)
+
+    def _state(self, s):
+        n = s["x"].shape[0]
+        out = []
+        for k in _FIELDS:
+            a = s.get(k)
+            if a is None and k == "dFc":
+                a = np.zeros((n, 3, 3), np.float32)
+            out.append(self._f3(a))
+        return out
+
+    def forward_step(self, prm: Params, s: dict) -> dict:
+        n = s["x"].shape[0]
+        dims, mn = self._grid(prm)
+        args = self._state(s)
+        o = {k: np.zeros((n, 3), np.float32) for k in ("x", "v")}
+        o.update({k: np.zeros((n, 3, 3), np.float32) for k in ("F", "C", "P")})
+        gm = np.zeros(prm.G, np.float32)
+        gv = np.zeros((prm.G, 3), np.float32)
+        fe = self._f3(prm.f_ext)
+        self._lib.ref_forward_step(n, *[_ptr(a) for a in args], dims, C.c_float(prm.dx), _ptr(mn),
+                                   C.c_float(prm.smoothing), C.c_float(prm.dt), C.c_float(prm.drag), _ptr(fe),
+                                   _ptr(o["x"]), _ptr(o["v"]), _ptr(o["F"]), _ptr(o["C"]), _ptr(o["P"]),
+                                   _ptr(gm), _ptr(gv))
+        o["grid_m"], o["grid_v"] = gm, gv
+        return o
+
+    def backward_step(self, prm: Params, s: dict, adj1: dict) -> dict:
+        n = s["x"].shape[0]
+        dims, mn = self._grid(prm)
+        args = self._state(s)
+        a1 = [self._f3(adj1[k]) for k in ("x", "v", "F", "C")]
+        o = {k: np.zeros((n, 3), np.float32) for k in ("x", "v")}
+        o.update({k: np.zeros((n, 3, 3), np.float32) for k in ("F", "C", "P")})
+        fe = self._f3(prm.f_ext)
+        self._lib.ref_backward_step(n, *[_ptr(a) for a in args], dims, C.c_float(prm.dx), _ptr(mn),
+                                    C.c_float(prm.smoothing), C.c_float(prm.dt), C.c_float(prm.drag), _ptr(fe),
+                                    *[_ptr(a) for a in a1], _ptr(o["x"]), _ptr(o["v"]), _ptr(o["F"]),
+                                    _ptr(o["C"]), _ptr(o["P"]))
+        return o
+
+    def episode(self, prm: Params, s: dict, target_m, num_layers: int, control_layer: int = 0,
+                render_gF=None, render_gx=None) -> dict:
+        n = s["x"].shape[0]
+        dims, mn = self._grid(prm)
+        args = self._state(s)
+        tm = self._f3(target_m)
+        o = {k: np.zeros((n, 3), np.float32) for k in ("x", "v", "gx", "gv")}
+        o.update({k: np.zeros((n, 3, 3), np.float32) for k in ("F", "C", "gF", "gC")})
+        fe = self._f3(prm.f_ext)
+        rgF = None if render_gF is None else self._f3(render_gF)
+        rgx = None if render_gx is None else self._f3(render_gx)
+        loss = self._lib.ref_episode(n, *[_ptr(a) for a in args], dims, C.c_float(prm.dx), _ptr(mn), _ptr(tm),
+                                     int(num_layers), int(control_layer), C.c_float(prm.smoothing),
+                                     C.c_float(prm.dt), C.c_float(prm.drag), _ptr(fe), _ptr(rgF), _ptr(rgx),
+                                     _ptr(o["x"]), _ptr(o["v"]), _ptr(o["F"]), _ptr(o["C"]),
+                                     _ptr(o["gx"]), _ptr(o["gv"]), _ptr(o["gF"]), _ptr(o["gC"]))
+        o["loss"] = float(loss)
+        return o
+
+    def mass_loss(self, prm: Params, s: dict, target_m):
+        n = s["x"].shape[0]
+        dims, mn = self._grid(prm)
+        args = self._state(s)
+        del args[4]  # no dFc
+        gx = np.zeros((n, 3), np.float32)
+        gm = np.zeros(prm.G, np.float32)
+        gdm = np.zeros(prm.G, np.float32)
+        loss = self._lib.ref_mass_loss(n, *[_ptr(a) for a in args], dims, C.c_float(prm.dx), _ptr(mn),
+                                       _ptr(self._f3(target_m)), _ptr(gx), _ptr(gm), _ptr(gdm))
+        return float(loss), gx, gm, gdm
+
+    def p2g_mass(self, prm: Params, x, m):
+        dims, mn = self._grid(prm)
+        gm = np.zeros(prm.G, np.float32)
+        x, m = self._f3(x), self._f3(m)
+        self._lib.ref_p2g_mass(x.shape[0], _ptr(x), _ptr(m), dims, C.c_float(prm.dx), _ptr(mn), _ptr(gm))
+        return gm
+
+    def volumes(self, prm: Params, x, m):
+        dims, mn = self._grid(prm)
+        x, m = self._f3(x), self._f3(m)
+        vol = np.zeros(x.shape[0], np.float32)
+        gm = np.zeros(prm.G, np.float32)
+        self._lib.ref_volumes(x.shape[0], _ptr(x), _ptr(m), dims, C.c_float(prm.dx), _ptr(mn), _ptr(vol), _ptr(gm))
+        return vol, gm
+
+    def adam(self, dLdF, dFc, mom, vec, vmax, alpha, grad_norm, beta1=0.9, beta2=0.999, eps=1e-3, timestep=1):
+        arrs = [np.array(a, dtype=np.float32, order="C", copy=True) for a in (dFc, mom, vec, vmax)]
+        g = self._f3(dLdF)
+        self._lib.ref_adam(g.shape[0], _ptr(g), *[_ptr(a) for a in arrs], C.c_float(alpha), C.c_float(grad_norm),
+                           C.c_float(beta1), C.c_float(beta2), C.c_float(eps), int(timestep))
+        return tuple(arrs)
+
+    def dLdF_norm(self, dLdF):
+        g = self._f3(dLdF)
+        return float(self._lib.ref_dLdF_norm(g.shape[0], _ptr(g)))
+
+    def _mat(self, fn, F, lam, mu, extra=None):
+        F = self._f3(F)
+        n = F.shape[0]
+        lam = self._f3(np.broadcast_to(lam, (n,)))
+        mu = self._f3(np.broadcast_to(mu, (n,)))
+        out = np.zeros_like(F) if fn != "ref_energy_fcr" else np.zeros(n, np.float32)
+        a = [n, _ptr(F), _ptr(lam), _ptr(mu)]
+        if extra is not None:
+            a.append(_ptr(self._f3(extra)))
+        getattr(self._lib, fn)(*a, _ptr(out))
+        return out
+
+    def pk_fcr(self, F, lam, mu):
+        return self._mat("ref_pk_fcr", F, lam, mu)
+
+    def pk_fcr_hvp(self, F, lam, mu, dF):
+        return self._mat("ref_pk_fcr_hvp", F, lam, mu, dF)
+
+    def energy_fcr(self, F, lam, mu):
+        return self._mat("ref_energy_fcr", F, lam, mu)
+
+    def polar(self, F):
+        F = self._f3(F)
+        R, S = np.zeros_like(F), np.zeros_like(F)
+        self._lib.ref_polar(F.shape[0], _ptr(F), _ptr(R), _ptr(S))
+        return R, S
+
+    def lame(self, E, nu):
+        lam, mu = C.c_float(), C.c_float()
+        self._lib.ref_lame(C.c_float(E), C.c_float(nu), C.byref(lam), C.byref(mu))
+        return lam.value, mu.value
