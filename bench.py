@@ -1,0 +1,272 @@
+#!/usr/bin/env python3
+"""Benchmark of the differentiable MPM stepper on one MI355X.
+
+Metric: forward+backward particle-steps/s at 1M particles on a 128^3 grid,
+% of HBM peak, gradient error versus FP64.
+
+Workload (the precision-stress configuration): 1,048,576 particles in a
+sphere of radius 0.3 (jittered lattice), 128^3 cells (129^3 nodes,
+dx = 1/128), E = 5e4, nu = 0.45, rho = 1e3, dt = 1e-4, an 8-mode sine
+velocity field of amplitude 0.5, 300 steps, plain physics (quadratic
+B-spline, Neo-Hookean, sticky walls at 3 cells).  A "step" of this benchmark
+is one episode: 300 forward steps plus the 300-step adjoint back to layer 0
+(with checkpoint recomputation), seeded at the last layer with a Gaussian
+dL/dx (sigma 1e-3) and dL/dF (sigma 1e-4).  value = particles * 300 * K /
+wall time of K episodes.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "shape-morphing-binder_amd"))
+
+METRIC = "fwd+bwd particle-steps/s at 1M pts,128³ grid; % HBM peak; grad err vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per launch, FP32 state (4-byte elements): per particle and
+# per active grid node (a node any particle's stencil touches).
+#   p2g         reads x, v, F, C, m, vol, lam, mu (28 floats); writes m, p per node
+#   g2p         reads x, F (12 floats) and m, p per node; writes x, v, F, C (24 floats)
+#   adj_scatter reads x, F (12) and the adjoint carry (24), m, p per node; writes dL/dv per node
+#   adj_gather  reads x, v, F, C, m, vol, lam, mu (28) and the carry (24), m, p, dL/dv per node;
+#               writes the carry (24)
+BYTES = {
+    "p2g": (28 * 4, 16),
+    "g2p": (36 * 4, 16),
+    "adj_scatter": (36 * 4, 28),
+    "adj_gather": (76 * 4, 28),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(sc, seed_gx, steps: int = 2):
+    """The reference's own OpenMP C++ path (compiled from its sources into
+    oracle/_ref) on a bounded sample of the same workload: all particles,
+    `steps` forward steps and `steps` adjoint steps, its own physics
+    (cubic B-spline, fixed-corotated).  Falls back to the oracle's port when
+    the reference library was not built."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle, Ref, plain_params, reference_params
+    threads = cpu_threads()
+    s = {k: np.asarray(v, np.float32) for k, v in sc.state().items()}
+    if Ref.available():
+        ref = Ref(threads)
+        prm = reference_params(dims=sc.dims, dx=sc.dx, dt=1e-4, drag=0.0, smoothing=0.0)
+        t0 = time.perf_counter()
+        fwd, bwd = ref.time_steps(prm, s, steps, seed_gx.astype(np.float32))
+        log(f"[cpu] reference: {steps} steps fwd {fwd:.2f}s bwd {bwd:.2f}s ({time.perf_counter() - t0:.1f}s total)")
+        return {"value": sc.n * steps / (fwd + bwd), "unit": "particle-steps/s", "cores": threads,
+                "kind": "reference",
+                "sample": f"{sc.n} particles, 128^3 grid, {steps} forward + {steps} adjoint steps of the "
+                          f"reference's ForwardTimeStep/Back_Timestep (its physics: cubic B-spline, "
+                          f"fixed-corotated), {threads} OpenMP threads",
+                "fwd_s": fwd, "bwd_s": bwd}
+    n = min(sc.n, 65536)
+    sub = {k: v[:n] for k, v in s.items()}
+    orc = Oracle("f32")
+    prm = plain_params(dims=sc.dims, dx=sc.dx, dt=1e-4)
+    t0 = time.perf_counter()
+    layers = orc.rollout(prm, dict(sub, dFc=None), steps)
+    gT = dict(x=seed_gx[:n], v=np.zeros((n, 3)), F=np.zeros((n, 3, 3)), C=np.zeros((n, 3, 3)))
+    orc.rollout_grad(prm, layers, gT)
+    dt = time.perf_counter() - t0
+    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} particles, {steps} fwd+bwd steps of the oracle (plain physics)"}
+
+
+def active_nodes(sc) -> int:
+    from dmpm import p2g_mass
+    gm = p2g_mass(sc.dims, sc.dx, sc.grid_min, sc.x, sc.m, order=2)
+    return int((gm > 0).sum())
+
+
+def episode_sim(sc, prec, K, T, seed_gx, seed_gF, physics=None):
+    from dmpm import Physics, Simulator
+    ph = physics or Physics.plain(dt=1e-4)
+    sim = Simulator(sc.dims, sc.dx, sc.grid_min, physics=ph, precision=prec, checkpoint_interval=K)
+    sim.set_state(sc.state())
+    sim.set_up(T + 1)
+    sim.set_last_grad(x=seed_gx, F=seed_gF)
+    return sim
+
+
+def run_episodes(sim, count):
+    for _ in range(count):
+        sim.forward(0, keep_controls=True)
+        sim.backward(0)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed episodes")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed episodes")
+    ap.add_argument("--particles", type=int, default=1048576)
+    ap.add_argument("--cells", type=int, default=128)
+    ap.add_argument("--episode-steps", type=int, default=300)
+    ap.add_argument("--checkpoint", type=int, default=25, help="checkpoint interval K")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-reference-physics", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(value is the one GPU's throughput).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch  # noqa: F401  (one HIP runtime for torch and libdmpm)
+    from dmpm import Physics
+    from dmpm.scenes import config3
+
+    T, K = args.episode_steps, args.checkpoint
+    t0 = time.perf_counter()
+    sc = config3(n=args.particles, cells=args.cells)
+    log(f"[bench] scene: {sc.n} particles, {sc.dims} nodes, h={sc.h:.5f} ({time.perf_counter() - t0:.1f}s)")
+    rng = np.random.default_rng(3)
+    seed_gx = 1e-3 * rng.normal(size=(sc.n, 3))
+    seed_gF = 1e-4 * rng.normal(size=(sc.n, 3, 3))
+    gact = active_nodes(sc)
+
+    # ---- FP32 (FP64 det/logJ/F^-T), checkpointed: the measured configuration
+    sim = episode_sim(sc, "f32", K, T, seed_gx, seed_gF)
+    run_episodes(sim, args.warmup)
+    sim.synchronize()
+    sim.kernel_timer(args.steps * 12 * T + 64)
+    t0 = time.perf_counter()
+    run_episodes(sim, args.steps)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    kstats = sim.kernel_stats()
+    sim.kernel_timer(0)
+    value = sc.n * T * args.steps / wall
+    ms_per_step = 1e3 * wall / args.steps
+    mem_cur, mem_peak = sim.memory()
+    log(f"[bench] f32 K={K}: {ms_per_step:.1f} ms/episode, {value:.4g} particle-steps/s, peak {mem_peak / 1e9:.2f} GB")
+
+    kernels = {}
+    for name, (ms, cnt) in kstats.items():
+        if cnt == 0:
+            continue
+        avg = ms / cnt
+        entry = {"total_ms": ms, "launches": cnt, "avg_us": 1e3 * avg}
+        if name in BYTES:
+            bp, bn = BYTES[name]
+            b = bp * sc.n + bn * gact
+            entry["bytes_per_launch"] = b
+            entry["GBps"] = b / (avg * 1e-3) / 1e9
+        kernels[name] = entry
+    dom = max((k for k in kernels if k in BYTES), key=lambda k: kernels[k]["total_ms"])
+    d = kernels[dom]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": d["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                "bytes_per_launch": d["bytes_per_launch"], "avg_launch_us": d["avg_us"]}
+
+    # whole-step algorithmic roofline: 532 B per particle-step + 128 B per active node
+    step_bytes = 532 * sc.n + 128 * gact
+    step_frac = step_bytes * (value / sc.n) / (HBM_PEAK_GBS * 1e9)
+
+    out32 = {k: sim.get(T, k, dtype=np.float64) for k in ("x", "F")}
+    g32 = {k: sim.get_grad(0, k, dtype=np.float64) for k in ("x", "v", "F", "C")}
+    sim.close()
+
+    extra = {}
+    if not args.no_fp64:
+        sim64 = episode_sim(sc, "f64", K, T, seed_gx, seed_gF)
+        run_episodes(sim64, 1)
+        sim64.synchronize()
+        t0 = time.perf_counter()
+        run_episodes(sim64, 1)
+        sim64.synchronize()
+        w64 = time.perf_counter() - t0
+        out64 = {k: sim64.get(T, k, dtype=np.float64) for k in ("x", "F")}
+        g64 = {k: sim64.get_grad(0, k, dtype=np.float64) for k in ("x", "v", "F", "C")}
+        sim64.close()
+        dx = np.linalg.norm(out32["x"] - out64["x"], axis=1)
+        dF = np.linalg.norm((out32["F"] - out64["F"]).reshape(sc.n, 9), axis=1)
+        J = np.linalg.det(out64["F"])
+        extra["fp64"] = {"value": sc.n * T / w64, "ms_per_step": 1e3 * w64,
+                         "speedup_fp32_over_fp64": value / (sc.n * T / w64)}
+        extra["errors_vs_fp64"] = {
+            "x_max": float(dx.max()), "x_rms": float(np.sqrt((dx ** 2).mean())),
+            "F_max": float(dF.max()), "F_rms": float(np.sqrt((dF ** 2).mean())),
+            **{f"grad_{k}_rel_l2": float(np.linalg.norm(g32[k] - g64[k]) / max(np.linalg.norm(g64[k]), 1e-300))
+               for k in g32},
+            "detF_min": float(J.min()), "detF_p01": float(np.percentile(J, 1)), "detF_p50": float(np.median(J)),
+        }
+        log(f"[bench] fp64: {1e3 * w64:.1f} ms/episode; errors {extra['errors_vs_fp64']}")
+
+    if not args.no_reference_physics:
+        simr = episode_sim(sc, "f32", K, T, seed_gx, seed_gF,
+                           physics=Physics.reference(dt=1e-4, drag=0.0, smoothing=0.0))
+        run_episodes(simr, 1)
+        simr.synchronize()
+        t0 = time.perf_counter()
+        run_episodes(simr, 1)
+        simr.synchronize()
+        wr = time.perf_counter() - t0
+        simr.close()
+        extra["reference_physics"] = {"value": sc.n * T / wr, "ms_per_step": 1e3 * wr,
+                                      "physics": "cubic B-spline, fixed-corotated, bulk viscosity 0.25, "
+                                                 "trust region 0.6, reference adjoint"}
+        log(f"[bench] reference physics: {1e3 * wr:.1f} ms/episode")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(sc, seed_gx)
+        if "reference_physics" in extra and cpu["kind"] == "reference":
+            extra["reference_physics"]["vs_cpu_reference"] = extra["reference_physics"]["value"] / cpu["value"]
+
+    rec = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "precision-stress sphere: 1,048,576 particles, 128^3 grid, 300-step fwd+bwd episode",
+            "particles": sc.n, "grid_nodes": list(sc.dims), "episode_steps": T, "checkpoint_interval": K,
+            "physics": "plain: quadratic B-spline, Neo-Hookean, sticky walls (3 cells)",
+            "precision": "FP32 state/arithmetic, FP64 det/log J/F^-T", "E": 5e4, "nu": 0.45, "dt": 1e-4,
+            "active_nodes": gact, "peak_device_bytes": mem_peak,
+        },
+        "roofline": roofline,
+        "step_roofline": {"bytes_per_particle_step": step_bytes / sc.n, "achieved_GBps":
+                          step_bytes * value / sc.n / 1e9, "frac": step_frac},
+        "cpu_baseline": cpu,
+        "vs_cpu": (value / cpu["value"]) if cpu else None,
+        "kernels": kernels,
+        **extra,
+    }
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,218 @@
+/* dmpm -- MI355X-native differentiable MPM stepper, C ABI.
+ *
+ * This is the drop-in boundary for the reference's `diffmpm_bindings` module
+ * (bind/bind.cpp:54-559).  Every entry point below replaces one binding (or
+ * the C++ member it forwards to); the comment names it.  The Python module
+ * `diffmpm_bindings` in this repository re-exposes the same surface on top of
+ * these functions.
+ *
+ * Conventions
+ *  - Every function returns DMPM_OK (0) or a negative error code; nothing
+ *    throws across the ABI.  dmpm_last_error() describes the last failure on
+ *    the calling thread.
+ *  - Particle arrays are in the caller's particle order, row-major:
+ *      x, v: (N,3)   F, C, dFc, gradients: (N,3,3), element (r,c) at 9p+3r+c.
+ *    `dtype` says whether they are float32 (DMPM_DTYPE_F32) or float64
+ *    (DMPM_DTYPE_F64); `on_device` says whether the pointer is device memory
+ *    (1) or host memory (0).  Pointers are borrowed for the call only.
+ *  - Grid arrays are dense, node (i,j,k) at i*ny*nz + j*nz + k (Grid.h:89-94).
+ *  - A dmpm_sim owns all its device memory.  Calls on one dmpm_sim must be
+ *    serialised by the caller; distinct sims are independent.
+ *  - Work is queued on the sim's stream; functions that return host values
+ *    (losses, norms, exported host arrays) synchronise that stream.
+ */
+#ifndef DMPM_H
+#define DMPM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMPM_OK 0
+#define DMPM_ERR_ARG -1      /* bad argument (shape, index, null pointer) */
+#define DMPM_ERR_INDEX -2    /* layer index out of range (IndexError)     */
+#define DMPM_ERR_HIP -3      /* HIP runtime error                          */
+#define DMPM_ERR_STATE -4    /* call not valid in the sim's current state  */
+#define DMPM_ERR_NOMEM -5    /* device allocation failed                   */
+
+#define DMPM_DTYPE_F32 0
+#define DMPM_DTYPE_F64 1
+
+#define DMPM_PREC_F32 0  /* FP32 state and arithmetic; FP64 det/log J/F^-T */
+#define DMPM_PREC_F64 1  /* FP64 everywhere */
+
+#define DMPM_ORDER_QUADRATIC 2
+#define DMPM_ORDER_CUBIC 3     /* the reference's B-spline (Interpolation.h:5-30) */
+
+#define DMPM_MODEL_FCR 0       /* fixed-corotated (Elasticity.cpp:45-55) */
+#define DMPM_MODEL_NEOHOOKEAN 1
+
+#define DMPM_ADJOINT_EXACT 0
+#define DMPM_ADJOINT_REFERENCE 1  /* Back_Timestep's formulas verbatim (BackPropagation.cpp:9-144) */
+
+/* Layer fields for dmpm_get_layer / dmpm_get_layer_grad. */
+#define DMPM_FIELD_X 0
+#define DMPM_FIELD_V 1
+#define DMPM_FIELD_F 2
+#define DMPM_FIELD_C 3
+#define DMPM_FIELD_DFC 4      /* control dFc of the layer            */
+#define DMPM_FIELD_FTOTAL 5   /* F + dFc                             */
+#define DMPM_FIELD_M 6
+#define DMPM_FIELD_VOL 7
+#define DMPM_FIELD_LAM 8
+#define DMPM_FIELD_MU 9
+
+typedef struct dmpm_sim dmpm_sim;
+
+/* Simulation configuration.  Replaces the grid/material part of OptInput
+ * (bind.cpp:30-52) plus the constants the reference hard-codes
+ * (ForwardSimulation.cpp:13: zeta = 0.25, J_min = 0.6).  dt, drag,
+ * smoothing and f_ext start at CompGraph's defaults (CompGraph.h:80-83) when
+ * created through dmpm_config_init. */
+typedef struct {
+    int dims[3];            /* grid nodes per axis (Grid(nx,ny,nz,...), bind.cpp:159-160) */
+    double dx;
+    double grid_min[3];
+    double dt, drag, smoothing;
+    double f_ext[3];
+    double zeta;            /* bulk viscosity coefficient */
+    double jmin;            /* trust-region J_min */
+    int order;              /* DMPM_ORDER_* */
+    int model;              /* DMPM_MODEL_* */
+    int trust_region;       /* 1: J trust region of P_op_2 on */
+    int wall_cells;         /* sticky walls: v = 0 on nodes within this many cells of a face (0: off) */
+    int adjoint;            /* DMPM_ADJOINT_* */
+    int precision;          /* DMPM_PREC_* */
+    int checkpoint_interval;/* K: store every K-th layer and recompute the rest (1: store all) */
+    int lds_bytes;          /* LDS budget per block for the grid window (0: default) */
+    int block;              /* threads per block (0: default 256) */
+} dmpm_config;
+
+/* Fill `cfg` with the reference's physics (cubic, fixed-corotated, zeta
+ * 0.25, J_min 0.6, trust region on, reference adjoint) and CompGraph's
+ * default dt = 1/120, drag = 0.5, smoothing = 0.1 (CompGraph.h:80-83). */
+void dmpm_config_init(dmpm_config* cfg, int nx, int ny, int nz, double dx, const double grid_min[3]);
+
+const char* dmpm_last_error(void);
+const char* dmpm_version(void);
+
+/* --- lifetime -------------------------------------------------------------- */
+/* CompGraph(pc, grid, target_grid) (bind.cpp:164, CompGraph.cpp:21-28) is
+ * dmpm_create + dmpm_set_particles + dmpm_set_target_grid_mass. */
+int dmpm_create(const dmpm_config* cfg, dmpm_sim** out);
+int dmpm_destroy(dmpm_sim* sim);
+/* Use `stream` (a hipStream_t) for all work; 0 = the library's own stream. */
+int dmpm_set_stream(dmpm_sim* sim, void* stream);
+int dmpm_synchronize(dmpm_sim* sim);
+
+/* Layer 0 particles (the PointCloud handed to CompGraph).  dFc may be NULL.
+ * Particles are re-ordered internally (Morton order of their cell); every
+ * export returns them in the caller's order. */
+int dmpm_set_particles(dmpm_sim* sim, int n, const void* x, const void* v, const void* F, const void* C,
+                       const void* dFc, const void* m, const void* vol, const void* lam, const void* mu,
+                       int dtype, int on_device);
+int dmpm_num_particles(const dmpm_sim* sim, int* n);
+/* Target node masses (the target Grid after calculate_point_cloud_volumes). */
+int dmpm_set_target_grid_mass(dmpm_sim* sim, const void* m, int dtype, int on_device);
+
+/* CompGraph's private simulation parameters, set by run_optimization
+ * (CompGraph.cpp:220-224). */
+int dmpm_set_sim_params(dmpm_sim* sim, double dt, double drag, double smoothing, const double f_ext[3]);
+
+/* --- CompGraph ------------------------------------------------------------- */
+/* set_up_comp_graph(num_layers) (bind.cpp:345-347, CompGraph.cpp:30-40):
+ * every layer becomes a copy of layer 0. */
+int dmpm_set_up_comp_graph(dmpm_sim* sim, int num_layers);
+/* get_num_layers() (bind.cpp:173-175). */
+int dmpm_get_num_layers(const dmpm_sim* sim, int* num_layers);
+/* compute_forward_pass(start_layer, current_episode) (bind.cpp:349-353,
+ * CompGraph.cpp:131-141). */
+int dmpm_compute_forward_pass(dmpm_sim* sim, int start_layer);
+/* Forward pass that keeps every layer's dFc: a rollout driven by per-step
+ * controls (the configs' control forces).  No counterpart in the reference,
+ * whose forward pass zeroes dFc of every layer it writes
+ * (ForwardSimulation.cpp:156). */
+int dmpm_rollout_forward(dmpm_sim* sim, int start_layer);
+/* end_layer_mass_loss() (bind.cpp:183-186, CompGraph.cpp:42-129). */
+int dmpm_end_layer_mass_loss(dmpm_sim* sim, double* loss);
+/* compute_backward_pass(control_layer) (bind.cpp:355-358,
+ * CompGraph.cpp:143-211), including the stored render-gradient injection. */
+int dmpm_compute_backward_pass(dmpm_sim* sim, int control_layer);
+/* Read a field of layer `layer` (get_point_cloud(i).get_positions() etc.,
+ * bind.cpp:84-157).  `out` has n*ncomp elements of `dtype`. */
+int dmpm_get_layer(dmpm_sim* sim, int layer, int field, void* out, int dtype, int on_device);
+/* Read dL/dx, dL/dv, dL/dF or dL/dC (field X, V, F, C) of the layer the
+ * last backward pass stopped at (the control layer), or of the last layer.
+ * DMPM_FIELD_DFC reads dL/d dFc of the control layer. */
+int dmpm_get_layer_grad(dmpm_sim* sim, int layer, int field, void* out, int dtype, int on_device);
+/* accumulate_render_grads(dLdF, dLdx) (bind.cpp:189-279): add to the last
+ * layer's adjoint now.  Either pointer may be NULL. */
+int dmpm_accumulate_last_layer_grad(dmpm_sim* sim, const void* dLdF, const void* dLdx, const void* dLdv,
+                                    const void* dLdC, int dtype, int on_device);
+/* Replace the last layer's adjoint (seed for an autograd backward). */
+int dmpm_set_last_layer_grad(dmpm_sim* sim, const void* dLdx, const void* dLdv, const void* dLdF,
+                             const void* dLdC, int dtype, int on_device);
+/* set_render_gradients / clear_render_gradients / has_render_gradients /
+ * get_render_gradient_info (bind.cpp:359-528). */
+int dmpm_set_render_gradients(dmpm_sim* sim, int n, const void* dLdF, const void* dLdx, int dtype, int on_device);
+int dmpm_clear_render_gradients(dmpm_sim* sim);
+int dmpm_render_gradient_info(const dmpm_sim* sim, int* has, int* num_points, double* norm_F, double* norm_x);
+/* promote_last_as_initial(carry_grid) (bind.cpp:282-291). */
+int dmpm_promote_last_as_initial(dmpm_sim* sim, int carry_grid);
+
+/* --- controls and optimisation ------------------------------------------- */
+/* Set/read dFc of a layer (the per-step control).  NULL clears it. */
+int dmpm_set_control(dmpm_sim* sim, int layer, const void* dFc, int dtype, int on_device);
+/* Keep dL/d dFc of `layer` from the next backward passes (1) or not (0);
+ * read it with dmpm_get_control_grad. */
+int dmpm_request_control_grad(dmpm_sim* sim, int layer, int on);
+int dmpm_get_control_grad(dmpm_sim* sim, int layer, void* out, int dtype, int on_device);
+/* PointCloud::Compute_dLdF_Norm of the control layer's gradient (PointCloud.cpp:24-35). */
+int dmpm_control_grad_norm(dmpm_sim* sim, double* norm);
+/* PointCloud::Descend_Adam on layer `layer` (PointCloud.cpp:83-147), using
+ * the control gradient of the last backward pass. */
+int dmpm_descend_adam(dmpm_sim* sim, int layer, double alpha, double grad_norm, double beta1, double beta2,
+                      double eps, int timestep);
+/* optimize_single_timestep(...) (bind.cpp:294-317, CompGraph.cpp:369-465). */
+int dmpm_optimize_single_timestep(dmpm_sim* sim, int timestep_idx, int max_gd_iters, int current_episode,
+                                  double initial_alpha, int max_line_search_iters);
+/* run_optimization(opt) (bind.cpp:165-172, CompGraph.cpp:213-368). */
+int dmpm_run_optimization(dmpm_sim* sim, int num_steps, double dt, double drag, const double f_ext[3],
+                          int control_stride, int max_gd_iters, int max_line_search_iters, double initial_alpha,
+                          double gd_tol, double smoothing, int current_episodes, double* final_loss);
+
+/* --- free functions (bind.cpp:531-559) ------------------------------------- */
+/* p2g(pc, grid): node masses of a dt = 0 P2G (bind.cpp:546-549). */
+int dmpm_p2g_mass(const dmpm_config* cfg, int n, const void* x, const void* m, void* grid_m, int dtype,
+                  int on_device);
+/* calculate_point_cloud_volumes(pc, grid) (bind.cpp:551-554,
+ * ForwardSimulation.cpp:234-271): writes vol and the grid's node masses. */
+int dmpm_calculate_volumes(const dmpm_config* cfg, int n, const void* x, const void* m, void* vol, void* grid_m,
+                           int dtype, int on_device);
+/* calculate_lame_parameters(E, nu) (bind.cpp:540-544, Elasticity.cpp:149-153). */
+int dmpm_lame_parameters(double E, double nu, double* lam, double* mu);
+
+/* --- measurement ----------------------------------------------------------- */
+/* Time of the last forward/backward pass in ms (HIP events), and the number
+ * of particle steps it covered. */
+int dmpm_last_timing(const dmpm_sim* sim, double* fwd_ms, double* bwd_ms);
+/* Per-kernel timing with HIP events on the sim's stream.  capacity > 0
+ * starts recording (up to `capacity` launches, events pre-created here);
+ * 0 stops.  Kinds: DMPM_KERNEL_*.  dmpm_kernel_stats synchronises and
+ * returns the summed duration and the launch count of one kind. */
+#define DMPM_KERNEL_P2G 0
+#define DMPM_KERNEL_G2P 1
+#define DMPM_KERNEL_ADJ_SCATTER 2
+#define DMPM_KERNEL_ADJ_GATHER 3
+#define DMPM_KERNEL_GRID_CLEAR 4
+int dmpm_kernel_timer(dmpm_sim* sim, int capacity);
+int dmpm_kernel_stats(dmpm_sim* sim, int kind, double* total_ms, int* count);
+/* Device bytes currently allocated by the sim, and the peak. */
+int dmpm_memory(const dmpm_sim* sim, size_t* current, size_t* peak);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -576,8 +576,8 @@ void backward_step(const orc_params* prm, const State<T>& s, const T* gx1, const
             for (int k = 0; k < cx.nz; ++k) {
                 size_t id = cx.id(i, j, k);
                 T mi = g.m[id];
-                if (!(std::fabs(mi) > T(1e-12))) continue;
-                if (!ref && cx.wall(i, j, k)) continue;
+                if (!(mi > T(1e-12))) continue;
+                if (cx.wall(i, j, k)) continue;  // wall nodes: v = 0, no adjoint
                 V3<T> vi{{g.v[3 * id], g.v[3 * id + 1], g.v[3 * id + 2]}};
                 if (!ref) for (int a = 0; a < 3; ++a) vi[a] -= dt * (T)prm->f_ext[a];
                 T vd = vi[0] * gvg[3 * id] + vi[1] * gvg[3 * id + 1] + vi[2] * gvg[3 * id + 2];

@@ -443,6 +443,20 @@ class Ref:
         self._lib.ref_polar(F.shape[0], _ptr(F), _ptr(R), _ptr(S))
         return R, S
 
+    def time_steps(self, prm: Params, s: dict, steps: int, seed_gx) -> tuple:
+        """Seconds spent in `steps` ForwardTimeStep calls and `steps`
+        Back_Timestep calls of the reference (its own physics)."""
+        n = s["x"].shape[0]
+        dims, mn = self._grid(prm)
+        args = self._state(s)
+        del args[4]  # no dFc
+        fe = self._f3(prm.f_ext)
+        f, b = C.c_double(), C.c_double()
+        self._lib.ref_time_steps(n, *[_ptr(a) for a in args], dims, C.c_float(prm.dx), _ptr(mn), int(steps),
+                                 C.c_float(prm.smoothing), C.c_float(prm.dt), C.c_float(prm.drag), _ptr(fe),
+                                 _ptr(self._f3(seed_gx)), C.byref(f), C.byref(b))
+        return f.value, b.value
+
     def lame(self, E, nu):
         lam, mu = C.c_float(), C.c_float()
         self._lib.ref_lame(C.c_float(E), C.c_float(nu), C.byref(lam), C.byref(mu))

@@ -207,6 +207,43 @@ float ref_episode(int n, const float* x, const float* v, const float* F,
     return loss;
 }
 
+// Timed baseline: `steps` ForwardTimeStep calls, then `steps` Back_Timestep
+// calls (seeded with dL/dx of the last layer), on the reference's own layers,
+// with OpenMP threads as set by ref_set_threads.  Only the two loops are
+// timed (omp_get_wtime); fwd_s and bwd_s receive their durations.
+void ref_time_steps(int n, const float* x, const float* v, const float* F,
+                    const float* C, const float* m, const float* vol,
+                    const float* lam, const float* mu, const int* dims,
+                    float dx, const float* grid_min, int steps,
+                    float smoothing, float dt, float drag, const float* f_ext,
+                    const float* seed_gx, double* fwd_s, double* bwd_s) {
+    RefParticles s{n, x, v, F, C, nullptr, m, vol, lam, mu};
+    std::vector<CompGraphLayer> layers(steps + 1);
+    layers[0].point_cloud = make_pc(s);
+    layers[0].grid = make_grid(dims, dx, grid_min);
+    for (int i = 1; i <= steps; ++i) {
+        layers[i].point_cloud = std::make_shared<PointCloud>(*layers[0].point_cloud);
+        layers[i].grid = std::make_shared<Grid>(*layers[0].grid);
+    }
+    double t0 = omp_get_wtime();
+    for (int i = 0; i < steps; ++i)
+        SingleThreadMPM::ForwardTimeStep(*layers[i + 1].point_cloud, *layers[i].point_cloud,
+                                         *layers[i].grid, smoothing, dt, drag, load_v3(f_ext), 0);
+    double t1 = omp_get_wtime();
+    PointCloud& last = *layers[steps].point_cloud;
+    last.ResetGradients();
+    for (int p = 0; p < n; ++p) last.points[p].dLdx = load_v3(seed_gx + 3 * p);
+    double t2 = omp_get_wtime();
+    for (int i = steps - 1; i >= 0; --i) {
+        layers[i].grid->ResetGradients();
+        layers[i].point_cloud->ResetGradients();
+        Back_Timestep(layers[i + 1], layers[i], drag, dt, smoothing);
+    }
+    double t3 = omp_get_wtime();
+    *fwd_s = t1 - t0;
+    *bwd_s = t3 - t2;
+}
+
 // EndLayerMassLoss (CompGraph.cpp:42-129) on a single layer: returns the loss,
 // writes the particles' dLdx and the grid's per-node mass and dLdm.
 float ref_mass_loss(int n, const float* x, const float* v, const float* F,

@@ -1,0 +1,103 @@
+// Host-side launchers for the MPM kernels (dmpm_kernels.hip).
+//
+// Device data layout (struct-of-arrays, "planes"):
+//   layer state  : 24 planes of `ld` elements: x0..2, v0..2, F00..F22, C00..C22
+//   statics      : 4 planes: m, vol, lam, mu
+//   control dFc  : 9 planes
+//   adjoint carry: 24 planes, same order as the state (dL/dx, dL/dv, dL/dF, dL/dC)
+//   grid         : node (i,j,k) at i*ny*nz + j*nz + k (the reference's order,
+//                  Grid.h:89-94); arrays of `gld` elements: m, p0..2 (forward),
+//                  gv0..2 (adjoint dL/dv_i).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace dmpm {
+
+enum Prec { PREC_F32 = 0, PREC_F64 = 1 };
+
+// Plane offsets inside a layer state / adjoint carry.
+enum { PX = 0, PV = 3, PF = 6, PC = 15, NSTATE = 24 };
+enum { SM = 0, SVOL = 1, SLAM = 2, SMU = 3, NSTAT = 4 };
+
+struct Phys {
+    int dims[3];
+    double dx, grid_min[3];
+    double dt, drag, smoothing, zeta, jmin, f_ext[3];
+    int order, model, trust_region, wall_cells, adjoint;
+};
+
+// Everything one kernel launch may need; pointers are device pointers of the
+// precision in use (float or double), passed as void*.
+struct Launch {
+    Phys ph;
+    int n;          // particles
+    long long ld;   // particle plane stride (elements)
+    long long gld;  // grid plane stride (elements)
+    const void* st;    // layer state (24 planes)
+    const void* stat;  // statics (4 planes)
+    const void* ctl;   // dFc (9 planes) or null
+    void* out;         // next layer state (24 planes)
+    void* grid;        // m, p0, p1, p2 (4 grid planes)
+    void* ggv;         // dL/dv_i (3 grid planes)
+    const void* adj1;  // adjoint carry of layer t+1
+    void* adj0;        // adjoint carry of layer t (output)
+    void* gctl;        // dL/d dFc of layer t (output, 9 planes) or null
+    int lds_bytes;     // LDS budget per block for the window
+    int block;         // threads per block (multiple of 64)
+};
+
+// Forward: P2G of layer `st` (stress computed inside) into `grid`.
+// mass_only: scatter m only (the dt = 0 P2G used by the loss and volumes).
+void p2g(Prec pr, const Launch& L, bool mass_only, hipStream_t s);
+// Forward: G2P + particle update, `st` + grid -> `out`.
+void g2p(Prec pr, const Launch& L, hipStream_t s);
+// Adjoint phases 1+2: scatter dL/dv_i into ggv.
+void adj_scatter(Prec pr, const Launch& L, hipStream_t s);
+// Adjoint phases 3+4 (+ the stress VJP): writes adj0 and gctl.
+void adj_gather(Prec pr, const Launch& L, hipStream_t s);
+
+// Fill `n` elements of `p` with zero.
+void fill_zero(Prec pr, void* p, long long n, hipStream_t s);
+
+// AoS user arrays <-> internal planes with permutation:
+//   import: dst[c*ld + i] = src[perm[i]*nc + c]   (src float or double)
+//   export: dst[perm[i]*nc + c] = src[c*ld + i]
+void import_planes(Prec pr, const void* src, int src_f64, void* dst, const int* perm, int n, int nc,
+                   long long ld, hipStream_t s);
+void export_planes(Prec pr, const void* src, void* dst, int dst_f64, const int* perm, int n, int nc,
+                   long long ld, hipStream_t s);
+// export of F + dFc
+void export_F_total(Prec pr, const void* st, const void* ctl, void* dst, int dst_f64, const int* perm, int n,
+                    long long ld, hipStream_t s);
+// Copy between same-layout plane buffers (`count` elements).
+void copy_elems(Prec pr, void* dst, const void* src, long long count, hipStream_t s);
+
+// EndLayerMassLoss node pass: per-block partial sums of (c, t) -> partial[2*b]
+void mass_sums(Prec pr, const void* grid_m, const void* target_m, long long G, double* partial, int nblocks,
+               hipStream_t s);
+// node pass 2: dLdm (written into `dldm`) and per-block loss partials
+void mass_loss_nodes(Prec pr, const void* grid_m, const void* target_m, long long G, const double* sums,
+                     int nsum_blocks, void* dldm, double* partial, int nblocks, hipStream_t s);
+// particle pass: writes the last layer's adjoint carry (dL/dx only, rest 0)
+void mass_loss_particles(Prec pr, const Launch& L, const void* dldm, const void* target_m, hipStream_t s);
+// deterministic final reduction of `nblocks` partials (stride `stride`) into out[0..stride)
+void reduce_partials(const double* partial, int nblocks, int stride, double* out, hipStream_t s);
+
+// CalculatePointCloudVolumes particle pass: vol_p = m_p dx^3 / sum_i w m_i
+void volumes(Prec pr, const Launch& L, void* vol_out, hipStream_t s);
+
+// Squared Frobenius norm partials of 9 planes (dL/dF): partial[b]
+void sqnorm9(Prec pr, const void* planes, int n, long long ld, double* partial, int nblocks, hipStream_t s);
+// Descend_Adam element update on 9-plane arrays.
+void adam(Prec pr, const void* dLdF, void* dFc, void* mom, void* vec, void* vmax, int n, long long ld,
+          const double* sqnorm, double alpha, double beta1, double beta2, double eps, int timestep,
+          hipStream_t s);
+
+// Accumulate `src` user AoS gradient (float or double; host data must be
+// staged to device first) into carry planes: carry[off+c] += src (permuted).
+void accumulate_carry(Prec pr, const void* src, int src_f64, void* carry, int plane_off, int nc, const int* perm,
+                      int n, long long ld, hipStream_t s);
+
+}  // namespace dmpm

@@ -1,0 +1,1236 @@
+// MPM step kernels for gfx950 (MI355X).
+//
+// One thread per particle, 256-thread blocks (4 wave64s).  Particles are kept
+// in Morton order of their grid cell, so a block's particles occupy a compact
+// box of the grid.  Every particle<->grid kernel first reduces that box (the
+// "window": the bounding box of the block's B-spline stencils) with wave
+// shuffles, then
+//   * scatters (P2G, adjoint of G2P) accumulate into the window in LDS with
+//     ds_add and flush it to HBM with one global atomic per touched node
+//     value, instead of one per particle-node pair;
+//   * gathers (G2P, adjoint of P2G) stage the window's node values in LDS
+//     once and read the stencil from there.
+// When a block's window does not fit the LDS budget (scattered particles),
+// the block falls back to direct global atomics / global gathers, so the
+// result never depends on the particle order, only the speed does.
+//
+// Grid velocity is never stored: v_i = p_i/m_i + dt f_ext (0 when m_i <= 1e-12
+// or on a wall node) is recomputed from (m, p) wherever it is read, which is
+// the reference's grid update (ForwardSimulation.cpp:82-87, 165-170) fused
+// into its readers.
+#include <climits>
+
+#include "dmpm_kernels.h"
+#include "dmpm_math.h"
+
+namespace dmpm {
+
+template <typename T>
+struct KP {
+    int n;
+    long long ld, gld;
+    int nx, ny, nz;
+    T dx, mn0, mn1, mn2;
+    T dt, drag, sm, zeta, jmin, f0, f1, f2;
+    int trust, walls;
+    const T* st;
+    const T* stat;
+    const T* ctl;
+    T* out;
+    T* gm;
+    T* gp;   // 3 planes after gm: gp + a*gld
+    T* ggv;  // 3 planes
+    const T* adj1;
+    T* adj0;
+    T* gctl;
+    int cap;
+};
+
+template <typename T>
+static KP<T> make_kp(const Launch& L, int narrays) {
+    KP<T> k;
+    k.n = L.n;
+    k.ld = L.ld;
+    k.gld = L.gld;
+    k.nx = L.ph.dims[0];
+    k.ny = L.ph.dims[1];
+    k.nz = L.ph.dims[2];
+    k.dx = (T)L.ph.dx;
+    k.mn0 = (T)L.ph.grid_min[0];
+    k.mn1 = (T)L.ph.grid_min[1];
+    k.mn2 = (T)L.ph.grid_min[2];
+    k.dt = (T)L.ph.dt;
+    k.drag = (T)L.ph.drag;
+    k.sm = (T)L.ph.smoothing;
+    k.zeta = (T)L.ph.zeta;
+    k.jmin = (T)L.ph.jmin;
+    k.f0 = (T)L.ph.f_ext[0];
+    k.f1 = (T)L.ph.f_ext[1];
+    k.f2 = (T)L.ph.f_ext[2];
+    k.trust = L.ph.trust_region;
+    k.walls = L.ph.wall_cells;
+    k.st = (const T*)L.st;
+    k.stat = (const T*)L.stat;
+    k.ctl = (const T*)L.ctl;
+    k.out = (T*)L.out;
+    k.gm = (T*)L.grid;
+    k.gp = L.grid ? (T*)L.grid + L.gld : nullptr;
+    k.ggv = (T*)L.ggv;
+    k.adj1 = (const T*)L.adj1;
+    k.adj0 = (T*)L.adj0;
+    k.gctl = (T*)L.gctl;
+    k.cap = narrays > 0 ? (int)(L.lds_bytes / (narrays * (int)sizeof(T))) : 0;
+    return k;
+}
+
+// Blocks are handed out round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Remap so that each XCD gets a contiguous run of logical blocks:
+// spatially adjacent particle blocks then share that XCD's L2, which holds
+// the halo nodes their windows overlap on.  The launch grid is padded to a
+// multiple of 8; logical blocks past the end have no particles.
+DMPM_DI int logical_block() {
+    const int nb = gridDim.x;
+    const int per = nb >> 3;
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+static inline unsigned grid_blocks(int n, int block) {
+    unsigned nb = (unsigned)((n + block - 1) / block);
+    if (nb == 0) nb = 1;
+    return (nb + 7u) & ~7u;
+}
+
+struct Win {
+    int lo0, lo1, lo2, e1, e2, vol;
+    bool lds;
+    bool any;
+};
+
+// Bounding box of the block's stencils (base nodes b, width KS).
+template <int KS>
+DMPM_DI Win block_window(bool valid, int b0, int b1, int b2, int cap) {
+    __shared__ int s_red[6][16];
+    int v0 = valid ? b0 : INT_MAX, v1 = valid ? b1 : INT_MAX, v2 = valid ? b2 : INT_MAX;
+    int w0 = valid ? b0 : INT_MIN, w1 = valid ? b1 : INT_MIN, w2 = valid ? b2 : INT_MIN;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v0 = min(v0, __shfl_xor(v0, off));
+        v1 = min(v1, __shfl_xor(v1, off));
+        v2 = min(v2, __shfl_xor(v2, off));
+        w0 = max(w0, __shfl_xor(w0, off));
+        w1 = max(w1, __shfl_xor(w1, off));
+        w2 = max(w2, __shfl_xor(w2, off));
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        s_red[0][wave] = v0; s_red[1][wave] = v1; s_red[2][wave] = v2;
+        s_red[3][wave] = w0; s_red[4][wave] = w1; s_red[5][wave] = w2;
+    }
+    __syncthreads();
+    const int nw = blockDim.x >> 6;
+    v0 = s_red[0][0]; v1 = s_red[1][0]; v2 = s_red[2][0];
+    w0 = s_red[3][0]; w1 = s_red[4][0]; w2 = s_red[5][0];
+    for (int q = 1; q < nw; ++q) {
+        v0 = min(v0, s_red[0][q]); v1 = min(v1, s_red[1][q]); v2 = min(v2, s_red[2][q]);
+        w0 = max(w0, s_red[3][q]); w1 = max(w1, s_red[4][q]); w2 = max(w2, s_red[5][q]);
+    }
+    Win w;
+    w.any = v0 != INT_MAX;
+    w.lo0 = v0; w.lo1 = v1; w.lo2 = v2;
+    if (!w.any) {
+        w.e1 = w.e2 = 0; w.vol = 0; w.lds = true;
+        return w;
+    }
+    long long e0 = (long long)w0 - v0 + KS, e1 = (long long)w1 - v1 + KS, e2 = (long long)w2 - v2 + KS;
+    long long vol = e0 * e1 * e2;
+    w.e1 = (int)e1;
+    w.e2 = (int)e2;
+    w.lds = vol <= (long long)cap;
+    w.vol = w.lds ? (int)vol : 0;
+    return w;
+}
+
+template <typename T>
+DMPM_DI bool is_wall(const KP<T>& a, int i, int j, int k) {
+    const int w = a.walls;
+    return w > 0 && (i < w || j < w || k < w || i >= a.nx - w || j >= a.ny - w || k >= a.nz - w);
+}
+
+template <typename T>
+DMPM_DI bool in_grid(const KP<T>& a, int i, int j, int k) {
+    return (unsigned)i < (unsigned)a.nx && (unsigned)j < (unsigned)a.ny && (unsigned)k < (unsigned)a.nz;
+}
+
+template <typename T>
+DMPM_DI long long gindex(const KP<T>& a, int i, int j, int k) {
+    return ((long long)i * a.ny + j) * a.nz + k;
+}
+
+// Grid velocity update (ForwardSimulation.cpp:82-87) plus sticky walls.
+template <typename T>
+DMPM_DI void node_velocity(const KP<T>& a, long long g, int i, int j, int k, T& v0, T& v1, T& v2) {
+    const T m = a.gm[g];
+    if (m > T(1e-12) && !is_wall(a, i, j, k)) {
+        const T inv = T(1) / m;
+        v0 = a.gp[g] * inv + a.dt * a.f0;
+        v1 = a.gp[a.gld + g] * inv + a.dt * a.f1;
+        v2 = a.gp[2 * a.gld + g] * inv + a.dt * a.f2;
+    } else {
+        v0 = v1 = v2 = T(0);
+    }
+}
+
+template <typename T>
+DMPM_DI m3<T> load_m3(const T* base, long long ld, int plane, int i) {
+    m3<T> M;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) M.a[q] = base[(plane + q) * ld + i];
+    return M;
+}
+template <typename T>
+DMPM_DI v3<T> load_v3(const T* base, long long ld, int plane, int i) {
+    v3<T> v;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) v.a[q] = base[(plane + q) * ld + i];
+    return v;
+}
+template <typename T>
+DMPM_DI void store_m3(T* base, long long ld, int plane, int i, const m3<T>& M) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) base[(plane + q) * ld + i] = M.a[q];
+}
+template <typename T>
+DMPM_DI void store_v3(T* base, long long ld, int plane, int i, const v3<T>& v) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) base[(plane + q) * ld + i] = v.a[q];
+}
+
+// Per-particle stencil: base node and per-axis weights.
+template <typename T, int ORDER>
+struct Sten {
+    static constexpr int KS = Spline<ORDER>::K;
+    int b[3];
+    T u[3][KS], w[3][KS], dw[3][KS];
+    DMPM_DI void init(const KP<T>& a, const v3<T>& x) {
+        const T inv = T(1) / a.dx;
+        b[0] = Spline<ORDER>::weights((x.a[0] - a.mn0) * inv, u[0], w[0], dw[0]);
+        b[1] = Spline<ORDER>::weights((x.a[1] - a.mn1) * inv, u[1], w[1], dw[1]);
+        b[2] = Spline<ORDER>::weights((x.a[2] - a.mn2) * inv, u[2], w[2], dw[2]);
+    }
+};
+
+// Trust region of P_op_2 (ForwardSimulation.cpp:136-152).
+template <typename T>
+struct Trust {
+    T s;        // scale applied to C
+    int state;  // 0 inactive, 1 scaled, 2 clamped to zero
+    T denom, J0;
+};
+
+template <typename T>
+DMPM_DI Trust<T> trust_region(const KP<T>& a, const m3<T>& Ft, const m3<T>& Craw) {
+    Trust<T> t;
+    t.s = T(1);
+    t.state = 0;
+    t.denom = T(0);
+    t.J0 = (T)fmax(mdet(mcast<double>(Ft)), 1e-6);
+    if (a.trust) {
+        const T trC = mtrace(Craw);
+        if (trC < T(0)) {
+            const T denom = fmin(T(-1e-6), a.dt * trC);
+            T sv = (a.jmin / t.J0 - T(1)) / denom;
+            if (sv < T(1)) {
+                t.state = sv < T(0) ? 2 : 1;
+                t.s = fmin(fmax(sv, T(0)), T(1));
+                t.denom = denom;
+            }
+        }
+    }
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// P2G (ForwardSimulation.cpp:26-80) with the stress of P_op_1 (:16-24).
+template <typename T, int ORDER, int MODEL, bool MASS_ONLY>
+__global__ void __launch_bounds__(256) k_p2g(KP<T> a) {
+    constexpr int KS = Spline<ORDER>::K;
+    constexpr int NA = MASS_ONLY ? 1 : 4;
+    extern __shared__ double smem_d[];
+    T* sm = reinterpret_cast<T*>(smem_d);
+    const int i = logical_block() * blockDim.x + threadIdx.x;
+    const bool valid = i < a.n;
+    Sten<T, ORDER> S;
+    v3<T> x = vzero<T>();
+    if (valid) {
+        x = load_v3(a.st, a.ld, PX, i);
+        S.init(a, x);
+    } else {
+        S.b[0] = S.b[1] = S.b[2] = 0;
+    }
+    const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
+    if (win.lds)
+        for (int t = threadIdx.x; t < win.vol * NA; t += blockDim.x) sm[(t / win.vol) * a.cap + t % win.vol] = T(0);
+
+    T m = T(0);
+    m3<T> A = mzero<T>();
+    v3<T> mv = vzero<T>();
+    if (valid) {
+        m = a.stat[SM * a.ld + i];
+        if (!MASS_ONLY) {
+            const T vol = a.stat[SVOL * a.ld + i];
+            const T lam = a.stat[SLAM * a.ld + i], mu = a.stat[SMU * a.ld + i];
+            const v3<T> v = load_v3(a.st, a.ld, PV, i);
+            m3<T> F = load_m3(a.st, a.ld, PF, i);
+            if (a.ctl) F = madd(F, load_m3(a.ctl, a.ld, 0, i));
+            const m3<T> C = load_m3(a.st, a.ld, PC, i);
+            const m3<T> P = Model<MODEL>::pk1(F, lam, mu);
+            m3<T> str = mmult(P, F);
+            const T tau = a.zeta * mtrace(C);
+            str.a[0] += tau; str.a[4] += tau; str.a[8] += tau;
+            const T k0 = -Spline<ORDER>::dinv(a.dx) * a.dt * vol;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) A.a[q] = k0 * str.a[q] + m * C.a[q];
+            const T dragf = T(1) - a.dt * a.drag;
+            mv = {{m * v.a[0] * dragf, m * v.a[1] * dragf, m * v.a[2] * dragf}};
+        }
+    }
+    __syncthreads();
+    if (valid) {
+        T* lm = sm;
+        T* lp0 = sm + a.cap;
+        T* lp1 = sm + 2 * a.cap;
+        T* lp2 = sm + 3 * a.cap;
+#pragma unroll
+        for (int ia = 0; ia < KS; ++ia) {
+            const int gi = S.b[0] + ia;
+            const T d0 = S.u[0][ia] * a.dx;
+#pragma unroll
+            for (int ib = 0; ib < KS; ++ib) {
+                const int gj = S.b[1] + ib;
+                const T d1 = S.u[1][ib] * a.dx;
+                const T w01 = S.w[0][ia] * S.w[1][ib];
+                // A dgp with dgp = (d0, d1, d2): split the z-independent part
+                T q0 = mv.a[0] + A.a[0] * d0 + A.a[1] * d1;
+                T q1 = mv.a[1] + A.a[3] * d0 + A.a[4] * d1;
+                T q2 = mv.a[2] + A.a[6] * d0 + A.a[7] * d1;
+#pragma unroll
+                for (int ic = 0; ic < KS; ++ic) {
+                    const int gk = S.b[2] + ic;
+                    if (!in_grid(a, gi, gj, gk)) continue;
+                    const T d2 = S.u[2][ic] * a.dx;
+                    const T wt = w01 * S.w[2][ic];
+                    if (win.lds) {
+                        const int li = ((gi - win.lo0) * win.e1 + (gj - win.lo1)) * win.e2 + (gk - win.lo2);
+                        atomicAdd(&lm[li], wt * m);
+                        if (!MASS_ONLY) {
+                            atomicAdd(&lp0[li], wt * (q0 + A.a[2] * d2));
+                            atomicAdd(&lp1[li], wt * (q1 + A.a[5] * d2));
+                            atomicAdd(&lp2[li], wt * (q2 + A.a[8] * d2));
+                        }
+                    } else {
+                        const long long g = gindex(a, gi, gj, gk);
+                        unsafeAtomicAdd(a.gm + g, wt * m);
+                        if (!MASS_ONLY) {
+                            unsafeAtomicAdd(a.gp + g, wt * (q0 + A.a[2] * d2));
+                            unsafeAtomicAdd(a.gp + a.gld + g, wt * (q1 + A.a[5] * d2));
+                            unsafeAtomicAdd(a.gp + 2 * a.gld + g, wt * (q2 + A.a[8] * d2));
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (win.lds && win.any) {
+        __syncthreads();
+        const int e12 = win.e1 * win.e2;
+        for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
+            const int gi = win.lo0 + t / e12, gj = win.lo1 + (t / win.e2) % win.e1, gk = win.lo2 + t % win.e2;
+            if (!in_grid(a, gi, gj, gk)) continue;
+            const T vm = sm[t];
+            const long long g = gindex(a, gi, gj, gk);
+            if (MASS_ONLY) {
+                if (vm != T(0)) unsafeAtomicAdd(a.gm + g, vm);
+            } else {
+                const T p0 = sm[a.cap + t], p1 = sm[2 * a.cap + t], p2 = sm[3 * a.cap + t];
+                if (vm != T(0) || p0 != T(0) || p1 != T(0) || p2 != T(0)) {
+                    unsafeAtomicAdd(a.gm + g, vm);
+                    unsafeAtomicAdd(a.gp + g, p0);
+                    unsafeAtomicAdd(a.gp + a.gld + g, p1);
+                    unsafeAtomicAdd(a.gp + 2 * a.gld + g, p2);
+                }
+            }
+        }
+    }
+}
+
+// Stage the window's grid velocities in LDS (3 arrays of cap).
+template <typename T>
+DMPM_DI void stage_velocity(const KP<T>& a, const Win& win, T* sv) {
+    const int e12 = win.e1 * win.e2;
+    for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
+        const int gi = win.lo0 + t / e12, gj = win.lo1 + (t / win.e2) % win.e1, gk = win.lo2 + t % win.e2;
+        T v0 = T(0), v1 = T(0), v2 = T(0);
+        if (in_grid(a, gi, gj, gk)) node_velocity(a, gindex(a, gi, gj, gk), gi, gj, gk, v0, v1, v2);
+        sv[t] = v0;
+        sv[a.cap + t] = v1;
+        sv[2 * a.cap + t] = v2;
+    }
+}
+
+// Velocity of stencil node (gi,gj,gk), from LDS or from the grid.
+template <typename T>
+DMPM_DI void read_velocity(const KP<T>& a, const Win& win, const T* sv, int gi, int gj, int gk, T& v0, T& v1,
+                           T& v2) {
+    if (win.lds) {
+        const int li = ((gi - win.lo0) * win.e1 + (gj - win.lo1)) * win.e2 + (gk - win.lo2);
+        v0 = sv[li];
+        v1 = sv[a.cap + li];
+        v2 = sv[2 * a.cap + li];
+    } else {
+        node_velocity(a, gindex(a, gi, gj, gk), gi, gj, gk, v0, v1, v2);
+    }
+}
+
+// G2P gather (ForwardSimulation.cpp:89-123): v' = sum w v_i,
+// C' = D^{-1} sum w v_i (x_i - x_p)^T.
+template <typename T, int ORDER>
+DMPM_DI void g2p_gather(const KP<T>& a, const Win& win, const T* sv, const Sten<T, ORDER>& S, v3<T>& vn,
+                        m3<T>& Craw) {
+    constexpr int KS = Spline<ORDER>::K;
+    vn = vzero<T>();
+    Craw = mzero<T>();
+#pragma unroll
+    for (int ia = 0; ia < KS; ++ia) {
+        const int gi = S.b[0] + ia;
+        const T d0 = S.u[0][ia] * a.dx;
+#pragma unroll
+        for (int ib = 0; ib < KS; ++ib) {
+            const int gj = S.b[1] + ib;
+            const T d1 = S.u[1][ib] * a.dx;
+            const T w01 = S.w[0][ia] * S.w[1][ib];
+#pragma unroll
+            for (int ic = 0; ic < KS; ++ic) {
+                const int gk = S.b[2] + ic;
+                if (!in_grid(a, gi, gj, gk)) continue;
+                const T d2 = S.u[2][ic] * a.dx;
+                const T wt = w01 * S.w[2][ic];
+                T v0, v1, v2;
+                read_velocity(a, win, sv, gi, gj, gk, v0, v1, v2);
+                const T w0 = wt * v0, w1 = wt * v1, w2 = wt * v2;
+                vn.a[0] += w0; vn.a[1] += w1; vn.a[2] += w2;
+                Craw.a[0] += w0 * d0; Craw.a[1] += w0 * d1; Craw.a[2] += w0 * d2;
+                Craw.a[3] += w1 * d0; Craw.a[4] += w1 * d1; Craw.a[5] += w1 * d2;
+                Craw.a[6] += w2 * d0; Craw.a[7] += w2 * d1; Craw.a[8] += w2 * d2;
+            }
+        }
+    }
+    Craw = mscale(Spline<ORDER>::dinv(a.dx), Craw);
+}
+
+// G2P + P_op_2 (ForwardSimulation.cpp:89-157): writes layer t+1.
+template <typename T, int ORDER>
+__global__ void __launch_bounds__(256) k_g2p(KP<T> a) {
+    constexpr int KS = Spline<ORDER>::K;
+    extern __shared__ double smem_d[];
+    T* sv = reinterpret_cast<T*>(smem_d);
+    const int i = logical_block() * blockDim.x + threadIdx.x;
+    const bool valid = i < a.n;
+    Sten<T, ORDER> S;
+    v3<T> x = vzero<T>();
+    if (valid) {
+        x = load_v3(a.st, a.ld, PX, i);
+        S.init(a, x);
+    } else {
+        S.b[0] = S.b[1] = S.b[2] = 0;
+    }
+    const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
+    if (win.lds) {
+        stage_velocity(a, win, sv);
+        __syncthreads();
+    }
+    if (!valid) return;
+    v3<T> vn;
+    m3<T> Craw;
+    g2p_gather<T, ORDER>(a, win, sv, S, vn, Craw);
+    const m3<T> F = load_m3(a.st, a.ld, PF, i);
+    const m3<T> Ft = a.ctl ? madd(F, load_m3(a.ctl, a.ld, 0, i)) : F;
+    const Trust<T> tr = trust_region(a, Ft, Craw);
+    const m3<T> Cn = tr.state ? mscale(tr.s, Craw) : Craw;
+    m3<T> A1 = mscale(a.dt, Cn);
+    A1.a[0] += T(1); A1.a[4] += T(1); A1.a[8] += T(1);
+    const m3<T> Fu = mmul(A1, Ft);
+    m3<T> Fn;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) Fn.a[q] = (T(1) - a.sm) * Fu.a[q] + a.sm * F.a[q];
+    const v3<T> xn = {{x.a[0] + a.dt * vn.a[0], x.a[1] + a.dt * vn.a[1], x.a[2] + a.dt * vn.a[2]}};
+    store_v3(a.out, a.ld, PX, i, xn);
+    store_v3(a.out, a.ld, PV, i, vn);
+    store_m3(a.out, a.ld, PF, i, Fn);
+    store_m3(a.out, a.ld, PC, i, Cn);
+}
+
+// Phase 1 of the adjoint (BackPropagation.cpp:22-32).  REF reproduces the
+// reference's formulas; otherwise the exact VJP of P_op_2 including the
+// trust region and the smoothing blend.
+template <typename T, bool REF>
+struct Phase1 {
+    v3<T> gvtot;  // dL/dv' (including dt dL/dx')
+    m3<T> gCraw;  // dL/dC' before the trust region
+    m3<T> gFt;    // dL/d(F+dFc) from the F update
+    m3<T> gFdir;  // dL/dF through the smoothing blend (F only, not dFc)
+};
+
+template <typename T, bool REF>
+DMPM_DI Phase1<T, REF> phase1(const KP<T>& a, int i, const m3<T>& Ft, const m3<T>& Craw, const Trust<T>& tr) {
+    Phase1<T, REF> r;
+    const v3<T> gx1 = load_v3(a.adj1, a.ld, PX, i), gv1 = load_v3(a.adj1, a.ld, PV, i);
+    const m3<T> gF1 = load_m3(a.adj1, a.ld, PF, i), gC1 = load_m3(a.adj1, a.ld, PC, i);
+    const m3<T> Cn = tr.state ? mscale(tr.s, Craw) : Craw;
+    m3<T> A1 = mscale(a.dt, Cn);
+    A1.a[0] += T(1); A1.a[4] += T(1); A1.a[8] += T(1);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) r.gvtot.a[q] = a.dt * gx1.a[q] + gv1.a[q];
+    const m3<T> AtG = mtmul(A1, gF1);
+    const m3<T> GFt = mmult(gF1, Ft);  // gF1 Ft^T
+    if (REF) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            r.gFt.a[q] = (T(1) - a.sm) * AtG.a[q] + a.sm * gF1.a[q];
+            r.gFdir.a[q] = T(0);
+            r.gCraw.a[q] = a.dt * GFt.a[q] + gC1.a[q];
+        }
+    } else {
+        m3<T> gCfin;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            r.gFt.a[q] = (T(1) - a.sm) * AtG.a[q];
+            r.gFdir.a[q] = a.sm * gF1.a[q];
+            gCfin.a[q] = gC1.a[q] + (T(1) - a.sm) * a.dt * GFt.a[q];
+        }
+        if (tr.state == 0) {
+            r.gCraw = gCfin;
+        } else if (tr.state == 2) {
+            r.gCraw = mzero<T>();
+        } else {
+            const T cc = mddot(gCfin, Craw);
+            r.gCraw = mscale(tr.s, gCfin);
+            const T trC = mtrace(Craw);
+            if (a.dt * trC < T(-1e-6)) {
+                const T d = -tr.s / trC * cc;
+                r.gCraw.a[0] += d; r.gCraw.a[4] += d; r.gCraw.a[8] += d;
+            }
+            const m3<double> Ftd = mcast<double>(Ft);
+            const double detF = mdet(Ftd);
+            if (detF > 1e-6) {
+                const double dsdJ = -(double)a.jmin / ((double)tr.J0 * (double)tr.J0) / (double)tr.denom;
+                const m3<double> cf = mcof(Ftd);
+                const double k = (double)cc * dsdJ;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) r.gFt.a[q] += (T)(k * cf.a[q]);
+            }
+        }
+    }
+    return r;
+}
+
+// Adjoint phases 1 + 2: dL/dv_i += w gvtot + D^{-1} w gCraw (x_i - x_p)
+// (BackPropagation.cpp:36-55), accumulated in an LDS window.
+template <typename T, int ORDER, bool REF>
+__global__ void __launch_bounds__(256) k_adj_scatter(KP<T> a) {
+    constexpr int KS = Spline<ORDER>::K;
+    extern __shared__ double smem_d[];
+    T* sv = reinterpret_cast<T*>(smem_d);  // 3 velocity arrays, then 3 accumulators
+    T* sg = sv + 3 * a.cap;
+    const int i = logical_block() * blockDim.x + threadIdx.x;
+    const bool valid = i < a.n;
+    Sten<T, ORDER> S;
+    v3<T> x = vzero<T>();
+    if (valid) {
+        x = load_v3(a.st, a.ld, PX, i);
+        S.init(a, x);
+    } else {
+        S.b[0] = S.b[1] = S.b[2] = 0;
+    }
+    const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
+    if (win.lds) {
+        stage_velocity(a, win, sv);
+        for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
+            sg[t] = T(0);
+            sg[a.cap + t] = T(0);
+            sg[2 * a.cap + t] = T(0);
+        }
+        __syncthreads();
+    }
+    if (valid) {
+        v3<T> vn;
+        m3<T> Craw;
+        g2p_gather<T, ORDER>(a, win, sv, S, vn, Craw);
+        m3<T> Ft = load_m3(a.st, a.ld, PF, i);
+        if (a.ctl) Ft = madd(Ft, load_m3(a.ctl, a.ld, 0, i));
+        const Trust<T> tr = trust_region(a, Ft, Craw);
+        const Phase1<T, REF> p1 = phase1<T, REF>(a, i, Ft, Craw, tr);
+        const T dinv = Spline<ORDER>::dinv(a.dx);
+        const m3<T> Gc = mscale(dinv, p1.gCraw);
+#pragma unroll
+        for (int ia = 0; ia < KS; ++ia) {
+            const int gi = S.b[0] + ia;
+            const T d0 = S.u[0][ia] * a.dx;
+#pragma unroll
+            for (int ib = 0; ib < KS; ++ib) {
+                const int gj = S.b[1] + ib;
+                const T d1 = S.u[1][ib] * a.dx;
+                const T w01 = S.w[0][ia] * S.w[1][ib];
+                const T q0 = p1.gvtot.a[0] + Gc.a[0] * d0 + Gc.a[1] * d1;
+                const T q1 = p1.gvtot.a[1] + Gc.a[3] * d0 + Gc.a[4] * d1;
+                const T q2 = p1.gvtot.a[2] + Gc.a[6] * d0 + Gc.a[7] * d1;
+#pragma unroll
+                for (int ic = 0; ic < KS; ++ic) {
+                    const int gk = S.b[2] + ic;
+                    if (!in_grid(a, gi, gj, gk)) continue;
+                    const T d2 = S.u[2][ic] * a.dx;
+                    const T wt = w01 * S.w[2][ic];
+                    const T g0 = wt * (q0 + Gc.a[2] * d2), g1 = wt * (q1 + Gc.a[5] * d2), g2 = wt * (q2 + Gc.a[8] * d2);
+                    if (win.lds) {
+                        const int li = ((gi - win.lo0) * win.e1 + (gj - win.lo1)) * win.e2 + (gk - win.lo2);
+                        atomicAdd(&sg[li], g0);
+                        atomicAdd(&sg[a.cap + li], g1);
+                        atomicAdd(&sg[2 * a.cap + li], g2);
+                    } else {
+                        const long long g = gindex(a, gi, gj, gk);
+                        unsafeAtomicAdd(a.ggv + g, g0);
+                        unsafeAtomicAdd(a.ggv + a.gld + g, g1);
+                        unsafeAtomicAdd(a.ggv + 2 * a.gld + g, g2);
+                    }
+                }
+            }
+        }
+    }
+    if (win.lds && win.any) {
+        __syncthreads();
+        const int e12 = win.e1 * win.e2;
+        for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
+            const int gi = win.lo0 + t / e12, gj = win.lo1 + (t / win.e2) % win.e1, gk = win.lo2 + t % win.e2;
+            if (!in_grid(a, gi, gj, gk)) continue;
+            const T g0 = sg[t], g1 = sg[a.cap + t], g2 = sg[2 * a.cap + t];
+            if (g0 != T(0) || g1 != T(0) || g2 != T(0)) {
+                const long long g = gindex(a, gi, gj, gk);
+                unsafeAtomicAdd(a.ggv + g, g0);
+                unsafeAtomicAdd(a.ggv + a.gld + g, g1);
+                unsafeAtomicAdd(a.ggv + 2 * a.gld + g, g2);
+            }
+        }
+    }
+}
+
+// Grid adjoint (BackPropagation.cpp:59-72) for one node: h = dL/dp_i,
+// gm = dL/dm_i.  Exact: dm uses v_i - dt f_ext; REF uses v_i (as written).
+template <typename T, bool REF>
+DMPM_DI void node_adjoint(const KP<T>& a, long long g, int i, int j, int k, T* v, T* h, T& gmass) {
+    const T m = a.gm[g];
+    if (m > T(1e-12) && !is_wall(a, i, j, k)) {
+        const T inv = T(1) / m;
+        const T p0 = a.gp[g] * inv, p1 = a.gp[a.gld + g] * inv, p2 = a.gp[2 * a.gld + g] * inv;
+        v[0] = p0 + a.dt * a.f0;
+        v[1] = p1 + a.dt * a.f1;
+        v[2] = p2 + a.dt * a.f2;
+        const T g0 = a.ggv[g], g1 = a.ggv[a.gld + g], g2 = a.ggv[2 * a.gld + g];
+        h[0] = g0 * inv;
+        h[1] = g1 * inv;
+        h[2] = g2 * inv;
+        const T vd = REF ? (v[0] * g0 + v[1] * g1 + v[2] * g2) : (p0 * g0 + p1 * g1 + p2 * g2);
+        gmass = -inv * vd;
+    } else {
+        v[0] = v[1] = v[2] = T(0);
+        h[0] = h[1] = h[2] = T(0);
+        gmass = T(0);
+    }
+}
+
+// Adjoint phases 3 + 4 and the stress VJP (BackPropagation.cpp:59-143).
+// The window holds per node v_i (3), h_i = dL/dp_i (3), dL/dm_i (1).
+template <typename T, int ORDER, int MODEL, bool REF>
+__global__ void __launch_bounds__(256) k_adj_gather(KP<T> a) {
+    constexpr int KS = Spline<ORDER>::K;
+    extern __shared__ double smem_d[];
+    T* sv = reinterpret_cast<T*>(smem_d);  // v (3 arrays), h (3), gm (1)
+    const int i = logical_block() * blockDim.x + threadIdx.x;
+    const bool valid = i < a.n;
+    Sten<T, ORDER> S;
+    v3<T> x = vzero<T>();
+    if (valid) {
+        x = load_v3(a.st, a.ld, PX, i);
+        S.init(a, x);
+    } else {
+        S.b[0] = S.b[1] = S.b[2] = 0;
+    }
+    const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
+    if (win.lds) {
+        const int e12 = win.e1 * win.e2;
+        for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
+            const int gi = win.lo0 + t / e12, gj = win.lo1 + (t / win.e2) % win.e1, gk = win.lo2 + t % win.e2;
+            T v[3] = {T(0), T(0), T(0)}, h[3] = {T(0), T(0), T(0)}, gmass = T(0);
+            if (in_grid(a, gi, gj, gk)) node_adjoint<T, REF>(a, gindex(a, gi, gj, gk), gi, gj, gk, v, h, gmass);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                sv[q * a.cap + t] = v[q];
+                sv[(3 + q) * a.cap + t] = h[q];
+            }
+            sv[6 * a.cap + t] = gmass;
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+    v3<T> vn;
+    m3<T> Craw;
+    g2p_gather<T, ORDER>(a, win, sv, S, vn, Craw);
+    const m3<T> F = load_m3(a.st, a.ld, PF, i);
+    const m3<T> Ft = a.ctl ? madd(F, load_m3(a.ctl, a.ld, 0, i)) : F;
+    const Trust<T> tr = trust_region(a, Ft, Craw);
+    const Phase1<T, REF> p1 = phase1<T, REF>(a, i, Ft, Craw, tr);
+
+    const T m = a.stat[SM * a.ld + i], vol = a.stat[SVOL * a.ld + i];
+    const T lam = a.stat[SLAM * a.ld + i], mu = a.stat[SMU * a.ld + i];
+    const v3<T> vp = load_v3(a.st, a.ld, PV, i);
+    const m3<T> C = load_m3(a.st, a.ld, PC, i);
+    const m3<T> P = Model<MODEL>::pk1(Ft, lam, mu);
+    const T dinv = Spline<ORDER>::dinv(a.dx);
+    const T k0 = dinv * a.dt * vol;
+    // Gt = A_p = -k0 (P Ft^T + zeta tr(C) I) + m C
+    m3<T> Gt;
+    {
+        const m3<T> PFt = mmult(P, Ft);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) Gt.a[q] = -k0 * PFt.a[q] + m * C.a[q];
+        const T bulk = -k0 * a.zeta * mtrace(C);
+        Gt.a[0] += bulk; Gt.a[4] += bulk; Gt.a[8] += bulk;
+    }
+    const T mom = REF ? m : m * (T(1) - a.dt * a.drag);
+    const m3<T> Gc = mscale(dinv, p1.gCraw);
+    const v3<T> mvp = {{mom * vp.a[0], mom * vp.a[1], mom * vp.a[2]}};
+
+    // accumulators
+    m3<T> gA = mzero<T>();   // sum w h dgp^T
+    v3<T> sh = vzero<T>();   // sum w h
+    v3<T> dxa = vzero<T>();  // sum s_i grad w
+#pragma unroll
+    for (int ia = 0; ia < KS; ++ia) {
+        const int gi = S.b[0] + ia;
+        const T d0 = S.u[0][ia] * a.dx;
+#pragma unroll
+        for (int ib = 0; ib < KS; ++ib) {
+            const int gj = S.b[1] + ib;
+            const T d1 = S.u[1][ib] * a.dx;
+            const T w01 = S.w[0][ia] * S.w[1][ib];
+#pragma unroll
+            for (int ic = 0; ic < KS; ++ic) {
+                const int gk = S.b[2] + ic;
+                if (!in_grid(a, gi, gj, gk)) continue;
+                const T d2 = S.u[2][ic] * a.dx;
+                const T wt = w01 * S.w[2][ic];
+                T v[3], h[3], gmass;
+                if (win.lds) {
+                    const int li = ((gi - win.lo0) * win.e1 + (gj - win.lo1)) * win.e2 + (gk - win.lo2);
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        v[q] = sv[q * a.cap + li];
+                        h[q] = sv[(3 + q) * a.cap + li];
+                    }
+                    gmass = sv[6 * a.cap + li];
+                } else {
+                    node_adjoint<T, REF>(a, gindex(a, gi, gj, gk), gi, gj, gk, v, h, gmass);
+                }
+                // grad_{x_p} w = -(1/dx) (N' N N, N N' N, N N N')
+                const T invdx = T(-1) / a.dx;
+                const T gw0 = invdx * S.dw[0][ia] * S.w[1][ib] * S.w[2][ic];
+                const T gw1 = invdx * S.w[0][ia] * S.dw[1][ib] * S.w[2][ic];
+                const T gw2 = invdx * S.w[0][ia] * S.w[1][ib] * S.dw[2][ic];
+                // q = mom v_p + Gt dgp ; (Gc dgp)
+                const T Gd0 = Gt.a[0] * d0 + Gt.a[1] * d1 + Gt.a[2] * d2;
+                const T Gd1 = Gt.a[3] * d0 + Gt.a[4] * d1 + Gt.a[5] * d2;
+                const T Gd2 = Gt.a[6] * d0 + Gt.a[7] * d1 + Gt.a[8] * d2;
+                const T Cd0 = Gc.a[0] * d0 + Gc.a[1] * d1 + Gc.a[2] * d2;
+                const T Cd1 = Gc.a[3] * d0 + Gc.a[4] * d1 + Gc.a[5] * d2;
+                const T Cd2 = Gc.a[6] * d0 + Gc.a[7] * d1 + Gc.a[8] * d2;
+                const T s = m * gmass + (mvp.a[0] + Gd0) * h[0] + (mvp.a[1] + Gd1) * h[1] + (mvp.a[2] + Gd2) * h[2] +
+                            v[0] * (p1.gvtot.a[0] + Cd0) + v[1] * (p1.gvtot.a[1] + Cd1) +
+                            v[2] * (p1.gvtot.a[2] + Cd2);
+                dxa.a[0] += s * gw0;
+                dxa.a[1] += s * gw1;
+                dxa.a[2] += s * gw2;
+                const T wh0 = wt * h[0], wh1 = wt * h[1], wh2 = wt * h[2];
+                sh.a[0] += wh0; sh.a[1] += wh1; sh.a[2] += wh2;
+                gA.a[0] += wh0 * d0; gA.a[1] += wh0 * d1; gA.a[2] += wh0 * d2;
+                gA.a[3] += wh1 * d0; gA.a[4] += wh1 * d1; gA.a[5] += wh1 * d2;
+                gA.a[6] += wh2 * d0; gA.a[7] += wh2 * d1; gA.a[8] += wh2 * d2;
+            }
+        }
+    }
+    // dL/dx_p: sum s_i grad w - Gt^T sum w h - D^{-1} gCraw^T v' + dL/dx'
+    const v3<T> gx1 = load_v3(a.adj1, a.ld, PX, i);
+    const v3<T> Gsh = mtv(Gt, sh);
+    const v3<T> Cv = mtv(Gc, vn);
+    v3<T> gx;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) gx.a[q] = dxa.a[q] - Gsh.a[q] - Cv.a[q] + gx1.a[q];
+    const T dragf = m * (T(1) - a.dt * a.drag);
+    const v3<T> gv = {{dragf * sh.a[0], dragf * sh.a[1], dragf * sh.a[2]}};
+    // A_p = -k0 (P Ft^T + zeta tr C I) + m C
+    const m3<T> dP = mscale(-k0, mmul(gA, Ft));
+    m3<T> gFt = madd(p1.gFt, mscale(-k0, mtmul(gA, P)));
+    m3<T> gC = mscale(m, gA);
+    if (REF) {
+        // bulk-viscosity adjoint on the diagonal entries only (:117-123), and
+        // (I + dt C')^T dL/dF' added a second time (:140)
+        const T factor = -k0 * a.zeta;
+        gC.a[0] += factor * gA.a[0];
+        gC.a[4] += factor * gA.a[4];
+        gC.a[8] += factor * gA.a[8];
+        const m3<T> gF1 = load_m3(a.adj1, a.ld, PF, i);
+        const m3<T> Cn = tr.state ? mscale(tr.s, Craw) : Craw;
+        m3<T> A1 = mscale(a.dt, Cn);
+        A1.a[0] += T(1); A1.a[4] += T(1); A1.a[8] += T(1);
+        gFt = madd(gFt, mtmul(A1, gF1));
+    } else {
+        const T tb = -k0 * a.zeta * mtrace(gA);
+        gC.a[0] += tb; gC.a[4] += tb; gC.a[8] += tb;
+    }
+    gFt = madd(gFt, Model<MODEL>::hvp(Ft, lam, mu, dP));
+    store_v3(a.adj0, a.ld, PX, i, gx);
+    store_v3(a.adj0, a.ld, PV, i, gv);
+    const m3<T> gF = madd(gFt, p1.gFdir);
+    store_m3(a.adj0, a.ld, PF, i, gF);
+    store_m3(a.adj0, a.ld, PC, i, gC);
+    if (a.gctl) store_m3(a.gctl, a.ld, 0, i, REF ? gF : gFt);
+}
+
+// ---------------------------------------------------------------------------
+// Small element-wise kernels.
+template <typename T>
+__global__ void k_fill_zero(T* p, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = T(0);
+}
+
+template <typename T, typename S>
+__global__ void k_import(const S* src, T* dst, const int* perm, int n, int nc, long long ld) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long s = (long long)(perm ? perm[i] : i) * nc;
+    for (int c = 0; c < nc; ++c) dst[c * ld + i] = (T)src[s + c];
+}
+
+template <typename T, typename S>
+__global__ void k_accumulate(const S* src, T* dst, const int* perm, int n, int nc, long long ld) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long s = (long long)(perm ? perm[i] : i) * nc;
+    for (int c = 0; c < nc; ++c) dst[c * ld + i] += (T)src[s + c];
+}
+
+template <typename T, typename D>
+__global__ void k_export(const T* src, D* dst, const int* perm, int n, int nc, long long ld) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long d = (long long)(perm ? perm[i] : i) * nc;
+    for (int c = 0; c < nc; ++c) dst[d + c] = (D)src[c * ld + i];
+}
+
+template <typename T, typename D>
+__global__ void k_export_Ftot(const T* st, const T* ctl, D* dst, const int* perm, int n, long long ld) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long d = (long long)(perm ? perm[i] : i) * 9;
+    for (int c = 0; c < 9; ++c) dst[d + c] = (D)(st[(PF + c) * ld + i] + (ctl ? ctl[c * ld + i] : T(0)));
+}
+
+template <typename T>
+__global__ void k_copy(T* dst, const T* src, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// Block sum of doubles; result valid in thread 0.
+DMPM_DI double block_sum(double v) {
+    __shared__ double red[16];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += red[q];
+    }
+    return s;
+}
+
+template <typename T>
+__global__ void k_mass_sums(const T* gm, const T* tm, long long G, double* partial) {
+    double c = 0, t = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < G; i += (long long)gridDim.x * blockDim.x) {
+        c += (double)gm[i];
+        t += (double)tm[i];
+    }
+    c = block_sum(c);
+    t = block_sum(t);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = c;
+        partial[2 * blockIdx.x + 1] = t;
+    }
+}
+
+// EndLayerMassLoss node terms (CompGraph.cpp:60-93).
+template <typename T>
+__global__ void k_mass_loss_nodes(const T* gm, const T* tm, long long G, const double* sums, T* dldm,
+                                  double* partial) {
+    const T sum_c = (T)sums[0], sum_t = (T)sums[1];
+    const T s = (sum_t + T(1e-6)) / (sum_c + T(1e-6));
+    const T eps = T(1e-4), min_mass = T(1e-3);
+    double acc = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < G; i += (long long)gridDim.x * blockDim.x) {
+        const T cms = s * gm[i];
+        const T ld = log(cms + T(1) + eps) - log(tm[i] + T(1) + eps);
+        acc += (double)(T(0.5) * ld * ld);
+        T d = (s / (cms + T(1) + eps)) * ld;
+        if (cms < min_mass) {
+            const T diff = min_mass - cms;
+            acc += (double)(diff * diff);
+            d += T(-2) * diff * s;
+        }
+        dldm[i] = d;
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// Per-particle dL/dx of EndLayerMassLoss (CompGraph.cpp:95-125); the other
+// adjoint fields of the last layer are zeroed (:122-124).
+template <typename T, int ORDER>
+__global__ void k_mass_loss_particles(KP<T> a, const T* dldm, const T* tm) {
+    constexpr int KS = Spline<ORDER>::K;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const v3<T> x = load_v3(a.st, a.ld, PX, i);
+    Sten<T, ORDER> S;
+    S.init(a, x);
+    const T m = a.stat[SM * a.ld + i];
+    v3<T> g = vzero<T>();
+    const T invdx = T(-1) / a.dx;
+    for (int ia = 0; ia < KS; ++ia)
+        for (int ib = 0; ib < KS; ++ib)
+            for (int ic = 0; ic < KS; ++ic) {
+                const int gi = S.b[0] + ia, gj = S.b[1] + ib, gk = S.b[2] + ic;
+                if (!in_grid(a, gi, gj, gk)) continue;
+                const long long gg = gindex(a, gi, gj, gk);
+                const T pen = tm[gg] > T(1e-12) ? T(1) : T(5);
+                const T c = pen * m * dldm[gg];
+                g.a[0] += c * invdx * S.dw[0][ia] * S.w[1][ib] * S.w[2][ic];
+                g.a[1] += c * invdx * S.w[0][ia] * S.dw[1][ib] * S.w[2][ic];
+                g.a[2] += c * invdx * S.w[0][ia] * S.w[1][ib] * S.dw[2][ic];
+            }
+    store_v3(a.adj0, a.ld, PX, i, g);
+    for (int q = 3; q < NSTATE; ++q) a.adj0[q * a.ld + i] = T(0);
+}
+
+// CalculatePointCloudVolumes particle pass (ForwardSimulation.cpp:241-269).
+template <typename T, int ORDER>
+__global__ void k_volumes(KP<T> a, T* vol) {
+    constexpr int KS = Spline<ORDER>::K;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const v3<T> x = load_v3(a.st, a.ld, PX, i);
+    Sten<T, ORDER> S;
+    S.init(a, x);
+    T mg = T(0);
+    for (int ia = 0; ia < KS; ++ia)
+        for (int ib = 0; ib < KS; ++ib)
+            for (int ic = 0; ic < KS; ++ic) {
+                const int gi = S.b[0] + ia, gj = S.b[1] + ib, gk = S.b[2] + ic;
+                if (!in_grid(a, gi, gj, gk)) continue;
+                mg += S.w[0][ia] * S.w[1][ib] * S.w[2][ic] * a.gm[gindex(a, gi, gj, gk)];
+            }
+    const T cell = a.dx * a.dx * a.dx;
+    const T rho = mg / fmax(cell, T(1e-12));
+    const T eps = sizeof(T) == 4 ? T(1.1920929e-07) : T(2.220446049250313e-16);
+    vol[i] = rho > eps ? a.stat[SM * a.ld + i] / rho : T(0);
+}
+
+template <typename T>
+__global__ void k_sqnorm9(const T* p, int n, long long ld, double* partial) {
+    double acc = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int q = 0; q < 9; ++q) {
+            const double v = (double)p[q * ld + i];
+            acc += v * v;
+        }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+__global__ void k_reduce_partials(const double* partial, int nblocks, int stride, double* out) {
+    // one thread per output component, fixed order: deterministic
+    const int c = threadIdx.x;
+    if (c >= stride) return;
+    double s = 0;
+    for (int b = 0; b < nblocks; ++b) s += partial[b * stride + c];
+    out[c] = s;
+}
+
+// Descend_Adam (PointCloud.cpp:83-147).
+template <typename T>
+__global__ void k_adam(const T* dLdF, T* dFc, T* mom, T* vec, T* vmax, int n, long long ld, const double* sqnorm,
+                       T alpha, T b1, T b2, T eps, int timestep) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T gn = (T)sqrt(sqnorm[0]);
+    const T inv_gn = T(1) / fmax(fmax(gn, T(1e-12)), T(1e-12));
+    const int t = timestep < 1 ? 1 : timestep;
+    const T c1 = T(1) - (T)pow((double)b1, t);
+    const T c2 = T(1) - (T)pow((double)b2, t);
+    T step[9], xo[9];
+    T sn = T(0), xn = T(0);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const long long o = q * ld + i;
+        const T g = dLdF[o] * inv_gn;
+        const T mq = b1 * mom[o] + (T(1) - b1) * g;
+        const T vq = b2 * vec[o] + (T(1) - b2) * g * g;
+        mom[o] = mq;
+        vec[o] = vq;
+        const T vm = fmax(vmax[o], vq / c2);
+        vmax[o] = vm;
+        const T ds = fmax(sqrt(vm), T(1e-8));
+        step[q] = (mq / c1) / (ds + eps);
+        sn += step[q] * step[q];
+        xo[q] = dFc[o];
+        xn += xo[q] * xo[q];
+    }
+    sn = sqrt(sn);
+    xn = sqrt(xn);
+    const T cap = fmax(T(5e-3), T(5e-2) * fmax(T(1), xn));
+    T f = T(1);
+    if (sn > T(0)) {
+        const T scaled = alpha * sn;
+        if (scaled > cap) f = cap / scaled;
+    }
+    const T small_cap = fmax(T(1e-4), T(1e-3) * fmax(T(1), xn));
+    const bool near_stationary = alpha * sn <= small_cap;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        T xq = xo[q] - alpha * (f * step[q]);
+        if (near_stationary) xq = xo[q] + T(0.2) * (xq - xo[q]);
+        dFc[q * ld + i] = xq;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+template <typename T>
+static void launch_p2g(const Launch& L, bool mass_only, hipStream_t s) {
+    const int na = mass_only ? 1 : 4;
+    KP<T> k = make_kp<T>(L, na);
+    const unsigned nb = grid_blocks(L.n, L.block);
+    const size_t sh = (size_t)k.cap * na * sizeof(T);
+#define P2G_CASE(O, M, MO) hipLaunchKernelGGL((k_p2g<T, O, M, MO>), dim3(nb), dim3(L.block), sh, s, k)
+    const int o = L.ph.order, md = L.ph.model;
+    if (mass_only) {
+        if (o == 3) P2G_CASE(3, 0, true); else P2G_CASE(2, 0, true);
+    } else if (o == 3) {
+        if (md == 0) P2G_CASE(3, 0, false); else P2G_CASE(3, 1, false);
+    } else {
+        if (md == 0) P2G_CASE(2, 0, false); else P2G_CASE(2, 1, false);
+    }
+#undef P2G_CASE
+}
+
+template <typename T>
+static void launch_g2p(const Launch& L, hipStream_t s) {
+    KP<T> k = make_kp<T>(L, 3);
+    const unsigned nb = grid_blocks(L.n, L.block);
+    const size_t sh = (size_t)k.cap * 3 * sizeof(T);
+    if (L.ph.order == 3) hipLaunchKernelGGL((k_g2p<T, 3>), dim3(nb), dim3(L.block), sh, s, k);
+    else hipLaunchKernelGGL((k_g2p<T, 2>), dim3(nb), dim3(L.block), sh, s, k);
+}
+
+template <typename T>
+static void launch_adj_scatter(const Launch& L, hipStream_t s) {
+    KP<T> k = make_kp<T>(L, 6);
+    const unsigned nb = grid_blocks(L.n, L.block);
+    const size_t sh = (size_t)k.cap * 6 * sizeof(T);
+    const bool ref = L.ph.adjoint == 1;
+#define AS_CASE(O, R) hipLaunchKernelGGL((k_adj_scatter<T, O, R>), dim3(nb), dim3(L.block), sh, s, k)
+    if (L.ph.order == 3) { if (ref) AS_CASE(3, true); else AS_CASE(3, false); }
+    else { if (ref) AS_CASE(2, true); else AS_CASE(2, false); }
+#undef AS_CASE
+}
+
+template <typename T>
+static void launch_adj_gather(const Launch& L, hipStream_t s) {
+    KP<T> k = make_kp<T>(L, 7);
+    const unsigned nb = grid_blocks(L.n, L.block);
+    const size_t sh = (size_t)k.cap * 7 * sizeof(T);
+    const bool ref = L.ph.adjoint == 1;
+    const int o = L.ph.order, md = L.ph.model;
+#define AG_CASE(O, M, R) hipLaunchKernelGGL((k_adj_gather<T, O, M, R>), dim3(nb), dim3(L.block), sh, s, k)
+    if (o == 3) {
+        if (md == 0) { if (ref) AG_CASE(3, 0, true); else AG_CASE(3, 0, false); }
+        else { if (ref) AG_CASE(3, 1, true); else AG_CASE(3, 1, false); }
+    } else {
+        if (md == 0) { if (ref) AG_CASE(2, 0, true); else AG_CASE(2, 0, false); }
+        else { if (ref) AG_CASE(2, 1, true); else AG_CASE(2, 1, false); }
+    }
+#undef AG_CASE
+}
+
+static inline unsigned ew_blocks(long long n, int bs = 256) {
+    long long nb = (n + bs - 1) / bs;
+    if (nb < 1) nb = 1;
+    if (nb > 65536) nb = 65536;
+    return (unsigned)nb;
+}
+
+void p2g(Prec pr, const Launch& L, bool mass_only, hipStream_t s) {
+    if (pr == PREC_F32) launch_p2g<float>(L, mass_only, s);
+    else launch_p2g<double>(L, mass_only, s);
+}
+void g2p(Prec pr, const Launch& L, hipStream_t s) {
+    if (pr == PREC_F32) launch_g2p<float>(L, s);
+    else launch_g2p<double>(L, s);
+}
+void adj_scatter(Prec pr, const Launch& L, hipStream_t s) {
+    if (pr == PREC_F32) launch_adj_scatter<float>(L, s);
+    else launch_adj_scatter<double>(L, s);
+}
+void adj_gather(Prec pr, const Launch& L, hipStream_t s) {
+    if (pr == PREC_F32) launch_adj_gather<float>(L, s);
+    else launch_adj_gather<double>(L, s);
+}
+
+void fill_zero(Prec pr, void* p, long long n, hipStream_t s) {
+    if (n <= 0) return;
+    if (pr == PREC_F32) hipLaunchKernelGGL(k_fill_zero<float>, dim3(ew_blocks(n)), dim3(256), 0, s, (float*)p, n);
+    else hipLaunchKernelGGL(k_fill_zero<double>, dim3(ew_blocks(n)), dim3(256), 0, s, (double*)p, n);
+}
+
+void import_planes(Prec pr, const void* src, int src_f64, void* dst, const int* perm, int n, int nc, long long ld,
+                   hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (pr == PREC_F32) {
+        if (src_f64) hipLaunchKernelGGL((k_import<float, double>), dim3(nb), dim3(256), 0, s, (const double*)src, (float*)dst, perm, n, nc, ld);
+        else hipLaunchKernelGGL((k_import<float, float>), dim3(nb), dim3(256), 0, s, (const float*)src, (float*)dst, perm, n, nc, ld);
+    } else {
+        if (src_f64) hipLaunchKernelGGL((k_import<double, double>), dim3(nb), dim3(256), 0, s, (const double*)src, (double*)dst, perm, n, nc, ld);
+        else hipLaunchKernelGGL((k_import<double, float>), dim3(nb), dim3(256), 0, s, (const float*)src, (double*)dst, perm, n, nc, ld);
+    }
+}
+
+void accumulate_carry(Prec pr, const void* src, int src_f64, void* carry, int plane_off, int nc, const int* perm,
+                      int n, long long ld, hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (pr == PREC_F32) {
+        float* d = (float*)carry + plane_off * ld;
+        if (src_f64) hipLaunchKernelGGL((k_accumulate<float, double>), dim3(nb), dim3(256), 0, s, (const double*)src, d, perm, n, nc, ld);
+        else hipLaunchKernelGGL((k_accumulate<float, float>), dim3(nb), dim3(256), 0, s, (const float*)src, d, perm, n, nc, ld);
+    } else {
+        double* d = (double*)carry + plane_off * ld;
+        if (src_f64) hipLaunchKernelGGL((k_accumulate<double, double>), dim3(nb), dim3(256), 0, s, (const double*)src, d, perm, n, nc, ld);
+        else hipLaunchKernelGGL((k_accumulate<double, float>), dim3(nb), dim3(256), 0, s, (const float*)src, d, perm, n, nc, ld);
+    }
+}
+
+void export_planes(Prec pr, const void* src, void* dst, int dst_f64, const int* perm, int n, int nc, long long ld,
+                   hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (pr == PREC_F32) {
+        if (dst_f64) hipLaunchKernelGGL((k_export<float, double>), dim3(nb), dim3(256), 0, s, (const float*)src, (double*)dst, perm, n, nc, ld);
+        else hipLaunchKernelGGL((k_export<float, float>), dim3(nb), dim3(256), 0, s, (const float*)src, (float*)dst, perm, n, nc, ld);
+    } else {
+        if (dst_f64) hipLaunchKernelGGL((k_export<double, double>), dim3(nb), dim3(256), 0, s, (const double*)src, (double*)dst, perm, n, nc, ld);
+        else hipLaunchKernelGGL((k_export<double, float>), dim3(nb), dim3(256), 0, s, (const double*)src, (float*)dst, perm, n, nc, ld);
+    }
+}
+
+void export_F_total(Prec pr, const void* st, const void* ctl, void* dst, int dst_f64, const int* perm, int n,
+                    long long ld, hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (pr == PREC_F32) {
+        if (dst_f64) hipLaunchKernelGGL((k_export_Ftot<float, double>), dim3(nb), dim3(256), 0, s, (const float*)st, (const float*)ctl, (double*)dst, perm, n, ld);
+        else hipLaunchKernelGGL((k_export_Ftot<float, float>), dim3(nb), dim3(256), 0, s, (const float*)st, (const float*)ctl, (float*)dst, perm, n, ld);
+    } else {
+        if (dst_f64) hipLaunchKernelGGL((k_export_Ftot<double, double>), dim3(nb), dim3(256), 0, s, (const double*)st, (const double*)ctl, (double*)dst, perm, n, ld);
+        else hipLaunchKernelGGL((k_export_Ftot<double, float>), dim3(nb), dim3(256), 0, s, (const double*)st, (const double*)ctl, (float*)dst, perm, n, ld);
+    }
+}
+
+void copy_elems(Prec pr, void* dst, const void* src, long long count, hipStream_t s) {
+    if (count <= 0) return;
+    if (pr == PREC_F32) hipLaunchKernelGGL(k_copy<float>, dim3(ew_blocks(count)), dim3(256), 0, s, (float*)dst, (const float*)src, count);
+    else hipLaunchKernelGGL(k_copy<double>, dim3(ew_blocks(count)), dim3(256), 0, s, (double*)dst, (const double*)src, count);
+}
+
+void mass_sums(Prec pr, const void* gm, const void* tm, long long G, double* partial, int nblocks, hipStream_t s) {
+    if (pr == PREC_F32) hipLaunchKernelGGL(k_mass_sums<float>, dim3(nblocks), dim3(256), 0, s, (const float*)gm, (const float*)tm, G, partial);
+    else hipLaunchKernelGGL(k_mass_sums<double>, dim3(nblocks), dim3(256), 0, s, (const double*)gm, (const double*)tm, G, partial);
+}
+
+void mass_loss_nodes(Prec pr, const void* gm, const void* tm, long long G, const double* sums, int, void* dldm,
+                     double* partial, int nblocks, hipStream_t s) {
+    if (pr == PREC_F32) hipLaunchKernelGGL(k_mass_loss_nodes<float>, dim3(nblocks), dim3(256), 0, s, (const float*)gm, (const float*)tm, G, sums, (float*)dldm, partial);
+    else hipLaunchKernelGGL(k_mass_loss_nodes<double>, dim3(nblocks), dim3(256), 0, s, (const double*)gm, (const double*)tm, G, sums, (double*)dldm, partial);
+}
+
+void mass_loss_particles(Prec pr, const Launch& L, const void* dldm, const void* tm, hipStream_t s) {
+    const unsigned nb = (unsigned)((L.n + 255) / 256);
+    if (L.n <= 0) return;
+    if (pr == PREC_F32) {
+        KP<float> k = make_kp<float>(L, 0);
+        if (L.ph.order == 3) hipLaunchKernelGGL((k_mass_loss_particles<float, 3>), dim3(nb), dim3(256), 0, s, k, (const float*)dldm, (const float*)tm);
+        else hipLaunchKernelGGL((k_mass_loss_particles<float, 2>), dim3(nb), dim3(256), 0, s, k, (const float*)dldm, (const float*)tm);
+    } else {
+        KP<double> k = make_kp<double>(L, 0);
+        if (L.ph.order == 3) hipLaunchKernelGGL((k_mass_loss_particles<double, 3>), dim3(nb), dim3(256), 0, s, k, (const double*)dldm, (const double*)tm);
+        else hipLaunchKernelGGL((k_mass_loss_particles<double, 2>), dim3(nb), dim3(256), 0, s, k, (const double*)dldm, (const double*)tm);
+    }
+}
+
+void reduce_partials(const double* partial, int nblocks, int stride, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partial, nblocks, stride, out);
+}
+
+void volumes(Prec pr, const Launch& L, void* vol, hipStream_t s) {
+    if (L.n <= 0) return;
+    const unsigned nb = (unsigned)((L.n + 255) / 256);
+    if (pr == PREC_F32) {
+        KP<float> k = make_kp<float>(L, 0);
+        if (L.ph.order == 3) hipLaunchKernelGGL((k_volumes<float, 3>), dim3(nb), dim3(256), 0, s, k, (float*)vol);
+        else hipLaunchKernelGGL((k_volumes<float, 2>), dim3(nb), dim3(256), 0, s, k, (float*)vol);
+    } else {
+        KP<double> k = make_kp<double>(L, 0);
+        if (L.ph.order == 3) hipLaunchKernelGGL((k_volumes<double, 3>), dim3(nb), dim3(256), 0, s, k, (double*)vol);
+        else hipLaunchKernelGGL((k_volumes<double, 2>), dim3(nb), dim3(256), 0, s, k, (double*)vol);
+    }
+}
+
+void sqnorm9(Prec pr, const void* p, int n, long long ld, double* partial, int nblocks, hipStream_t s) {
+    if (pr == PREC_F32) hipLaunchKernelGGL(k_sqnorm9<float>, dim3(nblocks), dim3(256), 0, s, (const float*)p, n, ld, partial);
+    else hipLaunchKernelGGL(k_sqnorm9<double>, dim3(nblocks), dim3(256), 0, s, (const double*)p, n, ld, partial);
+}
+
+void adam(Prec pr, const void* dLdF, void* dFc, void* mom, void* vec, void* vmax, int n, long long ld,
+          const double* sqn, double alpha, double beta1, double beta2, double eps, int timestep, hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (pr == PREC_F32)
+        hipLaunchKernelGGL(k_adam<float>, dim3(nb), dim3(256), 0, s, (const float*)dLdF, (float*)dFc, (float*)mom, (float*)vec, (float*)vmax, n, ld, sqn, (float)alpha, (float)beta1, (float)beta2, (float)eps, timestep);
+    else
+        hipLaunchKernelGGL(k_adam<double>, dim3(nb), dim3(256), 0, s, (const double*)dLdF, (double*)dFc, (double*)mom, (double*)vec, (double*)vmax, n, ld, sqn, alpha, beta1, beta2, eps, timestep);
+}
+
+}  // namespace dmpm

@@ -1,0 +1,1309 @@
+// Host runtime behind the C ABI (include/dmpm.h).
+//
+// A dmpm_sim is the device-resident counterpart of the reference's CompGraph
+// (CompGraph.h:20-99): a sequence of layers (particle states), the grid
+// scratch of one step, the adjoint carry, per-layer controls and Adam state.
+// Unlike the reference's store-every-step tape (one full PointCloud and Grid
+// per layer, CompGraph.cpp:30-40), only every K-th layer (and the last) is
+// stored; the others are recomputed segment by segment during the backward
+// pass.  The grid is never stored: each backward step re-runs that step's P2G.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dmpm.h"
+#include "dmpm_kernels.h"
+
+using namespace dmpm;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw Error(DMPM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        f();
+        return DMPM_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return DMPM_ERR_ARG;
+    }
+}
+
+// Process-wide Adam step counter of OptimizeSingleTimestep, which the
+// reference keeps in a function-local static (CompGraph.cpp:396).
+int g_single_timestep_adam_t = 0;
+
+long long round_up(long long n, long long a) { return (n + a - 1) / a * a; }
+
+}  // namespace
+
+struct dmpm_sim {
+    dmpm_config cfg{};
+    Phys ph{};
+    Prec pr = PREC_F32;
+    size_t esz = 4;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+
+    int n = 0;
+    long long ld = 0;
+    long long G = 0, gld = 0;
+    int L = 1;
+    int K = 1;
+
+    // device memory bookkeeping
+    size_t mem_cur = 0, mem_peak = 0;
+    std::vector<std::pair<void*, size_t>> allocs;
+
+    int* perm = nullptr;   // internal index -> caller index
+    void* stat = nullptr;  // m, vol, lam, mu
+    std::vector<void*> layer;  // stored layer states (nullptr = not stored)
+    std::vector<char> valid;   // layer computed (stored ones only)
+    std::vector<void*> ctl;    // dFc per layer (nullptr = zero)
+    std::vector<void*> am, av, avmax;  // Adam moments per layer (PointCloud momentum/vector/vector_max)
+    std::vector<char> want_gctl;
+    std::vector<void*> gctl;   // dL/d dFc per layer
+    std::vector<void*> seg;    // K segment buffers (recomputed states)
+    int seg_base = -1;
+    void* scratch[2] = {nullptr, nullptr};
+    void* grid = nullptr;  // m, p (4 planes)
+    void* ggv = nullptr;   // 3 planes
+    void* carry[2] = {nullptr, nullptr};
+    int carry_cur = 0;
+    int carry_layer = -1;   // layer whose adjoint the current carry holds
+    void* adj_last = nullptr;  // adjoint of the last layer (loss / render / seed)
+    void* gctl_stop = nullptr; // dL/d dFc of the control layer of the last backward
+    void* target_m = nullptr;
+    void* dldm = nullptr;
+    double* partial = nullptr;  // reduction partials
+    double* red = nullptr;      // reduction outputs (8 doubles)
+    int red_blocks = 1024;
+    // render gradients, internal order, planes: F (9) + x (3)
+    void* rg = nullptr;
+    bool has_rg = false;
+    int rg_n = 0;
+    double rg_normF = 0, rg_normx = 0;
+    // timing
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double fwd_ms = 0, bwd_ms = 0;
+    // per-kernel event timing (dmpm_kernel_timer): pairs of events per launch
+    static constexpr int NKIND = 5;  // p2g, g2p, adj_scatter, adj_gather, grid clear
+    bool ktimer = false;
+    std::vector<hipEvent_t> kpool;
+    size_t knext = 0;
+    std::vector<std::pair<int, size_t>> krec;  // (kind, index of start event)
+    int koverflow = 0;
+
+    ~dmpm_sim() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& a : allocs) (void)hipFree(a.first);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : kpool) (void)hipEventDestroy(e);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+
+    // ---- memory ----
+    void* dalloc(size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(DMPM_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                            hipGetErrorString(e));
+        }
+        allocs.emplace_back(p, bytes);
+        mem_cur += bytes;
+        mem_peak = std::max(mem_peak, mem_cur);
+        return p;
+    }
+    void dfree(void*& p) {
+        if (!p) return;
+        for (size_t i = 0; i < allocs.size(); ++i)
+            if (allocs[i].first == p) {
+                mem_cur -= allocs[i].second;
+                allocs.erase(allocs.begin() + i);
+                break;
+            }
+        (void)hipFree(p);
+        p = nullptr;
+    }
+    size_t state_bytes() const { return (size_t)NSTATE * ld * esz; }
+    size_t ctl_bytes() const { return (size_t)9 * ld * esz; }
+    void zero(void* p, long long elems) { fill_zero(pr, p, elems, stream); }
+
+    // ---- launch descriptor ----
+    Launch launch() const {
+        Launch Lc{};
+        Lc.ph = ph;
+        Lc.n = n;
+        Lc.ld = ld;
+        Lc.gld = gld;
+        Lc.lds_bytes = cfg.lds_bytes > 0 ? cfg.lds_bytes : 48 * 1024;
+        Lc.block = cfg.block > 0 ? cfg.block : 256;
+        return Lc;
+    }
+
+    // Run `launch` between two pooled events when the kernel timer is on.
+    template <typename Fn>
+    void timed(int kind, Fn&& launch) {
+        if (!ktimer) {
+            launch();
+            return;
+        }
+        if (knext + 2 > kpool.size()) {
+            ++koverflow;
+            launch();
+            return;
+        }
+        const size_t i = knext;
+        knext += 2;
+        HIPCHK(hipEventRecord(kpool[i], stream));
+        launch();
+        HIPCHK(hipEventRecord(kpool[i + 1], stream));
+        krec.emplace_back(kind, i);
+    }
+
+    bool stored(int t) const { return K <= 1 || t % K == 0 || t == L - 1; }
+
+    // (Re)allocate layer storage for L layers.
+    void alloc_layers() {
+        for (int t = 1; t < (int)layer.size(); ++t) dfree(layer[t]);
+        // seg[0] aliases a stored layer; only seg[1..] are owned
+        for (size_t j = 1; j < seg.size(); ++j) dfree(seg[j]);
+        seg.clear();
+        seg_base = -1;
+        layer.resize(L, nullptr);
+        valid.assign(L, 0);
+        valid[0] = 1;
+        for (int t = 1; t < L; ++t)
+            if (stored(t)) layer[t] = dalloc(state_bytes());
+        if (K > 1) {
+            seg.resize(K, nullptr);
+            for (int j = 1; j < K; ++j) seg[j] = dalloc(state_bytes());
+        }
+        for (auto& p : scratch)
+            if (!p) p = dalloc(state_bytes());
+    }
+
+    // Device pointer of the state of layer t, recomputing it from the nearest
+    // stored layer below when it is not stored.
+    const void* state(int t) {
+        if (stored(t)) {
+            if (!valid[t]) throw Error(DMPM_ERR_STATE, "layer " + std::to_string(t) + " has not been computed");
+            return layer[t];
+        }
+        const int c = (t / K) * K;
+        if (seg_base != c) {
+            if (!valid[c]) throw Error(DMPM_ERR_STATE, "checkpoint layer " + std::to_string(c) + " missing");
+            seg[0] = layer[c];
+            const int last = std::min(c + K - 1, L - 2);
+            for (int j = c; j < last; ++j) step(seg[j - c], j, seg[j - c + 1]);
+            seg_base = c;
+        }
+        return seg[t - c];
+    }
+
+    // One forward step: state `in` of layer t -> `out`.
+    void step(const void* in, int t, void* out) {
+        Launch Lc = launch();
+        Lc.st = in;
+        Lc.stat = stat;
+        Lc.ctl = ctl[t];
+        Lc.grid = grid;
+        Lc.out = out;
+        timed(4, [&] { zero(grid, 4 * gld); });
+        timed(0, [&] { p2g(pr, Lc, false, stream); });
+        timed(1, [&] { g2p(pr, Lc, stream); });
+    }
+
+    void forward(int start, bool zero_controls = true) {
+        if (start < 0 || start >= L) throw Error(DMPM_ERR_INDEX, "start layer out of range");
+        const void* cur = state(start);
+        // The reference zeroes dFc of every layer it writes (ForwardSimulation.cpp:156);
+        // a rollout with per-step controls keeps them.
+        if (zero_controls)
+            for (int t = start + 1; t < L; ++t) dfree(ctl[t]);
+        if (start + 1 < L) {
+            HIPCHK(hipEventRecord(ev[0], stream));
+            int sc = 0;
+            for (int t = start; t < L - 1; ++t) {
+                void* out = stored(t + 1) ? layer[t + 1] : scratch[sc];
+                if (!stored(t + 1)) sc ^= 1;
+                step(cur, t, out);
+                if (stored(t + 1)) valid[t + 1] = 1;
+                cur = out;
+            }
+            HIPCHK(hipEventRecord(ev[1], stream));
+        }
+        seg_base = -1;
+    }
+
+    void backward(int stop) {
+        if (stop < 0 || stop > L - 1) throw Error(DMPM_ERR_INDEX, "control layer out of range");
+        // render gradients are added to the last layer on every call (CompGraph.cpp:148-197)
+        if (has_rg && rg_n == n) {
+            accumulate_planes(rg, PF, 9);
+            accumulate_planes((char*)rg + 9 * ld * esz, PX, 3);
+        } else if (has_rg) {
+            fprintf(stderr, "[WARN] Render gradient size mismatch: stored %d but layer has %d points. Skipping injection.\n",
+                    rg_n, n);
+        }
+        copy_elems(pr, carry[carry_cur], adj_last, NSTATE * ld, stream);
+        carry_layer = L - 1;
+        HIPCHK(hipEventRecord(ev[2], stream));
+        for (int t = L - 2; t >= stop; --t) {
+            const void* st = state(t);
+            Launch Lc = launch();
+            Lc.st = st;
+            Lc.stat = stat;
+            Lc.ctl = ctl[t];
+            Lc.grid = grid;
+            Lc.ggv = ggv;
+            Lc.adj1 = carry[carry_cur];
+            Lc.adj0 = carry[carry_cur ^ 1];
+            void* gout = nullptr;
+            if (t == stop) gout = gctl_stop;
+            else if (want_gctl[t]) {
+                if (!gctl[t]) gctl[t] = dalloc(ctl_bytes());
+                gout = gctl[t];
+            }
+            Lc.gctl = gout;
+            timed(4, [&] { zero(grid, 4 * gld); });
+            timed(0, [&] { p2g(pr, Lc, false, stream); });
+            timed(4, [&] { zero(ggv, 3 * gld); });
+            timed(2, [&] { adj_scatter(pr, Lc, stream); });
+            timed(3, [&] { adj_gather(pr, Lc, stream); });
+            if (t == stop && want_gctl[t]) {
+                if (!gctl[t]) gctl[t] = dalloc(ctl_bytes());
+                copy_elems(pr, gctl[t], gctl_stop, 9 * ld, stream);
+            }
+            carry_cur ^= 1;
+            carry_layer = t;
+        }
+        HIPCHK(hipEventRecord(ev[3], stream));
+        if (stop == L - 1) {
+            // no step: dL/d dFc of the last layer is dL/dF
+            copy_elems(pr, gctl_stop, (char*)carry[carry_cur] + PF * ld * esz, 9 * ld, stream);
+        }
+    }
+
+    // adj_last[plane_off..] += src (internal-order planes)
+    void accumulate_planes(const void* src, int plane_off, int nc) {
+        // src holds internal-order planes; add them plane by plane (identity permutation)
+        for (int c = 0; c < nc; ++c)
+            accumulate_carry(pr, (const char*)src + (size_t)c * ld * esz, pr == PREC_F64, adj_last,
+                             plane_off + c, 1, nullptr, n, ld, stream);
+    }
+
+    // Upload a caller array (n x nc, caller order) into a device staging
+    // buffer of the caller's dtype; returns a device pointer valid until the
+    // next call.
+    std::vector<void*> staging;
+    const void* stage(const void* src, size_t bytes, int on_device) {
+        if (on_device || !src) return src;
+        void* d = dalloc(bytes);
+        staging.push_back(d);
+        HIPCHK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, stream));
+        return d;
+    }
+    void unstage() {
+        if (staging.empty()) return;
+        HIPCHK(hipStreamSynchronize(stream));
+        for (auto& p : staging) dfree(p);
+        staging.clear();
+    }
+};
+
+namespace {
+
+void check_sim(const dmpm_sim* s) {
+    if (!s) throw Error(DMPM_ERR_ARG, "null sim");
+}
+
+Phys phys_of(const dmpm_config& c) {
+    Phys p{};
+    for (int i = 0; i < 3; ++i) {
+        p.dims[i] = c.dims[i];
+        p.grid_min[i] = c.grid_min[i];
+        p.f_ext[i] = c.f_ext[i];
+    }
+    p.dx = c.dx;
+    p.dt = c.dt;
+    p.drag = c.drag;
+    p.smoothing = c.smoothing;
+    p.zeta = c.zeta;
+    p.jmin = c.jmin;
+    p.order = c.order;
+    p.model = c.model;
+    p.trust_region = c.trust_region;
+    p.wall_cells = c.wall_cells;
+    p.adjoint = c.adjoint;
+    return p;
+}
+
+void validate(const dmpm_config& c) {
+    for (int i = 0; i < 3; ++i)
+        if (c.dims[i] <= 0) throw Error(DMPM_ERR_ARG, "grid dims must be positive");
+    if (!(c.dx > 0)) throw Error(DMPM_ERR_ARG, "dx must be positive");
+    if (c.order != 2 && c.order != 3) throw Error(DMPM_ERR_ARG, "order must be 2 or 3");
+    if (c.model != 0 && c.model != 1) throw Error(DMPM_ERR_ARG, "model must be 0 (FCR) or 1 (Neo-Hookean)");
+    if (c.adjoint != 0 && c.adjoint != 1) throw Error(DMPM_ERR_ARG, "adjoint must be 0 or 1");
+    if (c.precision != 0 && c.precision != 1) throw Error(DMPM_ERR_ARG, "precision must be 0 or 1");
+    if (c.checkpoint_interval < 0) throw Error(DMPM_ERR_ARG, "checkpoint_interval must be >= 0");
+    if (c.block != 0 && (c.block % 64 != 0 || c.block > 1024)) throw Error(DMPM_ERR_ARG, "block must be a multiple of 64, <= 1024");
+}
+
+// Morton key of a cell (10 bits per axis).
+uint32_t spread10(uint32_t v) {
+    v &= 1023u;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+size_t dsize(int dtype) { return dtype == DMPM_DTYPE_F64 ? 8 : 4; }
+
+double host_get(const void* p, int dtype, size_t i) {
+    return dtype == DMPM_DTYPE_F64 ? ((const double*)p)[i] : (double)((const float*)p)[i];
+}
+
+// Sum of partials -> host double(s) (synchronises).
+void reduce_to_host(dmpm_sim* s, int nblocks, int stride, double* out) {
+    reduce_partials(s->partial, nblocks, stride, s->red, s->stream);
+    HIPCHK(hipMemcpyAsync(out, s->red, stride * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+}
+
+int red_blocks_for(long long count) {
+    long long nb = (count + 255) / 256;
+    return (int)std::max(1LL, std::min(nb, 1024LL));
+}
+
+double end_layer_mass_loss(dmpm_sim* s) {
+    const void* st = s->state(s->L - 1);
+    Launch Lc = s->launch();
+    Lc.st = st;
+    Lc.stat = s->stat;
+    Lc.grid = s->grid;
+    s->zero(s->grid, s->gld);
+    p2g(s->pr, Lc, true, s->stream);
+    const int nb = red_blocks_for(s->G);
+    mass_sums(s->pr, s->grid, s->target_m, s->G, s->partial, nb, s->stream);
+    reduce_partials(s->partial, nb, 2, s->red, s->stream);
+    mass_loss_nodes(s->pr, s->grid, s->target_m, s->G, s->red, nb, s->dldm, s->partial + 2 * 1024, nb, s->stream);
+    reduce_partials(s->partial + 2 * 1024, nb, 1, s->red + 2, s->stream);
+    Lc.adj0 = s->adj_last;
+    mass_loss_particles(s->pr, Lc, s->dldm, s->target_m, s->stream);
+    double loss = 0;
+    HIPCHK(hipMemcpyAsync(&loss, s->red + 2, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return loss;
+}
+
+double control_norm(dmpm_sim* s) {
+    const int nb = red_blocks_for(s->n);
+    sqnorm9(s->pr, s->gctl_stop, s->n, s->ld, s->partial, nb, s->stream);
+    double sq = 0;
+    reduce_to_host(s, nb, 1, &sq);
+    return std::sqrt(sq);
+}
+
+void ensure_adam(dmpm_sim* s, int t) {
+    for (auto* v : {&s->am, &s->av, &s->avmax}) {
+        if (!(*v)[t]) {
+            (*v)[t] = s->dalloc(s->ctl_bytes());
+            s->zero((*v)[t], 9 * s->ld);
+        }
+    }
+    if (!s->ctl[t]) {
+        s->ctl[t] = s->dalloc(s->ctl_bytes());
+        s->zero(s->ctl[t], 9 * s->ld);
+    }
+}
+
+void descend_adam(dmpm_sim* s, int t, double alpha, double grad_norm, double b1, double b2, double eps, int ts) {
+    ensure_adam(s, t);
+    // gradient_norm as passed by the caller: Descend_Adam only uses 1/max(gn,1e-12)
+    double sq = grad_norm * grad_norm;
+    HIPCHK(hipMemcpyAsync(s->red + 4, &sq, sizeof(double), hipMemcpyHostToDevice, s->stream));
+    adam(s->pr, s->gctl_stop, s->ctl[t], s->am[t], s->av[t], s->avmax[t], s->n, s->ld, s->red + 4, alpha, b1, b2,
+         eps, ts, s->stream);
+    HIPCHK(hipStreamSynchronize(s->stream));
+}
+
+struct Backup {
+    void* buf = nullptr;
+};
+
+// OptimizeSingleTimestep's line search state backup (CompGraph.cpp:384-391).
+void backup_layer(dmpm_sim* s, int t, void* buf) {
+    const long long e = 9 * s->ld;
+    copy_elems(s->pr, (char*)buf + 0 * e * s->esz, s->ctl[t], e, s->stream);
+    copy_elems(s->pr, (char*)buf + 1 * e * s->esz, s->am[t], e, s->stream);
+    copy_elems(s->pr, (char*)buf + 2 * e * s->esz, s->av[t], e, s->stream);
+    copy_elems(s->pr, (char*)buf + 3 * e * s->esz, s->avmax[t], e, s->stream);
+}
+void restore_layer(dmpm_sim* s, int t, const void* buf) {
+    const long long e = 9 * s->ld;
+    copy_elems(s->pr, s->ctl[t], (const char*)buf + 0 * e * s->esz, e, s->stream);
+    copy_elems(s->pr, s->am[t], (const char*)buf + 1 * e * s->esz, e, s->stream);
+    copy_elems(s->pr, s->av[t], (const char*)buf + 2 * e * s->esz, e, s->stream);
+    copy_elems(s->pr, s->avmax[t], (const char*)buf + 3 * e * s->esz, e, s->stream);
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+void dmpm_config_init(dmpm_config* c, int nx, int ny, int nz, double dx, const double grid_min[3]) {
+    std::memset(c, 0, sizeof(*c));
+    c->dims[0] = nx;
+    c->dims[1] = ny;
+    c->dims[2] = nz;
+    c->dx = dx;
+    for (int i = 0; i < 3; ++i) c->grid_min[i] = grid_min ? grid_min[i] : 0.0;
+    c->dt = 1.0 / 120.0;
+    c->drag = 0.5;
+    c->smoothing = 0.1;
+    c->zeta = 0.25;
+    c->jmin = 0.6;
+    c->order = 3;
+    c->model = 0;
+    c->trust_region = 1;
+    c->wall_cells = 0;
+    c->adjoint = 1;
+    c->precision = DMPM_PREC_F32;
+    c->checkpoint_interval = 1;
+}
+
+const char* dmpm_last_error(void) { return g_err.c_str(); }
+const char* dmpm_version(void) { return "dmpm 0.1 (gfx950)"; }
+
+int dmpm_create(const dmpm_config* cfg, dmpm_sim** out) {
+    return guard([&] {
+        if (!cfg || !out) throw Error(DMPM_ERR_ARG, "null argument");
+        validate(*cfg);
+        auto s = std::make_unique<dmpm_sim>();
+        s->cfg = *cfg;
+        s->ph = phys_of(*cfg);
+        s->pr = cfg->precision == DMPM_PREC_F64 ? PREC_F64 : PREC_F32;
+        s->esz = s->pr == PREC_F64 ? 8 : 4;
+        s->K = std::max(1, cfg->checkpoint_interval);
+        HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        s->own_stream = true;
+        for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
+        s->G = (long long)cfg->dims[0] * cfg->dims[1] * cfg->dims[2];
+        s->gld = round_up(s->G, 64);
+        s->grid = s->dalloc(4 * s->gld * s->esz);
+        s->ggv = s->dalloc(3 * s->gld * s->esz);
+        s->target_m = s->dalloc(s->gld * s->esz);
+        s->dldm = s->dalloc(s->gld * s->esz);
+        s->zero(s->target_m, s->gld);
+        s->partial = (double*)s->dalloc(4 * 1024 * sizeof(double));
+        s->red = (double*)s->dalloc(8 * sizeof(double));
+        *out = s.release();
+    });
+}
+
+int dmpm_destroy(dmpm_sim* sim) {
+    return guard([&] { delete sim; });
+}
+
+int dmpm_set_stream(dmpm_sim* s, void* stream) {
+    return guard([&] {
+        check_sim(s);
+        HIPCHK(hipStreamSynchronize(s->stream));
+        if (stream) {
+            if (s->own_stream) HIPCHK(hipStreamDestroy(s->stream));
+            s->stream = (hipStream_t)stream;
+            s->own_stream = false;
+        }
+    });
+}
+
+int dmpm_synchronize(dmpm_sim* s) {
+    return guard([&] {
+        check_sim(s);
+        HIPCHK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const void* F, const void* C,
+                       const void* dFc, const void* m, const void* vol, const void* lam, const void* mu, int dtype,
+                       int on_device) {
+    return guard([&] {
+        check_sim(s);
+        if (n <= 0) throw Error(DMPM_ERR_ARG, "n must be positive");
+        if (!x || !v || !F || !C || !m || !vol || !lam || !mu) throw Error(DMPM_ERR_ARG, "null particle array");
+        const size_t es = dsize(dtype);
+        // Morton order of the cell of each particle (host)
+        std::vector<char> xh((size_t)n * 3 * es);
+        if (on_device) HIPCHK(hipMemcpy(xh.data(), x, xh.size(), hipMemcpyDeviceToHost));
+        else std::memcpy(xh.data(), x, xh.size());
+        std::vector<uint64_t> key(n);
+        for (int i = 0; i < n; ++i) {
+            uint32_t c[3];
+            for (int a = 0; a < 3; ++a) {
+                double f = (host_get(xh.data(), dtype, 3 * (size_t)i + a) - s->cfg.grid_min[a]) / s->cfg.dx;
+                long long ci = (long long)std::floor(f);
+                ci = std::max(0LL, std::min(ci, 1023LL));
+                c[a] = (uint32_t)ci;
+            }
+            key[i] = ((uint64_t)((spread10(c[0]) << 2) | (spread10(c[1]) << 1) | spread10(c[2])) << 32) | (uint32_t)i;
+        }
+        std::sort(key.begin(), key.end());
+        std::vector<int> perm(n);
+        for (int i = 0; i < n; ++i) perm[i] = (int)(key[i] & 0xffffffffu);
+
+        // (re)allocate particle storage
+        for (auto& p : s->layer) s->dfree(p);
+        for (auto& p : s->ctl) s->dfree(p);
+        for (auto* v2 : {&s->am, &s->av, &s->avmax, &s->gctl})
+            for (auto& p : *v2) s->dfree(p);
+        for (size_t j = 1; j < s->seg.size(); ++j) s->dfree(s->seg[j]);
+        for (auto& p : s->scratch) s->dfree(p);
+        s->seg.clear();
+        s->dfree(s->stat);
+        s->dfree((void*&)s->perm);
+        for (auto& p : s->carry) s->dfree(p);
+        s->dfree(s->adj_last);
+        s->dfree(s->gctl_stop);
+        s->dfree(s->rg);
+        s->has_rg = false;
+
+        s->n = n;
+        s->ld = round_up(n, 64);
+        s->L = 1;
+        s->layer.assign(1, nullptr);
+        s->valid.assign(1, 1);
+        s->ctl.assign(1, nullptr);
+        s->am.assign(1, nullptr);
+        s->av.assign(1, nullptr);
+        s->avmax.assign(1, nullptr);
+        s->gctl.assign(1, nullptr);
+        s->want_gctl.assign(1, 0);
+        s->perm = (int*)s->dalloc(n * sizeof(int));
+        HIPCHK(hipMemcpyAsync(s->perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        s->layer[0] = s->dalloc(s->state_bytes());
+        s->stat = s->dalloc((size_t)NSTAT * s->ld * s->esz);
+        for (auto& p : s->carry) p = s->dalloc(s->state_bytes());
+        s->adj_last = s->dalloc(s->state_bytes());
+        s->zero(s->adj_last, NSTATE * s->ld);
+        s->gctl_stop = s->dalloc(s->ctl_bytes());
+        s->zero(s->gctl_stop, 9 * s->ld);
+        s->zero(s->layer[0], NSTATE * s->ld);
+        s->zero(s->stat, NSTAT * s->ld);
+        const int f64 = dtype == DMPM_DTYPE_F64;
+        auto imp = [&](const void* src, int nc, void* dst) {
+            const void* d = s->stage(src, (size_t)n * nc * es, on_device);
+            import_planes(s->pr, d, f64, dst, s->perm, n, nc, s->ld, s->stream);
+        };
+        char* L0 = (char*)s->layer[0];
+        imp(x, 3, L0 + PX * s->ld * s->esz);
+        imp(v, 3, L0 + PV * s->ld * s->esz);
+        imp(F, 9, L0 + PF * s->ld * s->esz);
+        imp(C, 9, L0 + PC * s->ld * s->esz);
+        char* S = (char*)s->stat;
+        imp(m, 1, S + SM * s->ld * s->esz);
+        imp(vol, 1, S + SVOL * s->ld * s->esz);
+        imp(lam, 1, S + SLAM * s->ld * s->esz);
+        imp(mu, 1, S + SMU * s->ld * s->esz);
+        if (dFc) {
+            s->ctl[0] = s->dalloc(s->ctl_bytes());
+            imp(dFc, 9, s->ctl[0]);
+        }
+        s->unstage();
+        s->alloc_layers();
+    });
+}
+
+int dmpm_num_particles(const dmpm_sim* s, int* n) {
+    return guard([&] {
+        check_sim(s);
+        *n = s->n;
+    });
+}
+
+int dmpm_set_target_grid_mass(dmpm_sim* s, const void* m, int dtype, int on_device) {
+    return guard([&] {
+        check_sim(s);
+        if (!m) throw Error(DMPM_ERR_ARG, "null target mass");
+        const void* d = s->stage(m, (size_t)s->G * dsize(dtype), on_device);
+        import_planes(s->pr, d, dtype == DMPM_DTYPE_F64, s->target_m, nullptr, (int)s->G, 1, s->gld, s->stream);
+        s->unstage();
+    });
+}
+
+int dmpm_set_sim_params(dmpm_sim* s, double dt, double drag, double smoothing, const double f_ext[3]) {
+    return guard([&] {
+        check_sim(s);
+        s->cfg.dt = dt;
+        s->cfg.drag = drag;
+        s->cfg.smoothing = smoothing;
+        if (f_ext)
+            for (int i = 0; i < 3; ++i) s->cfg.f_ext[i] = f_ext[i];
+        s->ph = phys_of(s->cfg);
+        s->seg_base = -1;
+    });
+}
+
+int dmpm_set_up_comp_graph(dmpm_sim* s, int num_layers) {
+    return guard([&] {
+        check_sim(s);
+        if (s->n <= 0) throw Error(DMPM_ERR_STATE, "no particles");
+        if (num_layers <= 0) throw Error(DMPM_ERR_ARG, "num_layers must be positive");
+        for (int t = 1; t < (int)s->ctl.size(); ++t) {
+            s->dfree(s->ctl[t]);
+            s->dfree(s->am[t]);
+            s->dfree(s->av[t]);
+            s->dfree(s->avmax[t]);
+            s->dfree(s->gctl[t]);
+        }
+        s->L = num_layers;
+        s->ctl.resize(s->L, nullptr);
+        s->am.resize(s->L, nullptr);
+        s->av.resize(s->L, nullptr);
+        s->avmax.resize(s->L, nullptr);
+        s->gctl.resize(s->L, nullptr);
+        s->want_gctl.resize(s->L, 0);
+        s->alloc_layers();
+        // every layer is a copy of layer 0, controls and Adam state included
+        for (int t = 1; t < s->L; ++t) {
+            if (s->layer[t]) {
+                copy_elems(s->pr, s->layer[t], s->layer[0], NSTATE * s->ld, s->stream);
+                s->valid[t] = 1;
+            }
+            auto dup = [&](std::vector<void*>& v) {
+                if (v[0]) {
+                    v[t] = s->dalloc(s->ctl_bytes());
+                    copy_elems(s->pr, v[t], v[0], 9 * s->ld, s->stream);
+                }
+            };
+            dup(s->ctl);
+            dup(s->am);
+            dup(s->av);
+            dup(s->avmax);
+        }
+        HIPCHK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int dmpm_get_num_layers(const dmpm_sim* s, int* num_layers) {
+    return guard([&] {
+        check_sim(s);
+        *num_layers = s->L;
+    });
+}
+
+int dmpm_compute_forward_pass(dmpm_sim* s, int start_layer) {
+    return guard([&] {
+        check_sim(s);
+        if (s->n <= 0) throw Error(DMPM_ERR_STATE, "no particles");
+        s->forward(start_layer);
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int dmpm_rollout_forward(dmpm_sim* s, int start_layer) {
+    return guard([&] {
+        check_sim(s);
+        if (s->n <= 0) throw Error(DMPM_ERR_STATE, "no particles");
+        s->forward(start_layer, false);
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int dmpm_end_layer_mass_loss(dmpm_sim* s, double* loss) {
+    return guard([&] {
+        check_sim(s);
+        if (s->n <= 0) throw Error(DMPM_ERR_STATE, "no particles");
+        double l = end_layer_mass_loss(s);
+        if (loss) *loss = l;
+    });
+}
+
+int dmpm_compute_backward_pass(dmpm_sim* s, int control_layer) {
+    return guard([&] {
+        check_sim(s);
+        if (s->n <= 0) throw Error(DMPM_ERR_STATE, "no particles");
+        s->backward(control_layer);
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int dmpm_get_layer(dmpm_sim* s, int layer, int field, void* out, int dtype, int on_device) {
+    return guard([&] {
+        check_sim(s);
+        if (layer < 0 || layer >= s->L) throw Error(DMPM_ERR_INDEX, "Layer index out of range.");
+        if (!out) throw Error(DMPM_ERR_ARG, "null output");
+        int nc = 0, plane = 0;
+        const void* base = nullptr;
+        const size_t es = dsize(dtype);
+        switch (field) {
+            case DMPM_FIELD_X: nc = 3; plane = PX; break;
+            case DMPM_FIELD_V: nc = 3; plane = PV; break;
+            case DMPM_FIELD_F: nc = 9; plane = PF; break;
+            case DMPM_FIELD_C: nc = 9; plane = PC; break;
+            case DMPM_FIELD_DFC: nc = 9; break;
+            case DMPM_FIELD_FTOTAL: nc = 9; break;
+            case DMPM_FIELD_M: nc = 1; plane = SM; base = s->stat; break;
+            case DMPM_FIELD_VOL: nc = 1; plane = SVOL; base = s->stat; break;
+            case DMPM_FIELD_LAM: nc = 1; plane = SLAM; base = s->stat; break;
+            case DMPM_FIELD_MU: nc = 1; plane = SMU; base = s->stat; break;
+            default: throw Error(DMPM_ERR_ARG, "unknown field");
+        }
+        const size_t bytes = (size_t)s->n * nc * es;
+        void* dst = on_device ? out : s->dalloc(bytes);
+        const int f64 = dtype == DMPM_DTYPE_F64;
+        if (field == DMPM_FIELD_DFC) {
+            if (s->ctl[layer]) export_planes(s->pr, s->ctl[layer], dst, f64, s->perm, s->n, 9, s->ld, s->stream);
+            else fill_zero(dtype == DMPM_DTYPE_F64 ? PREC_F64 : PREC_F32, dst, (long long)s->n * 9, s->stream);
+        } else if (field == DMPM_FIELD_FTOTAL) {
+            export_F_total(s->pr, s->state(layer), s->ctl[layer], dst, f64, s->perm, s->n, s->ld, s->stream);
+        } else {
+            if (!base) base = s->state(layer);
+            export_planes(s->pr, (const char*)base + plane * s->ld * s->esz, dst, f64, s->perm, s->n, nc, s->ld,
+                          s->stream);
+        }
+        if (!on_device) {
+            HIPCHK(hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, s->stream));
+            HIPCHK(hipStreamSynchronize(s->stream));
+            s->dfree(dst);
+        }
+    });
+}
+
+int dmpm_get_layer_grad(dmpm_sim* s, int layer, int field, void* out, int dtype, int on_device) {
+    return guard([&] {
+        check_sim(s);
+        if (layer < 0 || layer >= s->L) throw Error(DMPM_ERR_INDEX, "Layer index out of range.");
+        const void* src = nullptr;
+        int nc = 0, plane = 0;
+        switch (field) {
+            case DMPM_FIELD_X: nc = 3; plane = PX; break;
+            case DMPM_FIELD_V: nc = 3; plane = PV; break;
+            case DMPM_FIELD_F: nc = 9; plane = PF; break;
+            case DMPM_FIELD_C: nc = 9; plane = PC; break;
+            case DMPM_FIELD_DFC: nc = 9; break;
+            default: throw Error(DMPM_ERR_ARG, "unknown gradient field");
+        }
+        if (field == DMPM_FIELD_DFC) {
+            if (layer != s->carry_layer) throw Error(DMPM_ERR_STATE, "no control gradient for this layer");
+            src = s->gctl_stop;
+        } else if (layer == s->carry_layer) {
+            src = (const char*)s->carry[s->carry_cur] + plane * s->ld * s->esz;
+        } else if (layer == s->L - 1) {
+            src = (const char*)s->adj_last + plane * s->ld * s->esz;
+        } else {
+            throw Error(DMPM_ERR_STATE, "gradients are kept for the control layer and the last layer only");
+        }
+        const size_t bytes = (size_t)s->n * nc * dsize(dtype);
+        void* dst = on_device ? out : s->dalloc(bytes);
+        export_planes(s->pr, src, dst, dtype == DMPM_DTYPE_F64, s->perm, s->n, nc, s->ld, s->stream);
+        if (!on_device) {
+            HIPCHK(hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, s->stream));
+            HIPCHK(hipStreamSynchronize(s->stream));
+            s->dfree(dst);
+        }
+    });
+}
+
+int dmpm_accumulate_last_layer_grad(dmpm_sim* s, const void* dLdF, const void* dLdx, const void* dLdv,
+                                    const void* dLdC, int dtype, int on_device) {
+    return guard([&] {
+        check_sim(s);
+        const size_t es = dsize(dtype);
+        const int f64 = dtype == DMPM_DTYPE_F64;
+        auto acc = [&](const void* src, int off, int nc) {
+            if (!src) return;
+            const void* d = s->stage(src, (size_t)s->n * nc * es, on_device);
+            accumulate_carry(s->pr, d, f64, s->adj_last, off, nc, s->perm, s->n, s->ld, s->stream);
+        };
+        acc(dLdF, PF, 9);
+        acc(dLdx, PX, 3);
+        acc(dLdv, PV, 3);
+        acc(dLdC, PC, 9);
+        s->unstage();
+    });
+}
+
+int dmpm_set_last_layer_grad(dmpm_sim* s, const void* dLdx, const void* dLdv, const void* dLdF, const void* dLdC,
+                             int dtype, int on_device) {
+    return guard([&] {
+        check_sim(s);
+        s->zero(s->adj_last, NSTATE * s->ld);
+        const size_t es = dsize(dtype);
+        const int f64 = dtype == DMPM_DTYPE_F64;
+        auto put = [&](const void* src, int off, int nc) {
+            if (!src) return;
+            const void* d = s->stage(src, (size_t)s->n * nc * es, on_device);
+            import_planes(s->pr, d, f64, (char*)s->adj_last + off * s->ld * s->esz, s->perm, s->n, nc, s->ld,
+                          s->stream);
+        };
+        put(dLdx, PX, 3);
+        put(dLdv, PV, 3);
+        put(dLdF, PF, 9);
+        put(dLdC, PC, 9);
+        s->unstage();
+    });
+}
+
+int dmpm_set_render_gradients(dmpm_sim* s, int n, const void* dLdF, const void* dLdx, int dtype, int on_device) {
+    return guard([&] {
+        check_sim(s);
+        if (!dLdF || !dLdx || n < 0) throw Error(DMPM_ERR_ARG, "bad render gradients");
+        const size_t es = dsize(dtype);
+        // norms (computed from the caller's data, bind.cpp:413-422)
+        std::vector<char> h((size_t)n * 12 * es);
+        if (on_device) {
+            HIPCHK(hipMemcpy(h.data(), dLdF, (size_t)n * 9 * es, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(h.data() + (size_t)n * 9 * es, dLdx, (size_t)n * 3 * es, hipMemcpyDeviceToHost));
+        } else {
+            std::memcpy(h.data(), dLdF, (size_t)n * 9 * es);
+            std::memcpy(h.data() + (size_t)n * 9 * es, dLdx, (size_t)n * 3 * es);
+        }
+        double nf = 0, nx = 0;
+        for (size_t i = 0; i < (size_t)n * 9; ++i) nf += host_get(h.data(), dtype, i) * host_get(h.data(), dtype, i);
+        for (size_t i = 0; i < (size_t)n * 3; ++i) {
+            double v = host_get(h.data() + (size_t)n * 9 * es, dtype, i);
+            nx += v * v;
+        }
+        s->rg_normF = std::sqrt(nf);
+        s->rg_normx = std::sqrt(nx);
+        s->rg_n = n;
+        s->has_rg = true;
+        s->dfree(s->rg);
+        if (n == s->n) {
+            s->rg = s->dalloc((size_t)12 * s->ld * s->esz);
+            const void* dF = s->stage(dLdF, (size_t)n * 9 * es, on_device);
+            const void* dX = s->stage(dLdx, (size_t)n * 3 * es, on_device);
+            import_planes(s->pr, dF, dtype == DMPM_DTYPE_F64, s->rg, s->perm, n, 9, s->ld, s->stream);
+            import_planes(s->pr, dX, dtype == DMPM_DTYPE_F64, (char*)s->rg + 9 * s->ld * s->esz, s->perm, n, 3,
+                          s->ld, s->stream);
+            s->unstage();
+        }
+    });
+}
+
+int dmpm_clear_render_gradients(dmpm_sim* s) {
+    return guard([&] {
+        check_sim(s);
+        s->dfree(s->rg);
+        s->has_rg = false;
+        s->rg_n = 0;
+        s->rg_normF = s->rg_normx = 0;
+    });
+}
+
+int dmpm_render_gradient_info(const dmpm_sim* s, int* has, int* num_points, double* nF, double* nx) {
+    return guard([&] {
+        check_sim(s);
+        if (has) *has = s->has_rg;
+        if (num_points) *num_points = s->rg_n;
+        if (nF) *nF = s->has_rg ? s->rg_normF : 0.0;
+        if (nx) *nx = s->has_rg ? s->rg_normx : 0.0;
+    });
+}
+
+int dmpm_promote_last_as_initial(dmpm_sim* s, int /*carry_grid*/) {
+    return guard([&] {
+        check_sim(s);
+        if (s->L <= 0) return;
+        const int last = s->L - 1;
+        if (last > 0) {
+            const void* st = s->state(last);
+            copy_elems(s->pr, s->layer[0], st, NSTATE * s->ld, s->stream);
+            auto mv = [&](std::vector<void*>& v) {
+                s->dfree(v[0]);
+                v[0] = v[last];
+                v[last] = nullptr;
+            };
+            mv(s->ctl);
+            mv(s->am);
+            mv(s->av);
+            mv(s->avmax);
+        }
+        for (int t = 1; t < s->L; ++t) {
+            s->dfree(s->ctl[t]);
+            s->dfree(s->am[t]);
+            s->dfree(s->av[t]);
+            s->dfree(s->avmax[t]);
+            s->dfree(s->gctl[t]);
+        }
+        s->L = 1;
+        s->ctl.resize(1);
+        s->am.resize(1);
+        s->av.resize(1);
+        s->avmax.resize(1);
+        s->gctl.resize(1);
+        s->want_gctl.resize(1);
+        s->alloc_layers();
+        s->carry_layer = -1;
+        HIPCHK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int dmpm_set_control(dmpm_sim* s, int layer, const void* dFc, int dtype, int on_device) {
+    return guard([&] {
+        check_sim(s);
+        if (layer < 0 || layer >= s->L) throw Error(DMPM_ERR_INDEX, "Layer index out of range.");
+        if (!dFc) {
+            s->dfree(s->ctl[layer]);
+            return;
+        }
+        if (!s->ctl[layer]) s->ctl[layer] = s->dalloc(s->ctl_bytes());
+        const void* d = s->stage(dFc, (size_t)s->n * 9 * dsize(dtype), on_device);
+        import_planes(s->pr, d, dtype == DMPM_DTYPE_F64, s->ctl[layer], s->perm, s->n, 9, s->ld, s->stream);
+        s->unstage();
+        s->seg_base = -1;
+    });
+}
+
+int dmpm_request_control_grad(dmpm_sim* s, int layer, int on) {
+    return guard([&] {
+        check_sim(s);
+        if (layer < 0 || layer >= s->L) throw Error(DMPM_ERR_INDEX, "Layer index out of range.");
+        s->want_gctl[layer] = on ? 1 : 0;
+        if (!on) s->dfree(s->gctl[layer]);
+    });
+}
+
+int dmpm_get_control_grad(dmpm_sim* s, int layer, void* out, int dtype, int on_device) {
+    return guard([&] {
+        check_sim(s);
+        if (layer < 0 || layer >= s->L) throw Error(DMPM_ERR_INDEX, "Layer index out of range.");
+        const void* src = s->gctl[layer];
+        if (!src) {
+            if (layer == s->carry_layer) src = s->gctl_stop;
+            else throw Error(DMPM_ERR_STATE, "no control gradient recorded for this layer");
+        }
+        const size_t bytes = (size_t)s->n * 9 * dsize(dtype);
+        void* dst = on_device ? out : s->dalloc(bytes);
+        export_planes(s->pr, src, dst, dtype == DMPM_DTYPE_F64, s->perm, s->n, 9, s->ld, s->stream);
+        if (!on_device) {
+            HIPCHK(hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, s->stream));
+            HIPCHK(hipStreamSynchronize(s->stream));
+            s->dfree(dst);
+        }
+    });
+}
+
+int dmpm_control_grad_norm(dmpm_sim* s, double* norm) {
+    return guard([&] {
+        check_sim(s);
+        *norm = control_norm(s);
+    });
+}
+
+int dmpm_descend_adam(dmpm_sim* s, int layer, double alpha, double grad_norm, double beta1, double beta2,
+                      double eps, int timestep) {
+    return guard([&] {
+        check_sim(s);
+        if (layer < 0 || layer >= s->L) throw Error(DMPM_ERR_INDEX, "Layer index out of range.");
+        descend_adam(s, layer, alpha, grad_norm, beta1, beta2, eps, timestep);
+    });
+}
+
+int dmpm_optimize_single_timestep(dmpm_sim* s, int t, int max_gd_iters, int /*episode*/, double initial_alpha,
+                                  int max_ls) {
+    return guard([&] {
+        check_sim(s);
+        if (t < 0 || t >= s->L - 1) {
+            fprintf(stderr, "Invalid timestep index: %d\n", t);
+            return;
+        }
+        const double beta1 = 0.9, beta2 = 0.999, eps = 1e-3;
+        double alpha = initial_alpha;
+        ensure_adam(s, t);
+        void* bak = s->dalloc((size_t)4 * s->ctl_bytes());
+        for (int gd = 0; gd < max_gd_iters; ++gd) {
+            s->forward(t);
+            const double gd_loss = end_layer_mass_loss(s);
+            if (!std::isfinite(gd_loss)) {
+                printf("Warning: Non-finite loss. Aborting.\n");
+                break;
+            }
+            s->backward(t);
+            const double gn = std::max(control_norm(s), 1e-12);
+            backup_layer(s, t, bak);
+            bool accepted = false;
+            double alpha_try = alpha;
+            for (int ls = 0; ls < max_ls; ++ls) {
+                descend_adam(s, t, alpha_try, gn, beta1, beta2, eps, g_single_timestep_adam_t + 1);
+                s->forward(t);
+                const double nl = end_layer_mass_loss(s);
+                if (std::isfinite(nl) && nl < gd_loss) {
+                    g_single_timestep_adam_t++;
+                    alpha = std::min(alpha_try * 1.1, initial_alpha);
+                    accepted = true;
+                    break;
+                }
+                restore_layer(s, t, bak);
+                alpha_try *= 0.5;
+            }
+            if (!accepted) {
+                printf("  Line search failed.\n");
+                break;
+            }
+        }
+        HIPCHK(hipStreamSynchronize(s->stream));
+        s->dfree(bak);
+    });
+}
+
+int dmpm_run_optimization(dmpm_sim* s, int num_steps, double dt, double drag, const double f_ext[3],
+                          int control_stride, int max_gd_iters, int max_ls, double initial_alpha, double gd_tol,
+                          double smoothing, int /*current_episodes*/, double* final_loss) {
+    return guard([&] {
+        check_sim(s);
+        if (num_steps <= 0) throw Error(DMPM_ERR_ARG, "num_steps must be positive");
+        if (control_stride <= 0) throw Error(DMPM_ERR_ARG, "control_stride must be positive");
+        s->cfg.dt = dt;
+        s->cfg.drag = drag;
+        s->cfg.smoothing = smoothing;
+        for (int i = 0; i < 3; ++i) s->cfg.f_ext[i] = f_ext ? f_ext[i] : 0.0;
+        s->ph = phys_of(s->cfg);
+        if (dmpm_set_up_comp_graph(s, num_steps) != DMPM_OK) throw Error(DMPM_ERR_STATE, g_err);
+        s->forward(0);
+        double loss = end_layer_mass_loss(s);
+        s->backward(0);
+        const double beta1 = 0.9, beta2 = 0.999, eps = 1e-3;
+        int adam_t = 0;
+        void* bak = s->dalloc((size_t)4 * s->ctl_bytes());
+        const int passes = 3;  // CompGraph.cpp:246
+        for (int pass = 0; pass < passes; ++pass) {
+            for (int ct = 0; ct < num_steps - 1; ct += control_stride) {
+                double alpha = initial_alpha;
+                double norm0 = 0;
+                ensure_adam(s, ct);
+                for (int gd = 0; gd < max_gd_iters; ++gd) {
+                    s->forward(ct);
+                    const double gd_loss = end_layer_mass_loss(s);
+                    if (!std::isfinite(gd_loss)) {
+                        printf("Warning: Non-finite loss detected. Aborting step.\n");
+                        break;
+                    }
+                    s->backward(ct);
+                    const double gn = std::max(control_norm(s), 1e-12);
+                    if (gd == 0) norm0 = gn;
+                    if (gn < gd_tol * norm0) break;
+                    backup_layer(s, ct, bak);
+                    bool accepted = false;
+                    double alpha_try = alpha;
+                    for (int ls = 0; ls < max_ls; ++ls) {
+                        descend_adam(s, ct, alpha_try, gn, beta1, beta2, eps, adam_t + 1);
+                        s->forward(ct);
+                        const double nl = end_layer_mass_loss(s);
+                        if (std::isfinite(nl) && nl < gd_loss) {
+                            adam_t++;
+                            alpha = std::min(alpha_try * 1.1, initial_alpha);
+                            accepted = true;
+                            break;
+                        }
+                        restore_layer(s, ct, bak);
+                        alpha_try *= 0.5;
+                    }
+                    if (!accepted) break;
+                }
+            }
+        }
+        loss = end_layer_mass_loss(s);
+        if (final_loss) *final_loss = loss;
+        s->dfree(bak);
+    });
+}
+
+int dmpm_p2g_mass(const dmpm_config* cfg, int n, const void* x, const void* m, void* grid_m, int dtype,
+                  int on_device) {
+    return guard([&] {
+        if (!cfg || !x || !m || !grid_m) throw Error(DMPM_ERR_ARG, "null argument");
+        double vol_unused = 0;
+        (void)vol_unused;
+        dmpm_config c = *cfg;
+        validate(c);
+        dmpm_sim* s = nullptr;
+        if (dmpm_create(&c, &s) != DMPM_OK) throw Error(DMPM_ERR_HIP, g_err);
+        std::unique_ptr<dmpm_sim> own(s);
+        // particles with only x and m
+        const size_t es = dsize(dtype);
+        std::vector<char> zeros3((size_t)n * 9 * es, 0), ones((size_t)n * es);
+        for (int i = 0; i < n; ++i) {
+            if (dtype == DMPM_DTYPE_F64) ((double*)ones.data())[i] = 1.0;
+            else ((float*)ones.data())[i] = 1.0f;
+        }
+        const void* xx = x;
+        const void* mm = m;
+        std::vector<char> xh, mh;
+        if (on_device) {
+            xh.resize((size_t)n * 3 * es);
+            mh.resize((size_t)n * es);
+            HIPCHK(hipMemcpy(xh.data(), x, xh.size(), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(mh.data(), m, mh.size(), hipMemcpyDeviceToHost));
+            xx = xh.data();
+            mm = mh.data();
+        }
+        if (dmpm_set_particles(s, n, xx, zeros3.data(), zeros3.data(), zeros3.data(), nullptr, mm, ones.data(),
+                               ones.data(), ones.data(), dtype, 0) != DMPM_OK)
+            throw Error(DMPM_ERR_HIP, g_err);
+        Launch Lc = s->launch();
+        Lc.st = s->layer[0];
+        Lc.stat = s->stat;
+        Lc.grid = s->grid;
+        s->zero(s->grid, s->gld);
+        p2g(s->pr, Lc, true, s->stream);
+        void* dst = on_device ? grid_m : s->dalloc((size_t)s->G * es);
+        export_planes(s->pr, s->grid, dst, dtype == DMPM_DTYPE_F64, nullptr, (int)s->G, 1, s->gld, s->stream);
+        if (!on_device) {
+            HIPCHK(hipMemcpyAsync(grid_m, dst, (size_t)s->G * es, hipMemcpyDeviceToHost, s->stream));
+        }
+        HIPCHK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int dmpm_calculate_volumes(const dmpm_config* cfg, int n, const void* x, const void* m, void* vol, void* grid_m,
+                           int dtype, int on_device) {
+    return guard([&] {
+        if (!cfg || !x || !m || !vol) throw Error(DMPM_ERR_ARG, "null argument");
+        dmpm_config c = *cfg;
+        validate(c);
+        dmpm_sim* s = nullptr;
+        if (dmpm_create(&c, &s) != DMPM_OK) throw Error(DMPM_ERR_HIP, g_err);
+        std::unique_ptr<dmpm_sim> own(s);
+        const size_t es = dsize(dtype);
+        std::vector<char> zeros((size_t)n * 9 * es, 0), ones((size_t)n * es);
+        for (int i = 0; i < n; ++i) {
+            if (dtype == DMPM_DTYPE_F64) ((double*)ones.data())[i] = 1.0;
+            else ((float*)ones.data())[i] = 1.0f;
+        }
+        const void* xx = x;
+        const void* mm = m;
+        std::vector<char> xh, mh;
+        if (on_device) {
+            xh.resize((size_t)n * 3 * es);
+            mh.resize((size_t)n * es);
+            HIPCHK(hipMemcpy(xh.data(), x, xh.size(), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(mh.data(), m, mh.size(), hipMemcpyDeviceToHost));
+            xx = xh.data();
+            mm = mh.data();
+        }
+        if (dmpm_set_particles(s, n, xx, zeros.data(), zeros.data(), zeros.data(), nullptr, mm, ones.data(),
+                               ones.data(), ones.data(), dtype, 0) != DMPM_OK)
+            throw Error(DMPM_ERR_HIP, g_err);
+        Launch Lc = s->launch();
+        Lc.st = s->layer[0];
+        Lc.stat = s->stat;
+        Lc.grid = s->grid;
+        s->zero(s->grid, s->gld);
+        p2g(s->pr, Lc, true, s->stream);
+        void* vint = s->dalloc((size_t)s->ld * s->esz);
+        volumes(s->pr, Lc, vint, s->stream);
+        void* dv = on_device ? vol : s->dalloc((size_t)n * es);
+        export_planes(s->pr, vint, dv, dtype == DMPM_DTYPE_F64, s->perm, n, 1, s->ld, s->stream);
+        if (!on_device) HIPCHK(hipMemcpyAsync(vol, dv, (size_t)n * es, hipMemcpyDeviceToHost, s->stream));
+        if (grid_m) {
+            void* dg = on_device ? grid_m : s->dalloc((size_t)s->G * es);
+            export_planes(s->pr, s->grid, dg, dtype == DMPM_DTYPE_F64, nullptr, (int)s->G, 1, s->gld, s->stream);
+            if (!on_device) HIPCHK(hipMemcpyAsync(grid_m, dg, (size_t)s->G * es, hipMemcpyDeviceToHost, s->stream));
+        }
+        HIPCHK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int dmpm_lame_parameters(double E, double nu, double* lam, double* mu) {
+    return guard([&] {
+        if (!lam || !mu) throw Error(DMPM_ERR_ARG, "null output");
+        const float e = (float)E, p = (float)nu;
+        *lam = (double)(e * p / ((1.f + p) * (1.f - 2.f * p)));
+        *mu = (double)(e / (2.f + 2.f * p));
+    });
+}
+
+int dmpm_last_timing(const dmpm_sim* s, double* fwd_ms, double* bwd_ms) {
+    return guard([&] {
+        check_sim(s);
+        float a = 0, b = 0;
+        HIPCHK(hipEventSynchronize(s->ev[3]));
+        if (hipEventElapsedTime(&a, s->ev[0], s->ev[1]) != hipSuccess) a = 0;
+        if (hipEventElapsedTime(&b, s->ev[2], s->ev[3]) != hipSuccess) b = 0;
+        (void)hipGetLastError();
+        if (fwd_ms) *fwd_ms = a;
+        if (bwd_ms) *bwd_ms = b;
+    });
+}
+
+int dmpm_kernel_timer(dmpm_sim* s, int capacity) {
+    return guard([&] {
+        check_sim(s);
+        HIPCHK(hipStreamSynchronize(s->stream));
+        s->krec.clear();
+        s->knext = 0;
+        s->koverflow = 0;
+        s->ktimer = capacity > 0;
+        const size_t want = 2 * (size_t)std::max(capacity, 0);
+        while (s->kpool.size() < want) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            s->kpool.push_back(e);
+        }
+    });
+}
+
+int dmpm_kernel_stats(dmpm_sim* s, int kind, double* total_ms, int* count) {
+    return guard([&] {
+        check_sim(s);
+        if (kind < 0 || kind >= dmpm_sim::NKIND) throw Error(DMPM_ERR_ARG, "unknown kernel kind");
+        HIPCHK(hipStreamSynchronize(s->stream));
+        double tot = 0;
+        int c = 0;
+        for (auto& r : s->krec) {
+            if (r.first != kind) continue;
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, s->kpool[r.second], s->kpool[r.second + 1]));
+            tot += ms;
+            ++c;
+        }
+        if (total_ms) *total_ms = tot;
+        if (count) *count = c;
+    });
+}
+
+int dmpm_memory(const dmpm_sim* s, size_t* current, size_t* peak) {
+    return guard([&] {
+        check_sim(s);
+        if (current) *current = s->mem_cur;
+        if (peak) *peak = s->mem_peak;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+"""Writes the golden vectors in this folder from the reference's own code.
+
+Runs in the build container only: it needs oracle/_ref/libdmpm_ref.so, which
+`make -C oracle ref` compiles from the reference's DiffMPMLib3D sources (plus
+oracle/ref_capi.cpp, a C-ABI driver that only builds the reference's
+PointCloud/Grid/CompGraph objects from arrays and calls its functions).  The
+reference runs single-threaded (OMP_NUM_THREADS=1 semantics) so that its
+float atomics and critical sections are deterministic.
+
+    python tests/golden/make_golden.py
+
+Every fixture holds the inputs and the reference's outputs (float32).
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "shape-morphing-binder_amd"))
+
+from oracle import Ref, reference_params  # noqa: E402
+from dmpm.scenes import Body, make_scene  # noqa: E402
+
+
+def scene(n=400, cells=16, seed=7):
+    sc = make_scene(Body((0.5, 0.5, 0.5), (0.2, 0.2, 0.2)), n, cells, seed, 1e4, 0.3, h=1 / 32, velocity="sine",
+                    amplitude=0.3)
+    s = {k: np.asarray(v, np.float32) for k, v in sc.state().items()}
+    rng = np.random.default_rng(seed + 1)
+    s["F"] = (s["F"] + 0.05 * rng.normal(size=s["F"].shape)).astype(np.float32)
+    s["C"] = (0.5 * rng.normal(size=s["C"].shape)).astype(np.float32)
+    s["dFc"] = (0.01 * rng.normal(size=s["F"].shape)).astype(np.float32)
+    return sc, s
+
+
+def main():
+    ref = Ref(threads=1)
+    sc, s = scene()
+    prm = reference_params(dims=sc.dims, dx=sc.dx, dt=1e-3, drag=0.5, smoothing=0.1, f_ext=(0.0, -9.8, 0.0))
+    meta = dict(dims=np.array(prm.dims), dx=np.float64(prm.dx), dt=np.float64(prm.dt), drag=np.float64(prm.drag),
+                smoothing=np.float64(prm.smoothing), f_ext=np.array(prm.f_ext))
+    ins = {f"in_{k}": v for k, v in s.items()}
+
+    # 1. one ForwardTimeStep
+    o = ref.forward_step(prm, s)
+    np.savez_compressed(os.path.join(HERE, "ref_forward_step.npz"), **meta, **ins,
+                        **{f"out_{k}": o[k] for k in ("x", "v", "F", "C", "P", "grid_m", "grid_v")})
+
+    # 2. one Back_Timestep with a seeded adjoint of layer n+1
+    rng = np.random.default_rng(21)
+    adj = dict(x=rng.normal(size=(sc.n, 3)), v=rng.normal(size=(sc.n, 3)), F=rng.normal(size=(sc.n, 3, 3)),
+               C=rng.normal(size=(sc.n, 3, 3)))
+    adj = {k: v.astype(np.float32) for k, v in adj.items()}
+    b = ref.backward_step(prm, s, adj)
+    np.savez_compressed(os.path.join(HERE, "ref_backward_step.npz"), **meta, **ins,
+                        **{f"adj_{k}": v for k, v in adj.items()},
+                        **{f"out_{k}": b[k] for k in ("x", "v", "F", "C", "P")})
+
+    # 3. EndLayerMassLoss against the node masses of a shifted body
+    tsc = make_scene(Body((0.53, 0.5, 0.47), (0.2, 0.18, 0.2)), 400, 16, 8, 1e4, 0.3, h=1 / 32)
+    tm = ref.p2g_mass(prm, tsc.x.astype(np.float32), tsc.m.astype(np.float32))
+    loss, gx, gm, gdm = ref.mass_loss(prm, s, tm)
+    np.savez_compressed(os.path.join(HERE, "ref_mass_loss.npz"), **meta, **ins, target_m=tm,
+                        out_loss=np.float64(loss), out_gx=gx, out_grid_m=gm, out_grid_dLdm=gdm)
+
+    # 4. a 4-layer episode: forward, loss, render gradients, backward to layer 0
+    s4 = dict(s)
+    s4["dFc"] = np.zeros_like(s["F"])
+    rgF = (1e-3 * rng.normal(size=(sc.n, 3, 3))).astype(np.float32)
+    rgx = (1e-2 * rng.normal(size=(sc.n, 3))).astype(np.float32)
+    e = ref.episode(prm, s4, tm, num_layers=4, control_layer=0, render_gF=rgF, render_gx=rgx)
+    np.savez_compressed(os.path.join(HERE, "ref_episode.npz"), **meta, **{f"in_{k}": v for k, v in s4.items()},
+                        target_m=tm, render_gF=rgF, render_gx=rgx, num_layers=np.int64(4),
+                        out_loss=np.float64(e["loss"]),
+                        **{f"out_{k}": e[k] for k in ("x", "v", "F", "C", "gx", "gv", "gF", "gC")})
+
+    # 5. CalculatePointCloudVolumes and the dt=0 P2G
+    vol, gmv = ref.volumes(prm, s["x"], s["m"])
+    np.savez_compressed(os.path.join(HERE, "ref_volumes.npz"), **meta, in_x=s["x"], in_m=s["m"], out_vol=vol,
+                        out_grid_m=gmv)
+
+    # 6. Descend_Adam from a non-zero moment state, two calls
+    n = 64
+    dLdF = rng.normal(size=(n, 3, 3)).astype(np.float32)
+    dFc = (0.01 * rng.normal(size=(n, 3, 3))).astype(np.float32)
+    mom = (0.1 * rng.normal(size=(n, 3, 3))).astype(np.float32)
+    vec = np.abs(0.01 * rng.normal(size=(n, 3, 3))).astype(np.float32)
+    vmax = np.maximum(vec, np.abs(0.01 * rng.normal(size=(n, 3, 3)))).astype(np.float32)
+    gn = ref.dLdF_norm(dLdF)
+    out1 = ref.adam(dLdF, dFc, mom, vec, vmax, 0.5, gn, timestep=3)
+    dLdF2 = dLdF.copy()
+    dLdF2[::2] *= 1e-4  # near-stationary particles take the damped branch
+    out2 = ref.adam(dLdF2, *out1, 1e-3, gn, timestep=4)
+    np.savez_compressed(os.path.join(HERE, "ref_adam.npz"), dLdF=dLdF, dLdF2=dLdF2, dFc=dFc, mom=mom, vec=vec, vmax=vmax,
+                        grad_norm=np.float64(gn), out1_dFc=out1[0], out1_mom=out1[1], out1_vec=out1[2],
+                        out1_vmax=out1[3], out2_dFc=out2[0], out2_mom=out2[1], out2_vec=out2[2], out2_vmax=out2[3])
+
+    # 7. Elasticity primitives on sampled F (including inverted and near-rotation ones)
+    F = np.eye(3, dtype=np.float32) + 0.3 * rng.normal(size=(128, 3, 3)).astype(np.float32)
+    F[:4] = np.diag([1.0, 1.0, -0.5]).astype(np.float32) @ F[:4]
+    lam, mu = ref.lame(1e4, 0.3)
+    dF = rng.normal(size=F.shape).astype(np.float32)
+    R, S = ref.polar(F)
+    np.savez_compressed(os.path.join(HERE, "ref_elasticity.npz"), F=F, dF=dF, lam=np.float64(lam),
+                        mu=np.float64(mu), R=R, S=S, P=ref.pk_fcr(F, lam, mu), hvp=ref.pk_fcr_hvp(F, lam, mu, dF),
+                        psi=ref.energy_fcr(F, lam, mu))
+    print("wrote golden vectors to", HERE)
+
+
+if __name__ == "__main__":
+    main()
