@@ -39,7 +39,10 @@ HBM_PEAK_GBS = 8000.0
 #   adj_scatter reads x, F (12) and the adjoint carry (24), m, p per node; writes dL/dv per node
 #   adj_gather  reads x, v, F, C, m, vol, lam, mu (28) and the carry (24), m, p, dL/dv per node;
 #               writes the carry (24)
+#   step        the fused forward step: reads x, F, m, vol, lam, mu (16 floats) and m, p of grid t per node;
+#               writes x, v, F, C (24 floats) and m, p of grid t+1 per node
 BYTES = {
+    "step": (40 * 4, 32),
     "p2g": (28 * 4, 16),
     "g2p": (36 * 4, 16),
     "adj_scatter": (36 * 4, 28),

@@ -86,8 +86,11 @@ typedef struct {
     int adjoint;            /* DMPM_ADJOINT_* */
     int precision;          /* DMPM_PREC_* */
     int checkpoint_interval;/* K: store every K-th layer and recompute the rest (1: store all) */
-    int lds_bytes;          /* LDS budget per block for the grid window (0: default) */
-    int block;              /* threads per block (0: default 256) */
+    int lds_bytes;          /* LDS budget per block for the grid window (0: default 48 KiB; max 64 KiB) */
+    int block;              /* threads per block: 64, 128, 192 or 256 (0: default 256) */
+    int unfused;            /* 1: separate G2P and P2G kernels instead of the fused forward step */
+    int deterministic;      /* 1: two-pass scatters (sort by cell, node-centric sums in a fixed
+                               order): bitwise reproducible, slower */
 } dmpm_config;
 
 /* Fill `cfg` with the reference's physics (cubic, fixed-corotated, zeta
@@ -207,6 +210,7 @@ int dmpm_last_timing(const dmpm_sim* sim, double* fwd_ms, double* bwd_ms);
 #define DMPM_KERNEL_ADJ_SCATTER 2
 #define DMPM_KERNEL_ADJ_GATHER 3
 #define DMPM_KERNEL_GRID_CLEAR 4
+#define DMPM_KERNEL_STEP 5
 int dmpm_kernel_timer(dmpm_sim* sim, int capacity);
 int dmpm_kernel_stats(dmpm_sim* sim, int kind, double* total_ms, int* count);
 /* Device bytes currently allocated by the sim, and the peak. */

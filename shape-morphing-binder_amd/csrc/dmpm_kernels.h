@@ -44,6 +44,8 @@ struct Launch {
     const void* adj1;  // adjoint carry of layer t+1
     void* adj0;        // adjoint carry of layer t (output)
     void* gctl;        // dL/d dFc of layer t (output, 9 planes) or null
+    const void* ctl1;  // fused step: dFc of layer t+1 or null
+    void* grid1;       // fused step: grid t+1 (4 planes, zeroed by the caller)
     int lds_bytes;     // LDS budget per block for the window
     int block;         // threads per block (multiple of 64)
 };
@@ -53,10 +55,21 @@ struct Launch {
 void p2g(Prec pr, const Launch& L, bool mass_only, hipStream_t s);
 // Forward: G2P + particle update, `st` + grid -> `out`.
 void g2p(Prec pr, const Launch& L, hipStream_t s);
+// Fused forward step: G2P of `st` from `grid`, update -> `out`, then the
+// stress and P2G of `out` (with control ctl1) into `grid1`.
+void step(Prec pr, const Launch& L, hipStream_t s);
 // Adjoint phases 1+2: scatter dL/dv_i into ggv.
 void adj_scatter(Prec pr, const Launch& L, hipStream_t s);
 // Adjoint phases 3+4 (+ the stress VJP): writes adj0 and gctl.
 void adj_gather(Prec pr, const Launch& L, hipStream_t s);
+
+// Deterministic two-pass mode: scratch size for n particles on a grid of
+// `dims` nodes (element size esize), P2G and adjoint scatter that write every
+// node (no clear needed) in an order fixed by a stable sort of particles by
+// base cell.
+size_t det_scratch_bytes(int n, const int* dims, int esize);
+void det_p2g(Prec pr, const Launch& L, bool mass_only, void* scratch, hipStream_t s);
+void det_adj_scatter(Prec pr, const Launch& L, void* scratch, hipStream_t s);
 
 // Fill `n` elements of `p` with zero.
 void fill_zero(Prec pr, void* p, long long n, hipStream_t s);

@@ -20,6 +20,8 @@
 // into its readers.
 #include <climits>
 
+#include <hipcub/hipcub.hpp>
+
 #include "dmpm_kernels.h"
 #include "dmpm_math.h"
 
@@ -43,6 +45,9 @@ struct KP {
     const T* adj1;
     T* adj0;
     T* gctl;
+    const T* ctl1;  // control of layer t+1 (fused step)
+    T* gm1;         // grid t+1 (fused step): m, then 3 p planes
+    T* gp1;
     int cap;
 };
 
@@ -79,6 +84,9 @@ static KP<T> make_kp(const Launch& L, int narrays) {
     k.adj1 = (const T*)L.adj1;
     k.adj0 = (T*)L.adj0;
     k.gctl = (T*)L.gctl;
+    k.ctl1 = (const T*)L.ctl1;
+    k.gm1 = (T*)L.grid1;
+    k.gp1 = L.grid1 ? (T*)L.grid1 + L.gld : nullptr;
     k.cap = narrays > 0 ? (int)(L.lds_bytes / (narrays * (int)sizeof(T))) : 0;
     return k;
 }
@@ -250,7 +258,109 @@ DMPM_DI Trust<T> trust_region(const KP<T>& a, const m3<T>& Ft, const m3<T>& Craw
 }
 
 // ---------------------------------------------------------------------------
-// P2G (ForwardSimulation.cpp:26-80) with the stress of P_op_1 (:16-24).
+// P2G pieces (ForwardSimulation.cpp:26-80) with the stress of P_op_1 (:16-24).
+
+// Affine momentum matrix A_p = -D^{-1} dt vol (P Ft^T + zeta tr(C) I) + m C
+// and the drag-scaled momentum m (1 - dt drag) v of one particle.
+template <typename T, int ORDER, int MODEL>
+DMPM_DI void p2g_terms(const KP<T>& a, int i, const v3<T>& v, const m3<T>& Ft, const m3<T>& C, T m, m3<T>& A,
+                       v3<T>& mv) {
+    const T vol = a.stat[SVOL * a.ld + i];
+    const T lam = a.stat[SLAM * a.ld + i], mu = a.stat[SMU * a.ld + i];
+    const m3<T> P = Model<MODEL>::pk1(Ft, lam, mu);
+    m3<T> str = mmult(P, Ft);
+    const T tau = a.zeta * mtrace(C);
+    str.a[0] += tau; str.a[4] += tau; str.a[8] += tau;
+    const T k0 = -Spline<ORDER>::dinv(a.dx) * a.dt * vol;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) A.a[q] = k0 * str.a[q] + m * C.a[q];
+    const T dragf = T(1) - a.dt * a.drag;
+    mv = {{m * v.a[0] * dragf, m * v.a[1] * dragf, m * v.a[2] * dragf}};
+}
+
+// Scatter one particle's m and p into the LDS window (or the grid when the
+// window does not fit).
+template <typename T, int ORDER, bool MASS_ONLY>
+DMPM_DI void p2g_scatter(const KP<T>& a, const Win& win, T* sm, const Sten<T, ORDER>& S, T m, const m3<T>& A,
+                         const v3<T>& mv, T* gm, T* gp) {
+    constexpr int KS = Spline<ORDER>::K;
+    T* lm = sm;
+    T* lp0 = sm + a.cap;
+    T* lp1 = sm + 2 * a.cap;
+    T* lp2 = sm + 3 * a.cap;
+#pragma unroll
+    for (int ia = 0; ia < KS; ++ia) {
+        const int gi = S.b[0] + ia;
+        const T d0 = S.u[0][ia] * a.dx;
+#pragma unroll
+        for (int ib = 0; ib < KS; ++ib) {
+            const int gj = S.b[1] + ib;
+            const T d1 = S.u[1][ib] * a.dx;
+            const T w01 = S.w[0][ia] * S.w[1][ib];
+            // A dgp with dgp = (d0, d1, d2): the z-independent part first
+            const T q0 = mv.a[0] + A.a[0] * d0 + A.a[1] * d1;
+            const T q1 = mv.a[1] + A.a[3] * d0 + A.a[4] * d1;
+            const T q2 = mv.a[2] + A.a[6] * d0 + A.a[7] * d1;
+#pragma unroll
+            for (int ic = 0; ic < KS; ++ic) {
+                const int gk = S.b[2] + ic;
+                if (!in_grid(a, gi, gj, gk)) continue;
+                const T d2 = S.u[2][ic] * a.dx;
+                const T wt = w01 * S.w[2][ic];
+                if (win.lds) {
+                    const int li = ((gi - win.lo0) * win.e1 + (gj - win.lo1)) * win.e2 + (gk - win.lo2);
+                    atomicAdd(&lm[li], wt * m);
+                    if (!MASS_ONLY) {
+                        atomicAdd(&lp0[li], wt * (q0 + A.a[2] * d2));
+                        atomicAdd(&lp1[li], wt * (q1 + A.a[5] * d2));
+                        atomicAdd(&lp2[li], wt * (q2 + A.a[8] * d2));
+                    }
+                } else {
+                    const long long g = gindex(a, gi, gj, gk);
+                    unsafeAtomicAdd(gm + g, wt * m);
+                    if (!MASS_ONLY) {
+                        unsafeAtomicAdd(gp + g, wt * (q0 + A.a[2] * d2));
+                        unsafeAtomicAdd(gp + a.gld + g, wt * (q1 + A.a[5] * d2));
+                        unsafeAtomicAdd(gp + 2 * a.gld + g, wt * (q2 + A.a[8] * d2));
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Zero the window's accumulators (NA arrays).
+template <typename T, int NA>
+DMPM_DI void window_zero(const KP<T>& a, const Win& win, T* sm) {
+    for (int t = threadIdx.x; t < win.vol; t += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < NA; ++q) sm[q * a.cap + t] = T(0);
+}
+
+// Flush the window to the grid: one atomic per touched node value.
+template <typename T, bool MASS_ONLY>
+DMPM_DI void p2g_flush(const KP<T>& a, const Win& win, const T* sm, T* gm, T* gp) {
+    const int e12 = win.e1 * win.e2;
+    for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
+        const int gi = win.lo0 + t / e12, gj = win.lo1 + (t / win.e2) % win.e1, gk = win.lo2 + t % win.e2;
+        if (!in_grid(a, gi, gj, gk)) continue;
+        const T vm = sm[t];
+        const long long g = gindex(a, gi, gj, gk);
+        if (MASS_ONLY) {
+            if (vm != T(0)) unsafeAtomicAdd(gm + g, vm);
+        } else {
+            const T p0 = sm[a.cap + t], p1 = sm[2 * a.cap + t], p2 = sm[3 * a.cap + t];
+            if (vm != T(0) || p0 != T(0) || p1 != T(0) || p2 != T(0)) {
+                unsafeAtomicAdd(gm + g, vm);
+                unsafeAtomicAdd(gp + g, p0);
+                unsafeAtomicAdd(gp + a.gld + g, p1);
+                unsafeAtomicAdd(gp + 2 * a.gld + g, p2);
+            }
+        }
+    }
+}
+
+// P2G of layer `st` into the grid.
 template <typename T, int ORDER, int MODEL, bool MASS_ONLY>
 __global__ void __launch_bounds__(256) k_p2g(KP<T> a) {
     constexpr int KS = Spline<ORDER>::K;
@@ -268,98 +378,23 @@ __global__ void __launch_bounds__(256) k_p2g(KP<T> a) {
         S.b[0] = S.b[1] = S.b[2] = 0;
     }
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
-    if (win.lds)
-        for (int t = threadIdx.x; t < win.vol * NA; t += blockDim.x) sm[(t / win.vol) * a.cap + t % win.vol] = T(0);
-
+    if (win.lds) window_zero<T, NA>(a, win, sm);
     T m = T(0);
     m3<T> A = mzero<T>();
     v3<T> mv = vzero<T>();
     if (valid) {
         m = a.stat[SM * a.ld + i];
         if (!MASS_ONLY) {
-            const T vol = a.stat[SVOL * a.ld + i];
-            const T lam = a.stat[SLAM * a.ld + i], mu = a.stat[SMU * a.ld + i];
-            const v3<T> v = load_v3(a.st, a.ld, PV, i);
             m3<T> F = load_m3(a.st, a.ld, PF, i);
             if (a.ctl) F = madd(F, load_m3(a.ctl, a.ld, 0, i));
-            const m3<T> C = load_m3(a.st, a.ld, PC, i);
-            const m3<T> P = Model<MODEL>::pk1(F, lam, mu);
-            m3<T> str = mmult(P, F);
-            const T tau = a.zeta * mtrace(C);
-            str.a[0] += tau; str.a[4] += tau; str.a[8] += tau;
-            const T k0 = -Spline<ORDER>::dinv(a.dx) * a.dt * vol;
-#pragma unroll
-            for (int q = 0; q < 9; ++q) A.a[q] = k0 * str.a[q] + m * C.a[q];
-            const T dragf = T(1) - a.dt * a.drag;
-            mv = {{m * v.a[0] * dragf, m * v.a[1] * dragf, m * v.a[2] * dragf}};
+            p2g_terms<T, ORDER, MODEL>(a, i, load_v3(a.st, a.ld, PV, i), F, load_m3(a.st, a.ld, PC, i), m, A, mv);
         }
     }
     __syncthreads();
-    if (valid) {
-        T* lm = sm;
-        T* lp0 = sm + a.cap;
-        T* lp1 = sm + 2 * a.cap;
-        T* lp2 = sm + 3 * a.cap;
-#pragma unroll
-        for (int ia = 0; ia < KS; ++ia) {
-            const int gi = S.b[0] + ia;
-            const T d0 = S.u[0][ia] * a.dx;
-#pragma unroll
-            for (int ib = 0; ib < KS; ++ib) {
-                const int gj = S.b[1] + ib;
-                const T d1 = S.u[1][ib] * a.dx;
-                const T w01 = S.w[0][ia] * S.w[1][ib];
-                // A dgp with dgp = (d0, d1, d2): split the z-independent part
-                T q0 = mv.a[0] + A.a[0] * d0 + A.a[1] * d1;
-                T q1 = mv.a[1] + A.a[3] * d0 + A.a[4] * d1;
-                T q2 = mv.a[2] + A.a[6] * d0 + A.a[7] * d1;
-#pragma unroll
-                for (int ic = 0; ic < KS; ++ic) {
-                    const int gk = S.b[2] + ic;
-                    if (!in_grid(a, gi, gj, gk)) continue;
-                    const T d2 = S.u[2][ic] * a.dx;
-                    const T wt = w01 * S.w[2][ic];
-                    if (win.lds) {
-                        const int li = ((gi - win.lo0) * win.e1 + (gj - win.lo1)) * win.e2 + (gk - win.lo2);
-                        atomicAdd(&lm[li], wt * m);
-                        if (!MASS_ONLY) {
-                            atomicAdd(&lp0[li], wt * (q0 + A.a[2] * d2));
-                            atomicAdd(&lp1[li], wt * (q1 + A.a[5] * d2));
-                            atomicAdd(&lp2[li], wt * (q2 + A.a[8] * d2));
-                        }
-                    } else {
-                        const long long g = gindex(a, gi, gj, gk);
-                        unsafeAtomicAdd(a.gm + g, wt * m);
-                        if (!MASS_ONLY) {
-                            unsafeAtomicAdd(a.gp + g, wt * (q0 + A.a[2] * d2));
-                            unsafeAtomicAdd(a.gp + a.gld + g, wt * (q1 + A.a[5] * d2));
-                            unsafeAtomicAdd(a.gp + 2 * a.gld + g, wt * (q2 + A.a[8] * d2));
-                        }
-                    }
-                }
-            }
-        }
-    }
+    if (valid) p2g_scatter<T, ORDER, MASS_ONLY>(a, win, sm, S, m, A, mv, a.gm, a.gp);
     if (win.lds && win.any) {
         __syncthreads();
-        const int e12 = win.e1 * win.e2;
-        for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
-            const int gi = win.lo0 + t / e12, gj = win.lo1 + (t / win.e2) % win.e1, gk = win.lo2 + t % win.e2;
-            if (!in_grid(a, gi, gj, gk)) continue;
-            const T vm = sm[t];
-            const long long g = gindex(a, gi, gj, gk);
-            if (MASS_ONLY) {
-                if (vm != T(0)) unsafeAtomicAdd(a.gm + g, vm);
-            } else {
-                const T p0 = sm[a.cap + t], p1 = sm[2 * a.cap + t], p2 = sm[3 * a.cap + t];
-                if (vm != T(0) || p0 != T(0) || p1 != T(0) || p2 != T(0)) {
-                    unsafeAtomicAdd(a.gm + g, vm);
-                    unsafeAtomicAdd(a.gp + g, p0);
-                    unsafeAtomicAdd(a.gp + a.gld + g, p1);
-                    unsafeAtomicAdd(a.gp + 2 * a.gld + g, p2);
-                }
-            }
-        }
+        p2g_flush<T, MASS_ONLY>(a, win, sm, a.gm, a.gp);
     }
 }
 
@@ -427,6 +462,27 @@ DMPM_DI void g2p_gather(const KP<T>& a, const Win& win, const T* sv, const Sten<
     Craw = mscale(Spline<ORDER>::dinv(a.dx), Craw);
 }
 
+// P_op_2 (ForwardSimulation.cpp:131-157): trust region on C', F update with
+// smoothing, x update.  Writes layer t+1 and returns its x, v, F, C.
+template <typename T>
+DMPM_DI void particle_update(const KP<T>& a, int i, const v3<T>& x, const v3<T>& vn, const m3<T>& Craw,
+                             v3<T>& xn, m3<T>& Fn, m3<T>& Cn) {
+    const m3<T> F = load_m3(a.st, a.ld, PF, i);
+    const m3<T> Ft = a.ctl ? madd(F, load_m3(a.ctl, a.ld, 0, i)) : F;
+    const Trust<T> tr = trust_region(a, Ft, Craw);
+    Cn = tr.state ? mscale(tr.s, Craw) : Craw;
+    m3<T> A1 = mscale(a.dt, Cn);
+    A1.a[0] += T(1); A1.a[4] += T(1); A1.a[8] += T(1);
+    const m3<T> Fu = mmul(A1, Ft);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) Fn.a[q] = (T(1) - a.sm) * Fu.a[q] + a.sm * F.a[q];
+    xn = {{x.a[0] + a.dt * vn.a[0], x.a[1] + a.dt * vn.a[1], x.a[2] + a.dt * vn.a[2]}};
+    store_v3(a.out, a.ld, PX, i, xn);
+    store_v3(a.out, a.ld, PV, i, vn);
+    store_m3(a.out, a.ld, PF, i, Fn);
+    store_m3(a.out, a.ld, PC, i, Cn);
+}
+
 // G2P + P_op_2 (ForwardSimulation.cpp:89-157): writes layer t+1.
 template <typename T, int ORDER>
 __global__ void __launch_bounds__(256) k_g2p(KP<T> a) {
@@ -449,24 +505,63 @@ __global__ void __launch_bounds__(256) k_g2p(KP<T> a) {
         __syncthreads();
     }
     if (!valid) return;
-    v3<T> vn;
-    m3<T> Craw;
+    v3<T> vn, xn;
+    m3<T> Craw, Fn, Cn;
     g2p_gather<T, ORDER>(a, win, sv, S, vn, Craw);
-    const m3<T> F = load_m3(a.st, a.ld, PF, i);
-    const m3<T> Ft = a.ctl ? madd(F, load_m3(a.ctl, a.ld, 0, i)) : F;
-    const Trust<T> tr = trust_region(a, Ft, Craw);
-    const m3<T> Cn = tr.state ? mscale(tr.s, Craw) : Craw;
-    m3<T> A1 = mscale(a.dt, Cn);
-    A1.a[0] += T(1); A1.a[4] += T(1); A1.a[8] += T(1);
-    const m3<T> Fu = mmul(A1, Ft);
-    m3<T> Fn;
-#pragma unroll
-    for (int q = 0; q < 9; ++q) Fn.a[q] = (T(1) - a.sm) * Fu.a[q] + a.sm * F.a[q];
-    const v3<T> xn = {{x.a[0] + a.dt * vn.a[0], x.a[1] + a.dt * vn.a[1], x.a[2] + a.dt * vn.a[2]}};
-    store_v3(a.out, a.ld, PX, i, xn);
-    store_v3(a.out, a.ld, PV, i, vn);
-    store_m3(a.out, a.ld, PF, i, Fn);
-    store_m3(a.out, a.ld, PC, i, Cn);
+    particle_update(a, i, x, vn, Craw, xn, Fn, Cn);
+}
+
+// One fused forward step: G2P of step t from grid t (a.gm/a.gp), the
+// particle update writing layer t+1, then the stress of layer t+1 (with its
+// control a.ctl1) and its P2G into grid t+1 (a.gm1/a.gp1, zeroed by the
+// caller).  Layer t+1 is never re-read from HBM for its own P2G.
+template <typename T, int ORDER, int MODEL>
+__global__ void __launch_bounds__(256) k_step(KP<T> a) {
+    constexpr int KS = Spline<ORDER>::K;
+    extern __shared__ double smem_d[];
+    T* sm = reinterpret_cast<T*>(smem_d);
+    const int i = logical_block() * blockDim.x + threadIdx.x;
+    const bool valid = i < a.n;
+    Sten<T, ORDER> S;
+    v3<T> x = vzero<T>();
+    if (valid) {
+        x = load_v3(a.st, a.ld, PX, i);
+        S.init(a, x);
+    } else {
+        S.b[0] = S.b[1] = S.b[2] = 0;
+    }
+    const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
+    if (win.lds) {
+        stage_velocity(a, win, sm);
+        __syncthreads();
+    }
+    v3<T> vn = vzero<T>(), xn = vzero<T>();
+    m3<T> Craw, Fn = mzero<T>(), Cn = mzero<T>();
+    if (valid) {
+        g2p_gather<T, ORDER>(a, win, sm, S, vn, Craw);
+        particle_update(a, i, x, vn, Craw, xn, Fn, Cn);
+    }
+    // every wave is done with the staged velocities and the first window
+    __syncthreads();
+    Sten<T, ORDER> S1;
+    if (valid) S1.init(a, xn);
+    else S1.b[0] = S1.b[1] = S1.b[2] = 0;
+    const Win win1 = block_window<KS>(valid, S1.b[0], S1.b[1], S1.b[2], a.cap);
+    if (win1.lds) window_zero<T, 4>(a, win1, sm);
+    T m = T(0);
+    m3<T> A = mzero<T>();
+    v3<T> mv = vzero<T>();
+    if (valid) {
+        m = a.stat[SM * a.ld + i];
+        const m3<T> Ft1 = a.ctl1 ? madd(Fn, load_m3(a.ctl1, a.ld, 0, i)) : Fn;
+        p2g_terms<T, ORDER, MODEL>(a, i, vn, Ft1, Cn, m, A, mv);
+    }
+    __syncthreads();
+    if (valid) p2g_scatter<T, ORDER, false>(a, win1, sm, S1, m, A, mv, a.gm1, a.gp1);
+    if (win1.lds && win1.any) {
+        __syncthreads();
+        p2g_flush<T, false>(a, win1, sm, a.gm1, a.gp1);
+    }
 }
 
 // Phase 1 of the adjoint (BackPropagation.cpp:22-32).  REF reproduces the
@@ -804,6 +899,172 @@ __global__ void __launch_bounds__(256) k_adj_gather(KP<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Deterministic two-pass scatter.  Pass 1 computes each particle's
+// contribution terms and its base cell; a stable radix sort orders particles
+// by (base cell, particle index); pass 2 runs one thread per grid node, which
+// sums the contributions of the particles in the K^3 base cells that reach
+// it, in that fixed order, and stores the node (no atomics, no clear).  The
+// result is bitwise reproducible from run to run.
+
+// Base-cell key with a 4-cell margin on each side; particles beyond it reach
+// no grid node and are never gathered.
+template <typename T>
+DMPM_DI unsigned det_key(const KP<T>& a, const int* b) {
+    int c[3];
+    const int dm[3] = {a.nx, a.ny, a.nz};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) c[q] = min(max(b[q] + 4, 0), dm[q] + 7);
+    return (unsigned)(((long long)c[0] * (a.ny + 8) + c[1]) * (a.nz + 8) + c[2]);
+}
+
+// Pass 1 of P2G: payload m (1) | mv (3) | A (9) per particle.
+template <typename T, int ORDER, int MODEL, bool MASS_ONLY>
+__global__ void __launch_bounds__(256) k_det_prep_p2g(KP<T> a, unsigned* keys, int* idx, T* pay, long long pld) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const v3<T> x = load_v3(a.st, a.ld, PX, i);
+    Sten<T, ORDER> S;
+    S.init(a, x);
+    keys[i] = det_key(a, S.b);
+    idx[i] = i;
+    const T m = a.stat[SM * a.ld + i];
+    pay[i] = m;
+    if (!MASS_ONLY) {
+        m3<T> A;
+        v3<T> mv;
+        m3<T> F = load_m3(a.st, a.ld, PF, i);
+        if (a.ctl) F = madd(F, load_m3(a.ctl, a.ld, 0, i));
+        p2g_terms<T, ORDER, MODEL>(a, i, load_v3(a.st, a.ld, PV, i), F, load_m3(a.st, a.ld, PC, i), m, A, mv);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) pay[(1 + q) * pld + i] = mv.a[q];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) pay[(4 + q) * pld + i] = A.a[q];
+    }
+}
+
+__global__ void k_det_cell_ranges(const unsigned* keys, int n, int* cstart, int* cend) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const unsigned k = keys[s];
+    if (s == 0 || keys[s - 1] != k) cstart[k] = s;
+    if (s == n - 1 || keys[s + 1] != k) cend[k] = s + 1;
+}
+
+// B-spline value/argument for the node gather, from |u| (any piece).
+template <typename T, int ORDER>
+DMPM_DI T det_N(T u) {
+    const T ax = fabs(u);
+    if (ORDER == 3) {
+        if (ax < T(1)) return T(0.5) * ax * ax * ax - ax * ax + T(2) / T(3);
+        if (ax < T(2)) return (T(2) - ax) * (T(2) - ax) * (T(2) - ax) / T(6);
+        return T(0);
+    }
+    if (ax < T(0.5)) return T(0.75) - ax * ax;
+    if (ax < T(1.5)) return T(0.5) * (T(1.5) - ax) * (T(1.5) - ax);
+    return T(0);
+}
+
+// Visit, in fixed order, the particles whose stencil reaches node (i,j,k):
+// fn(p, u0, u1, u2, w) with u = node - fx_p in cell units.
+template <typename T, int ORDER, typename Fn>
+DMPM_DI void det_visit(const KP<T>& a, int i, int j, int k, const int* cstart, const int* cend, const int* idx,
+                       Fn&& fn) {
+    constexpr int KS = Spline<ORDER>::K;
+    const T inv = T(1) / a.dx;
+    for (int da = 0; da < KS; ++da)
+        for (int db = 0; db < KS; ++db)
+            for (int dc = 0; dc < KS; ++dc) {
+                const int b[3] = {i - KS + 1 + da, j - KS + 1 + db, k - KS + 1 + dc};
+                if (b[0] + 4 < 0 || b[1] + 4 < 0 || b[2] + 4 < 0 || b[0] + 4 > a.nx + 7 || b[1] + 4 > a.ny + 7 ||
+                    b[2] + 4 > a.nz + 7)
+                    continue;
+                const unsigned key = det_key(a, b);
+                const int e = cend[key];
+                for (int s = cstart[key]; s < e; ++s) {
+                    const int p = idx[s];
+                    const T u0 = (T)i - (a.st[PX * a.ld + p] - a.mn0) * inv;
+                    const T u1 = (T)j - (a.st[(PX + 1) * a.ld + p] - a.mn1) * inv;
+                    const T u2 = (T)k - (a.st[(PX + 2) * a.ld + p] - a.mn2) * inv;
+                    const T w = det_N<T, ORDER>(u0) * det_N<T, ORDER>(u1) * det_N<T, ORDER>(u2);
+                    fn(p, u0, u1, u2, w);
+                }
+            }
+}
+
+// Pass 2 of P2G: one thread per node.
+template <typename T, int ORDER, bool MASS_ONLY>
+__global__ void __launch_bounds__(256) k_det_gather_p2g(KP<T> a, const int* cstart, const int* cend, const int* idx,
+                                                        const T* pay, long long pld) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long G = (long long)a.nx * a.ny * a.nz;
+    if (g >= G) return;
+    const int k = (int)(g % a.nz), j = (int)((g / a.nz) % a.ny), i = (int)(g / ((long long)a.ny * a.nz));
+    T m = T(0), p0 = T(0), p1 = T(0), p2 = T(0);
+    det_visit<T, ORDER>(a, i, j, k, cstart, cend, idx, [&](int q, T u0, T u1, T u2, T w) {
+        m += w * pay[q];
+        if (!MASS_ONLY) {
+            const T d0 = u0 * a.dx, d1 = u1 * a.dx, d2 = u2 * a.dx;
+            const T* A = pay + 4 * pld + q;
+            p0 += w * (pay[pld + q] + A[0] * d0 + A[pld] * d1 + A[2 * pld] * d2);
+            p1 += w * (pay[2 * pld + q] + A[3 * pld] * d0 + A[4 * pld] * d1 + A[5 * pld] * d2);
+            p2 += w * (pay[3 * pld + q] + A[6 * pld] * d0 + A[7 * pld] * d1 + A[8 * pld] * d2);
+        }
+    });
+    a.gm[g] = m;
+    if (!MASS_ONLY) {
+        a.gp[g] = p0;
+        a.gp[a.gld + g] = p1;
+        a.gp[2 * a.gld + g] = p2;
+    }
+}
+
+// Pass 1 of the adjoint scatter: payload gvtot (3) | D^{-1} gCraw (9).
+template <typename T, int ORDER, bool REF>
+__global__ void __launch_bounds__(256) k_det_prep_adj(KP<T> a, unsigned* keys, int* idx, T* pay, long long pld) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const v3<T> x = load_v3(a.st, a.ld, PX, i);
+    Sten<T, ORDER> S;
+    S.init(a, x);
+    keys[i] = det_key(a, S.b);
+    idx[i] = i;
+    Win nowin;
+    nowin.lds = false;
+    v3<T> vn;
+    m3<T> Craw;
+    g2p_gather<T, ORDER>(a, nowin, nullptr, S, vn, Craw);
+    m3<T> Ft = load_m3(a.st, a.ld, PF, i);
+    if (a.ctl) Ft = madd(Ft, load_m3(a.ctl, a.ld, 0, i));
+    const Trust<T> tr = trust_region(a, Ft, Craw);
+    const Phase1<T, REF> p1 = phase1<T, REF>(a, i, Ft, Craw, tr);
+    const T dinv = Spline<ORDER>::dinv(a.dx);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) pay[q * pld + i] = p1.gvtot.a[q];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) pay[(3 + q) * pld + i] = dinv * p1.gCraw.a[q];
+}
+
+template <typename T, int ORDER>
+__global__ void __launch_bounds__(256) k_det_gather_adj(KP<T> a, const int* cstart, const int* cend, const int* idx,
+                                                        const T* pay, long long pld) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long G = (long long)a.nx * a.ny * a.nz;
+    if (g >= G) return;
+    const int k = (int)(g % a.nz), j = (int)((g / a.nz) % a.ny), i = (int)(g / ((long long)a.ny * a.nz));
+    T g0 = T(0), g1 = T(0), g2 = T(0);
+    det_visit<T, ORDER>(a, i, j, k, cstart, cend, idx, [&](int q, T u0, T u1, T u2, T w) {
+        const T d0 = u0 * a.dx, d1 = u1 * a.dx, d2 = u2 * a.dx;
+        const T* Gc = pay + 3 * pld + q;
+        g0 += w * (pay[q] + Gc[0] * d0 + Gc[pld] * d1 + Gc[2 * pld] * d2);
+        g1 += w * (pay[pld + q] + Gc[3 * pld] * d0 + Gc[4 * pld] * d1 + Gc[5 * pld] * d2);
+        g2 += w * (pay[2 * pld + q] + Gc[6 * pld] * d0 + Gc[7 * pld] * d1 + Gc[8 * pld] * d2);
+    });
+    a.ggv[g] = g0;
+    a.ggv[a.gld + g] = g1;
+    a.ggv[2 * a.gld + g] = g2;
+}
+
+// ---------------------------------------------------------------------------
 // Small element-wise kernels.
 template <typename T>
 __global__ void k_fill_zero(T* p, long long n) {
@@ -1056,6 +1317,18 @@ static void launch_g2p(const Launch& L, hipStream_t s) {
 }
 
 template <typename T>
+static void launch_step(const Launch& L, hipStream_t s) {
+    KP<T> k = make_kp<T>(L, 4);
+    const unsigned nb = grid_blocks(L.n, L.block);
+    const size_t sh = (size_t)k.cap * 4 * sizeof(T);
+    const int o = L.ph.order, md = L.ph.model;
+#define ST_CASE(O, M) hipLaunchKernelGGL((k_step<T, O, M>), dim3(nb), dim3(L.block), sh, s, k)
+    if (o == 3) { if (md == 0) ST_CASE(3, 0); else ST_CASE(3, 1); }
+    else { if (md == 0) ST_CASE(2, 0); else ST_CASE(2, 1); }
+#undef ST_CASE
+}
+
+template <typename T>
 static void launch_adj_scatter(const Launch& L, hipStream_t s) {
     KP<T> k = make_kp<T>(L, 6);
     const unsigned nb = grid_blocks(L.n, L.block);
@@ -1099,6 +1372,10 @@ void p2g(Prec pr, const Launch& L, bool mass_only, hipStream_t s) {
 void g2p(Prec pr, const Launch& L, hipStream_t s) {
     if (pr == PREC_F32) launch_g2p<float>(L, s);
     else launch_g2p<double>(L, s);
+}
+void step(Prec pr, const Launch& L, hipStream_t s) {
+    if (pr == PREC_F32) launch_step<float>(L, s);
+    else launch_step<double>(L, s);
 }
 void adj_scatter(Prec pr, const Launch& L, hipStream_t s) {
     if (pr == PREC_F32) launch_adj_scatter<float>(L, s);
@@ -1231,6 +1508,109 @@ void adam(Prec pr, const void* dLdF, void* dFc, void* mom, void* vec, void* vmax
         hipLaunchKernelGGL(k_adam<float>, dim3(nb), dim3(256), 0, s, (const float*)dLdF, (float*)dFc, (float*)mom, (float*)vec, (float*)vmax, n, ld, sqn, (float)alpha, (float)beta1, (float)beta2, (float)eps, timestep);
     else
         hipLaunchKernelGGL(k_adam<double>, dim3(nb), dim3(256), 0, s, (const double*)dLdF, (double*)dFc, (double*)mom, (double*)vec, (double*)vmax, n, ld, sqn, alpha, beta1, beta2, eps, timestep);
+}
+
+// ---- deterministic mode ----------------------------------------------------
+size_t det_scratch_bytes(int n, const int* dims, int esize) {
+    const long long ncells = (long long)(dims[0] + 8) * (dims[1] + 8) * (dims[2] + 8);
+    size_t temp = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                       (const int*)nullptr, (int*)nullptr, n, 0, 32, (hipStream_t)0);
+    const long long pld = (n + 63) / 64 * 64;
+    return 4 * (size_t)pld * 4 + 2 * (size_t)ncells * 4 + 13 * (size_t)pld * esize + temp + 1024;
+}
+
+struct DetLayout {
+    unsigned *kin, *kout;
+    int *iin, *iout, *cstart, *cend;
+    void* pay;
+    void* temp;
+    size_t temp_bytes;
+    long long pld, ncells;
+};
+
+static DetLayout det_layout(void* scratch, int n, const int* dims, int esize) {
+    DetLayout d;
+    d.ncells = (long long)(dims[0] + 8) * (dims[1] + 8) * (dims[2] + 8);
+    d.pld = (n + 63) / 64 * 64;
+    char* p = (char*)scratch;
+    auto take = [&](size_t bytes) {
+        void* r = p;
+        p += (bytes + 255) / 256 * 256;
+        return r;
+    };
+    d.kin = (unsigned*)take(d.pld * 4);
+    d.kout = (unsigned*)take(d.pld * 4);
+    d.iin = (int*)take(d.pld * 4);
+    d.iout = (int*)take(d.pld * 4);
+    d.cstart = (int*)take(d.ncells * 4);
+    d.cend = (int*)take(d.ncells * 4);
+    d.pay = take(13 * d.pld * esize);
+    d.temp_bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, d.temp_bytes, d.kin, d.kout, d.iin, d.iout, n, 0, 32, (hipStream_t)0);
+    d.temp = take(d.temp_bytes);
+    return d;
+}
+
+template <typename T>
+static void det_sort_ranges(const Launch& L, DetLayout& d, hipStream_t s) {
+    // empty cells: start = end = 0 (32-bit zero fill)
+    fill_zero(PREC_F32, d.cstart, d.ncells, s);
+    fill_zero(PREC_F32, d.cend, d.ncells, s);
+    (void)hipcub::DeviceRadixSort::SortPairs(d.temp, d.temp_bytes, d.kin, d.kout, d.iin, d.iout, L.n, 0, 32, s);
+    hipLaunchKernelGGL(k_det_cell_ranges, dim3((L.n + 255) / 256), dim3(256), 0, s, d.kout, L.n, d.cstart, d.cend);
+}
+
+template <typename T>
+static void launch_det_p2g(const Launch& L, bool mass_only, void* scratch, hipStream_t s) {
+    if (L.n <= 0) return;
+    KP<T> k = make_kp<T>(L, 0);
+    DetLayout d = det_layout(scratch, L.n, L.ph.dims, (int)sizeof(T));
+    const unsigned nbp = (unsigned)((L.n + 255) / 256);
+    const long long G = (long long)L.ph.dims[0] * L.ph.dims[1] * L.ph.dims[2];
+    const unsigned nbg = (unsigned)((G + 255) / 256);
+    T* pay = (T*)d.pay;
+    const int o = L.ph.order, md = L.ph.model;
+#define DP_CASE(O, M, MO) hipLaunchKernelGGL((k_det_prep_p2g<T, O, M, MO>), dim3(nbp), dim3(256), 0, s, k, d.kin, d.iin, pay, d.pld)
+    if (mass_only) { if (o == 3) DP_CASE(3, 0, true); else DP_CASE(2, 0, true); }
+    else if (o == 3) { if (md == 0) DP_CASE(3, 0, false); else DP_CASE(3, 1, false); }
+    else { if (md == 0) DP_CASE(2, 0, false); else DP_CASE(2, 1, false); }
+#undef DP_CASE
+    det_sort_ranges<T>(L, d, s);
+#define DG_CASE(O, MO) hipLaunchKernelGGL((k_det_gather_p2g<T, O, MO>), dim3(nbg), dim3(256), 0, s, k, d.cstart, d.cend, d.iout, (const T*)pay, d.pld)
+    if (mass_only) { if (o == 3) DG_CASE(3, true); else DG_CASE(2, true); }
+    else { if (o == 3) DG_CASE(3, false); else DG_CASE(2, false); }
+#undef DG_CASE
+}
+
+template <typename T>
+static void launch_det_adj(const Launch& L, void* scratch, hipStream_t s) {
+    if (L.n <= 0) return;
+    KP<T> k = make_kp<T>(L, 0);
+    DetLayout d = det_layout(scratch, L.n, L.ph.dims, (int)sizeof(T));
+    const unsigned nbp = (unsigned)((L.n + 255) / 256);
+    const long long G = (long long)L.ph.dims[0] * L.ph.dims[1] * L.ph.dims[2];
+    const unsigned nbg = (unsigned)((G + 255) / 256);
+    T* pay = (T*)d.pay;
+    const bool ref = L.ph.adjoint == 1;
+    const int o = L.ph.order;
+#define DA_CASE(O, R) hipLaunchKernelGGL((k_det_prep_adj<T, O, R>), dim3(nbp), dim3(256), 0, s, k, d.kin, d.iin, pay, d.pld)
+    if (o == 3) { if (ref) DA_CASE(3, true); else DA_CASE(3, false); }
+    else { if (ref) DA_CASE(2, true); else DA_CASE(2, false); }
+#undef DA_CASE
+    det_sort_ranges<T>(L, d, s);
+    if (o == 3) hipLaunchKernelGGL((k_det_gather_adj<T, 3>), dim3(nbg), dim3(256), 0, s, k, d.cstart, d.cend, d.iout, (const T*)pay, d.pld);
+    else hipLaunchKernelGGL((k_det_gather_adj<T, 2>), dim3(nbg), dim3(256), 0, s, k, d.cstart, d.cend, d.iout, (const T*)pay, d.pld);
+}
+
+void det_p2g(Prec pr, const Launch& L, bool mass_only, void* scratch, hipStream_t s) {
+    if (pr == PREC_F32) launch_det_p2g<float>(L, mass_only, scratch, s);
+    else launch_det_p2g<double>(L, mass_only, scratch, s);
+}
+
+void det_adj_scatter(Prec pr, const Launch& L, void* scratch, hipStream_t s) {
+    if (pr == PREC_F32) launch_det_adj<float>(L, scratch, s);
+    else launch_det_adj<double>(L, scratch, s);
 }
 
 }  // namespace dmpm

@@ -90,9 +90,19 @@ struct dmpm_sim {
     std::vector<char> want_gctl;
     std::vector<void*> gctl;   // dL/d dFc per layer
     std::vector<void*> seg;    // K segment buffers (recomputed states)
+    std::vector<void*> seg_grid;  // grids (m, p) of the segment's layers
     int seg_base = -1;
     void* scratch[2] = {nullptr, nullptr};
     void* grid = nullptr;  // m, p (4 planes)
+    void* grid2 = nullptr; // second grid for the fused forward step
+    void* det = nullptr;   // deterministic-mode scratch
+    void* det_buf() {
+        if (!det) {
+            const size_t b = det_scratch_bytes(n, cfg.dims, (int)esz);
+            det = dalloc(b);
+        }
+        return det;
+    }
     void* ggv = nullptr;   // 3 planes
     void* carry[2] = {nullptr, nullptr};
     int carry_cur = 0;
@@ -113,7 +123,7 @@ struct dmpm_sim {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double fwd_ms = 0, bwd_ms = 0;
     // per-kernel event timing (dmpm_kernel_timer): pairs of events per launch
-    static constexpr int NKIND = 5;  // p2g, g2p, adj_scatter, adj_gather, grid clear
+    static constexpr int NKIND = 6;  // p2g, g2p, adj_scatter, adj_gather, grid clear, fused step
     bool ktimer = false;
     std::vector<hipEvent_t> kpool;
     size_t knext = 0;
@@ -198,19 +208,39 @@ struct dmpm_sim {
         for (int t = 1; t < (int)layer.size(); ++t) dfree(layer[t]);
         // seg[0] aliases a stored layer; only seg[1..] are owned
         for (size_t j = 1; j < seg.size(); ++j) dfree(seg[j]);
+        for (auto& g : seg_grid) dfree(g);
         seg.clear();
+        seg_grid.clear();
         seg_base = -1;
         layer.resize(L, nullptr);
         valid.assign(L, 0);
         valid[0] = 1;
         for (int t = 1; t < L; ++t)
             if (stored(t)) layer[t] = dalloc(state_bytes());
-        if (K > 1) {
-            seg.resize(K, nullptr);
-            for (int j = 1; j < K; ++j) seg[j] = dalloc(state_bytes());
+        if (K > 1 && L > 1) {
+            const int k = std::min(K, L);
+            seg.resize(k, nullptr);
+            seg_grid.resize(k, nullptr);
+            for (int j = 1; j < k; ++j) seg[j] = dalloc(state_bytes());
+            for (int j = 0; j < k; ++j) seg_grid[j] = dalloc(4 * gld * esz);
         }
         for (auto& p : scratch)
             if (!p) p = dalloc(state_bytes());
+    }
+
+    // Recompute the K-layer segment holding layer t (K > 1): the states of
+    // layers c+1..top from the stored checkpoint c, and the grids (m, p) of
+    // layers c..top, which the backward steps of the segment read.
+    void ensure_segment(int t) {
+        const int c = (t / K) * K;
+        if (seg_base == c) return;
+        if (!valid[c]) throw Error(DMPM_ERR_STATE, "checkpoint layer " + std::to_string(c) + " missing");
+        const int top = std::min(c + K - 1, L - 2);
+        seg_base = -1;
+        seg[0] = layer[c];
+        p2g_into(seg[0], c, seg_grid[0]);
+        for (int j = c; j < top; ++j) fused_step(seg[j - c], j, seg_grid[j - c], seg[j - c + 1], seg_grid[j - c + 1]);
+        seg_base = c;
     }
 
     // Device pointer of the state of layer t, recomputing it from the nearest
@@ -220,28 +250,53 @@ struct dmpm_sim {
             if (!valid[t]) throw Error(DMPM_ERR_STATE, "layer " + std::to_string(t) + " has not been computed");
             return layer[t];
         }
-        const int c = (t / K) * K;
-        if (seg_base != c) {
-            if (!valid[c]) throw Error(DMPM_ERR_STATE, "checkpoint layer " + std::to_string(c) + " missing");
-            seg[0] = layer[c];
-            const int last = std::min(c + K - 1, L - 2);
-            for (int j = c; j < last; ++j) step(seg[j - c], j, seg[j - c + 1]);
-            seg_base = c;
-        }
-        return seg[t - c];
+        ensure_segment(t);
+        return seg[t - seg_base];
     }
 
-    // One forward step: state `in` of layer t -> `out`.
-    void step(const void* in, int t, void* out) {
+    // Zero `g` and P2G layer t's state `in` into it.
+    void p2g_into(const void* in, int t, void* g) {
         Launch Lc = launch();
         Lc.st = in;
         Lc.stat = stat;
         Lc.ctl = ctl[t];
-        Lc.grid = grid;
-        Lc.out = out;
-        timed(4, [&] { zero(grid, 4 * gld); });
+        Lc.grid = g;
+        if (cfg.deterministic) {
+            timed(0, [&] { det_p2g(pr, Lc, false, det_buf(), stream); });
+            return;
+        }
+        timed(4, [&] { zero(g, 4 * gld); });
         timed(0, [&] { p2g(pr, Lc, false, stream); });
+    }
+
+    // G2P of layer t (`in`, grid `g`) -> `out`; no P2G of the result.
+    void g2p_only(const void* in, int t, void* g, void* out) {
+        Launch Lc = launch();
+        Lc.st = in;
+        Lc.stat = stat;
+        Lc.ctl = ctl[t];
+        Lc.grid = g;
+        Lc.out = out;
         timed(1, [&] { g2p(pr, Lc, stream); });
+    }
+
+    // Layer t (`in`, grid `g`) -> layer t+1 (`out`) and its grid `g1`.
+    void fused_step(const void* in, int t, void* g, void* out, void* g1) {
+        Launch Lc = launch();
+        Lc.st = in;
+        Lc.stat = stat;
+        Lc.ctl = ctl[t];
+        Lc.grid = g;
+        Lc.out = out;
+        Lc.ctl1 = t + 1 < L ? ctl[t + 1] : nullptr;
+        Lc.grid1 = g1;
+        if (cfg.unfused || cfg.deterministic) {
+            timed(1, [&] { g2p(pr, Lc, stream); });
+            p2g_into(out, t + 1, g1);
+        } else {
+            timed(4, [&] { zero(g1, 4 * gld); });
+            timed(5, [&] { dmpm::step(pr, Lc, stream); });
+        }
     }
 
     void forward(int start, bool zero_controls = true) {
@@ -252,12 +307,21 @@ struct dmpm_sim {
         if (zero_controls)
             for (int t = start + 1; t < L; ++t) dfree(ctl[t]);
         if (start + 1 < L) {
+            // `cur` may live in a segment buffer, which this pass does not overwrite
             HIPCHK(hipEventRecord(ev[0], stream));
+            void* ga = grid;
+            void* gb = grid2;
+            p2g_into(cur, start, ga);
             int sc = 0;
             for (int t = start; t < L - 1; ++t) {
                 void* out = stored(t + 1) ? layer[t + 1] : scratch[sc];
                 if (!stored(t + 1)) sc ^= 1;
-                step(cur, t, out);
+                if (t + 1 < L - 1) {
+                    fused_step(cur, t, ga, out, gb);
+                    std::swap(ga, gb);
+                } else {
+                    g2p_only(cur, t, ga, out);
+                }
                 if (stored(t + 1)) valid[t + 1] = 1;
                 cur = out;
             }
@@ -280,12 +344,22 @@ struct dmpm_sim {
         carry_layer = L - 1;
         HIPCHK(hipEventRecord(ev[2], stream));
         for (int t = L - 2; t >= stop; --t) {
-            const void* st = state(t);
+            const void* st;
+            void* g;
+            if (K > 1) {
+                ensure_segment(t);
+                st = t == seg_base ? layer[t] : seg[t - seg_base];
+                g = seg_grid[t - seg_base];
+            } else {
+                st = state(t);
+                g = grid;
+                p2g_into(st, t, g);
+            }
             Launch Lc = launch();
             Lc.st = st;
             Lc.stat = stat;
             Lc.ctl = ctl[t];
-            Lc.grid = grid;
+            Lc.grid = g;
             Lc.ggv = ggv;
             Lc.adj1 = carry[carry_cur];
             Lc.adj0 = carry[carry_cur ^ 1];
@@ -296,10 +370,12 @@ struct dmpm_sim {
                 gout = gctl[t];
             }
             Lc.gctl = gout;
-            timed(4, [&] { zero(grid, 4 * gld); });
-            timed(0, [&] { p2g(pr, Lc, false, stream); });
-            timed(4, [&] { zero(ggv, 3 * gld); });
-            timed(2, [&] { adj_scatter(pr, Lc, stream); });
+            if (cfg.deterministic) {
+                timed(2, [&] { det_adj_scatter(pr, Lc, det_buf(), stream); });
+            } else {
+                timed(4, [&] { zero(ggv, 3 * gld); });
+                timed(2, [&] { adj_scatter(pr, Lc, stream); });
+            }
             timed(3, [&] { adj_gather(pr, Lc, stream); });
             if (t == stop && want_gctl[t]) {
                 if (!gctl[t]) gctl[t] = dalloc(ctl_bytes());
@@ -378,7 +454,9 @@ void validate(const dmpm_config& c) {
     if (c.adjoint != 0 && c.adjoint != 1) throw Error(DMPM_ERR_ARG, "adjoint must be 0 or 1");
     if (c.precision != 0 && c.precision != 1) throw Error(DMPM_ERR_ARG, "precision must be 0 or 1");
     if (c.checkpoint_interval < 0) throw Error(DMPM_ERR_ARG, "checkpoint_interval must be >= 0");
-    if (c.block != 0 && (c.block % 64 != 0 || c.block > 1024)) throw Error(DMPM_ERR_ARG, "block must be a multiple of 64, <= 1024");
+    if (c.block != 0 && (c.block % 64 != 0 || c.block > 256))
+        throw Error(DMPM_ERR_ARG, "block must be a multiple of 64, at most 256");
+    if (c.lds_bytes < 0 || c.lds_bytes > 64 * 1024) throw Error(DMPM_ERR_ARG, "lds_bytes must be in [0, 65536]");
 }
 
 // Morton key of a cell (10 bits per axis).
@@ -415,8 +493,12 @@ double end_layer_mass_loss(dmpm_sim* s) {
     Lc.st = st;
     Lc.stat = s->stat;
     Lc.grid = s->grid;
-    s->zero(s->grid, s->gld);
-    p2g(s->pr, Lc, true, s->stream);
+    if (s->cfg.deterministic) {
+        det_p2g(s->pr, Lc, true, s->det_buf(), s->stream);
+    } else {
+        s->zero(s->grid, s->gld);
+        p2g(s->pr, Lc, true, s->stream);
+    }
     const int nb = red_blocks_for(s->G);
     mass_sums(s->pr, s->grid, s->target_m, s->G, s->partial, nb, s->stream);
     reduce_partials(s->partial, nb, 2, s->red, s->stream);
@@ -526,6 +608,7 @@ int dmpm_create(const dmpm_config* cfg, dmpm_sim** out) {
         s->G = (long long)cfg->dims[0] * cfg->dims[1] * cfg->dims[2];
         s->gld = round_up(s->G, 64);
         s->grid = s->dalloc(4 * s->gld * s->esz);
+        s->grid2 = s->dalloc(4 * s->gld * s->esz);
         s->ggv = s->dalloc(3 * s->gld * s->esz);
         s->target_m = s->dalloc(s->gld * s->esz);
         s->dldm = s->dalloc(s->gld * s->esz);
@@ -592,14 +675,17 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
         for (auto* v2 : {&s->am, &s->av, &s->avmax, &s->gctl})
             for (auto& p : *v2) s->dfree(p);
         for (size_t j = 1; j < s->seg.size(); ++j) s->dfree(s->seg[j]);
+        for (auto& g : s->seg_grid) s->dfree(g);
         for (auto& p : s->scratch) s->dfree(p);
         s->seg.clear();
+        s->seg_grid.clear();
         s->dfree(s->stat);
         s->dfree((void*&)s->perm);
         for (auto& p : s->carry) s->dfree(p);
         s->dfree(s->adj_last);
         s->dfree(s->gctl_stop);
         s->dfree(s->rg);
+        s->dfree(s->det);
         s->has_rg = false;
 
         s->n = n;

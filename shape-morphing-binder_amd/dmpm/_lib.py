@@ -21,8 +21,8 @@ ADJOINT_EXACT, ADJOINT_REFERENCE = 0, 1
 (FIELD_X, FIELD_V, FIELD_F, FIELD_C, FIELD_DFC, FIELD_FTOTAL, FIELD_M, FIELD_VOL, FIELD_LAM,
  FIELD_MU) = range(10)
 
-KERNEL_P2G, KERNEL_G2P, KERNEL_ADJ_SCATTER, KERNEL_ADJ_GATHER, KERNEL_GRID_CLEAR = range(5)
-KERNEL_NAMES = ["p2g", "g2p", "adj_scatter", "adj_gather", "grid_clear"]
+KERNEL_P2G, KERNEL_G2P, KERNEL_ADJ_SCATTER, KERNEL_ADJ_GATHER, KERNEL_GRID_CLEAR, KERNEL_STEP = range(6)
+KERNEL_NAMES = ["p2g", "g2p", "adj_scatter", "adj_gather", "grid_clear", "step"]
 
 ERR_NAMES = {-1: "ERR_ARG", -2: "ERR_INDEX", -3: "ERR_HIP", -4: "ERR_STATE", -5: "ERR_NOMEM"}
 
@@ -47,6 +47,8 @@ class Config(C.Structure):
         ("checkpoint_interval", C.c_int),
         ("lds_bytes", C.c_int),
         ("block", C.c_int),
+        ("unfused", C.c_int),
+        ("deterministic", C.c_int),
     ]
 
 

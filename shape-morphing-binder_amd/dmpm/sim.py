@@ -46,7 +46,8 @@ class Physics:
 
 
 def make_config(dims, dx, grid_min=(0.0, 0.0, 0.0), physics: Optional[Physics] = None, precision: str = "f32",
-                checkpoint_interval: int = 1, lds_bytes: int = 0, block: int = 0) -> L.Config:
+                checkpoint_interval: int = 1, lds_bytes: int = 0, block: int = 0, unfused: bool = False,
+                deterministic: bool = False) -> L.Config:
     ph = physics or Physics()
     c = L.Config()
     gm = (C.c_double * 3)(*[float(a) for a in grid_min])
@@ -59,6 +60,8 @@ def make_config(dims, dx, grid_min=(0.0, 0.0, 0.0), physics: Optional[Physics] =
     c.precision = L.PREC_F64 if precision in ("f64", "float64", "fp64") else L.PREC_F32
     c.checkpoint_interval = int(checkpoint_interval)
     c.lds_bytes, c.block = int(lds_bytes), int(block)
+    c.unfused = 1 if unfused else 0
+    c.deterministic = 1 if deterministic else 0
     return c
 
 
@@ -117,14 +120,15 @@ class Simulator:
     scratch, adjoint carry, controls and Adam state."""
 
     def __init__(self, dims, dx, grid_min=(0.0, 0.0, 0.0), physics: Optional[Physics] = None,
-                 precision: str = "f32", checkpoint_interval: int = 1, lds_bytes: int = 0, block: int = 0):
+                 precision: str = "f32", checkpoint_interval: int = 1, lds_bytes: int = 0, block: int = 0,
+                 unfused: bool = False, deterministic: bool = False):
         self.dims = tuple(int(d) for d in dims)
         self.dx = float(dx)
         self.grid_min = tuple(float(a) for a in grid_min)
         self.physics = physics or Physics()
         self.precision = "f64" if precision in ("f64", "float64", "fp64") else "f32"
         self.cfg = make_config(dims, dx, grid_min, self.physics, self.precision, checkpoint_interval, lds_bytes,
-                               block)
+                               block, unfused, deterministic)
         h = C.c_void_p()
         L.check(L.lib().dmpm_create(C.byref(self.cfg), C.byref(h)))
         self._h = h
