@@ -103,10 +103,11 @@ def active_nodes(sc) -> int:
     return int((gm > 0).sum())
 
 
-def episode_sim(sc, prec, K, T, seed_gx, seed_gF, physics=None):
+def episode_sim(sc, prec, K, T, seed_gx, seed_gF, physics=None, lds_bytes=0, block=0):
     from dmpm import Physics, Simulator
     ph = physics or Physics.plain(dt=1e-4)
-    sim = Simulator(sc.dims, sc.dx, sc.grid_min, physics=ph, precision=prec, checkpoint_interval=K)
+    sim = Simulator(sc.dims, sc.dx, sc.grid_min, physics=ph, precision=prec, checkpoint_interval=K,
+                    lds_bytes=lds_bytes, block=block)
     sim.set_state(sc.state())
     sim.set_up(T + 1)
     sim.set_last_grad(x=seed_gx, F=seed_gF)
@@ -131,6 +132,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-reference-physics", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="use the default LDS budget and block size")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -151,10 +153,23 @@ def main():
     seed_gF = 1e-4 * rng.normal(size=(sc.n, 3, 3))
     gact = active_nodes(sc)
 
+    # ---- launch configuration (LDS window budget, block size) by the autotuner
+    tuned = None
+    lds_bytes, block = 0, 0
+    if not args.no_autotune:
+        from dmpm.autotune import tune
+        t0 = time.perf_counter()
+        tuned = tune(sc, Physics.plain(dt=1e-4), "f32", steps=20, seed_gx=seed_gx)
+        lds_bytes, block = tuned["lds_bytes"], tuned["block"]
+        log(f"[bench] autotune ({time.perf_counter() - t0:.1f}s): lds_bytes={lds_bytes} block={block}; "
+            + ", ".join(f"{r['lds_bytes'] // 1024}K/{r['block']}: {r['ms']:.1f}ms fb {r['fallback_frac']:.3f}"
+                        for r in tuned["table"]))
+
     # ---- FP32 (FP64 det/logJ/F^-T), checkpointed: the measured configuration
-    sim = episode_sim(sc, "f32", K, T, seed_gx, seed_gF)
+    sim = episode_sim(sc, "f32", K, T, seed_gx, seed_gF, lds_bytes=lds_bytes, block=block)
     run_episodes(sim, args.warmup)
     sim.synchronize()
+    sim.window_stats(reset=True)
     sim.kernel_timer(args.steps * 12 * T + 64)
     t0 = time.perf_counter()
     run_episodes(sim, args.steps)
@@ -162,6 +177,7 @@ def main():
     wall = time.perf_counter() - t0
     kstats = sim.kernel_stats()
     sim.kernel_timer(0)
+    win_lds, win_fb = sim.window_stats()
     value = sc.n * T * args.steps / wall
     ms_per_step = 1e3 * wall / args.steps
     mem_cur, mem_peak = sim.memory()
@@ -259,7 +275,10 @@ def main():
             "physics": "plain: quadratic B-spline, Neo-Hookean, sticky walls (3 cells)",
             "precision": "FP32 state/arithmetic, FP64 det/log J/F^-T", "E": 5e4, "nu": 0.45, "dt": 1e-4,
             "active_nodes": gact, "peak_device_bytes": mem_peak,
+            "lds_bytes": lds_bytes or 48 * 1024, "block": block or 256,
         },
+        "windows": {"lds_blocks": win_lds, "fallback_blocks": win_fb},
+        "autotune": tuned,
         "roofline": roofline,
         "step_roofline": {"bytes_per_particle_step": step_bytes / sc.n, "achieved_GBps":
                           step_bytes * value / sc.n / 1e9, "frac": step_frac},

@@ -197,6 +197,22 @@ int dmpm_calculate_volumes(const dmpm_config* cfg, int n, const void* x, const v
 /* calculate_lame_parameters(E, nu) (bind.cpp:540-544, Elasticity.cpp:149-153). */
 int dmpm_lame_parameters(double E, double nu, double* lam, double* mu);
 
+/* --- hipGraph capture of episodes ------------------------------------------ */
+/* No counterpart in the reference.  Captures, for each of `count` sims, the
+ * forward pass from `start_layer` (keeping every layer's dFc, as
+ * dmpm_rollout_forward) and the backward pass to `stop_layer`, each on its
+ * sim's stream, into one graph whose branches run concurrently (the batched
+ * workload: many independent instances in one launch).  Sims must have
+ * distinct streams and must not be in deterministic mode.  The graph reads
+ * the sims' buffers: set controls / seeds / particles through the usual calls
+ * (outside a launch) and relaunch; changing the number of layers, the
+ * particle count or the checkpoint interval needs a new capture. */
+typedef struct dmpm_graph dmpm_graph;
+int dmpm_graph_capture(dmpm_sim** sims, int count, int start_layer, int stop_layer, dmpm_graph** out);
+int dmpm_graph_launch(dmpm_graph* graph);
+int dmpm_graph_synchronize(dmpm_graph* graph);
+int dmpm_graph_destroy(dmpm_graph* graph);
+
 /* --- measurement ----------------------------------------------------------- */
 /* Time of the last forward/backward pass in ms (HIP events), and the number
  * of particle steps it covered. */
@@ -213,6 +229,9 @@ int dmpm_last_timing(const dmpm_sim* sim, double* fwd_ms, double* bwd_ms);
 #define DMPM_KERNEL_STEP 5
 int dmpm_kernel_timer(dmpm_sim* sim, int capacity);
 int dmpm_kernel_stats(dmpm_sim* sim, int kind, double* total_ms, int* count);
+/* Scatter windows since the last reset: blocks whose stencil box was held
+ * in LDS, and blocks that fell back to global atomics. */
+int dmpm_window_stats(dmpm_sim* sim, long long* lds_blocks, long long* fallback_blocks, int reset);
 /* Device bytes currently allocated by the sim, and the peak. */
 int dmpm_memory(const dmpm_sim* sim, size_t* current, size_t* peak);
 

@@ -46,6 +46,7 @@ struct Launch {
     void* gctl;        // dL/d dFc of layer t (output, 9 planes) or null
     const void* ctl1;  // fused step: dFc of layer t+1 or null
     void* grid1;       // fused step: grid t+1 (4 planes, zeroed by the caller)
+    unsigned long long* wstat;  // window counters: [0] blocks in LDS, [1] blocks falling back (or null)
     int lds_bytes;     // LDS budget per block for the window
     int block;         // threads per block (multiple of 64)
 };

@@ -48,6 +48,7 @@ struct KP {
     const T* ctl1;  // control of layer t+1 (fused step)
     T* gm1;         // grid t+1 (fused step): m, then 3 p planes
     T* gp1;
+    unsigned long long* wstat;
     int cap;
 };
 
@@ -87,7 +88,8 @@ static KP<T> make_kp(const Launch& L, int narrays) {
     k.ctl1 = (const T*)L.ctl1;
     k.gm1 = (T*)L.grid1;
     k.gp1 = L.grid1 ? (T*)L.grid1 + L.gld : nullptr;
-    k.cap = narrays > 0 ? (int)(L.lds_bytes / (narrays * (int)sizeof(T))) : 0;
+    k.wstat = L.wstat;
+    k.cap = narrays > 0 ? (int)(L.lds_bytes / narrays) : 0;  // narrays = LDS bytes per window node
     return k;
 }
 
@@ -156,6 +158,12 @@ DMPM_DI Win block_window(bool valid, int b0, int b1, int b2, int cap) {
     w.lds = vol <= (long long)cap;
     w.vol = w.lds ? (int)vol : 0;
     return w;
+}
+
+// Count a scatter window as held in LDS or fallen back to global atomics.
+template <typename T>
+DMPM_DI void count_window(const KP<T>& a, const Win& w) {
+    if (a.wstat && threadIdx.x == 0 && w.any) atomicAdd(a.wstat + (w.lds ? 0 : 1), 1ull);
 }
 
 template <typename T>
@@ -278,16 +286,104 @@ DMPM_DI void p2g_terms(const KP<T>& a, int i, const v3<T>& v, const m3<T>& Ft, c
     mv = {{m * v.a[0] * dragf, m * v.a[1] * dragf, m * v.a[2] * dragf}};
 }
 
+// Type of the LDS scatter accumulators.  ds_add_f64 sustains about twice
+// the throughput of ds_add_f32 on gfx950 under the same access pattern
+// (measured), so windows accumulate in FP64 for both precisions; the
+// flush converts once per node.
+using Acc = double;
+
+// Select a[i] from a small register array without dynamic indexing.
+template <int KS, typename T>
+DMPM_DI T pick(const T* a, int i) {
+    T r = a[0];
+#pragma unroll
+    for (int q = 1; q < KS; ++q) r = (i == q) ? a[q] : r;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Segmented reduction across lanes that share a base cell.  Particles are in
+// Hilbert order, so lanes of a wave with the same base cell -- and therefore
+// the same stencil nodes -- are adjacent.  Their contributions to each node
+// are summed across lanes with DPP row shifts (inside 8-lane chunks, 3
+// steps) and only the head lane of each run issues the LDS atomic: the atomic
+// count drops by the run length (~4 at 4-5 particles per cell), and LDS
+// atomics are what bounds the scatter.
+
+// lane i receives v of lane i+D (D = 1, 2, 4); lanes whose source is outside
+// their 16-lane row receive 0 (never used: runs stop at 8-lane chunk ends).
+template <int D>
+DMPM_DI float dpp_down(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + D, 0xF, 0xF, true));
+}
+template <int D>
+DMPM_DI double dpp_down(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x100 + D, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x100 + D, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+DMPM_DI int dpp_next(int k) { return __builtin_amdgcn_update_dpp(0, k, 0x101, 0xF, 0xF, true); }
+
+// Which Hillis-Steele steps a lane takes and whether it heads its run.
+struct SegRun {
+    bool a1, a2, a4, head;
+};
+
+// Runs are maximal sets of adjacent valid lanes inside one 8-lane chunk with
+// the same base cell b.  Every lane of the wave must call this, and every
+// cross-lane read below runs unconditionally: a read from a lane that is
+// switched off returns 0, which would split a run on one side only.  A lane
+// heads its run exactly when its left neighbour ends one, so the two views
+// always agree.  Values read across a 16-lane row boundary are never used:
+// such lanes sit at chunk ends.
+DMPM_DI SegRun seg_run(bool valid, const int* b) {
+    const int lane = threadIdx.x & 63;
+    const int vf = valid ? 1 : 0;
+    const int nv = dpp_next(vf);
+    const int n0 = dpp_next(b[0]);
+    const int n1 = dpp_next(b[1]);
+    const int n2 = dpp_next(b[2]);
+    const bool same_next = valid & (nv != 0) & (n0 == b[0]) & (n1 == b[1]) & (n2 == b[2]);
+    // f: the lane's run ends within [lane, lane + d - 1]
+    int f = (((lane & 7) == 7) | !same_next) ? 1 : 0;
+    const int end_prev = __builtin_amdgcn_update_dpp(1, f, 0x111, 0xF, 0xF, false);
+    SegRun r;
+    r.head = ((lane & 7) == 0) | (end_prev != 0);
+    r.a1 = !f;
+    f |= __builtin_amdgcn_update_dpp(1, f, 0x101, 0xF, 0xF, false);
+    r.a2 = !f;
+    f |= __builtin_amdgcn_update_dpp(1, f, 0x102, 0xF, 0xF, false);
+    r.a4 = !f;
+    return r;
+}
+
+// Sum v over the lane's run from this lane to the run's end; the head lane
+// ends with the run total.  Every lane of the wave must call this.
+template <typename T>
+DMPM_DI T seg_sum(T v, const SegRun& r) {
+    T o = dpp_down<1>(v);
+    v = r.a1 ? v + o : v;
+    o = dpp_down<2>(v);
+    v = r.a2 ? v + o : v;
+    o = dpp_down<4>(v);
+    v = r.a4 ? v + o : v;
+    return v;
+}
+
 // Scatter one particle's m and p into the LDS window (or the grid when the
-// window does not fit).
+// window does not fit).  Called by every lane; lanes without a particle pass
+// valid = false and contribute nothing.
 template <typename T, int ORDER, bool MASS_ONLY>
-DMPM_DI void p2g_scatter(const KP<T>& a, const Win& win, T* sm, const Sten<T, ORDER>& S, T m, const m3<T>& A,
-                         const v3<T>& mv, T* gm, T* gp) {
+DMPM_DI void p2g_scatter(const KP<T>& a, const Win& win, Acc* sm, const Sten<T, ORDER>& S, bool valid, T m,
+                         const m3<T>& A, const v3<T>& mv, T* gm, T* gp) {
     constexpr int KS = Spline<ORDER>::K;
-    T* lm = sm;
-    T* lp0 = sm + a.cap;
-    T* lp1 = sm + 2 * a.cap;
-    T* lp2 = sm + 3 * a.cap;
+    Acc* lm = sm;
+    Acc* lp0 = sm + a.cap;
+    Acc* lp1 = sm + 2 * a.cap;
+    Acc* lp2 = sm + 3 * a.cap;
+    const SegRun run = seg_run(valid, S.b);
+    const bool issue = valid && run.head;
 #pragma unroll
     for (int ia = 0; ia < KS; ++ia) {
         const int gi = S.b[0] + ia;
@@ -304,24 +400,31 @@ DMPM_DI void p2g_scatter(const KP<T>& a, const Win& win, T* sm, const Sten<T, OR
 #pragma unroll
             for (int ic = 0; ic < KS; ++ic) {
                 const int gk = S.b[2] + ic;
-                if (!in_grid(a, gi, gj, gk)) continue;
                 const T d2 = S.u[2][ic] * a.dx;
-                const T wt = w01 * S.w[2][ic];
+                const T wt = valid ? w01 * S.w[2][ic] : T(0);
+                const T cm = seg_sum(wt * m, run);
+                T c0 = T(0), c1 = T(0), c2 = T(0);
+                if (!MASS_ONLY) {
+                    c0 = seg_sum(wt * (q0 + A.a[2] * d2), run);
+                    c1 = seg_sum(wt * (q1 + A.a[5] * d2), run);
+                    c2 = seg_sum(wt * (q2 + A.a[8] * d2), run);
+                }
+                if (!issue || !in_grid(a, gi, gj, gk)) continue;
                 if (win.lds) {
                     const int li = ((gi - win.lo0) * win.e1 + (gj - win.lo1)) * win.e2 + (gk - win.lo2);
-                    atomicAdd(&lm[li], wt * m);
+                    atomicAdd(&lm[li], (Acc)cm);
                     if (!MASS_ONLY) {
-                        atomicAdd(&lp0[li], wt * (q0 + A.a[2] * d2));
-                        atomicAdd(&lp1[li], wt * (q1 + A.a[5] * d2));
-                        atomicAdd(&lp2[li], wt * (q2 + A.a[8] * d2));
+                        atomicAdd(&lp0[li], (Acc)c0);
+                        atomicAdd(&lp1[li], (Acc)c1);
+                        atomicAdd(&lp2[li], (Acc)c2);
                     }
                 } else {
                     const long long g = gindex(a, gi, gj, gk);
-                    unsafeAtomicAdd(gm + g, wt * m);
+                    unsafeAtomicAdd(gm + g, cm);
                     if (!MASS_ONLY) {
-                        unsafeAtomicAdd(gp + g, wt * (q0 + A.a[2] * d2));
-                        unsafeAtomicAdd(gp + a.gld + g, wt * (q1 + A.a[5] * d2));
-                        unsafeAtomicAdd(gp + 2 * a.gld + g, wt * (q2 + A.a[8] * d2));
+                        unsafeAtomicAdd(gp + g, c0);
+                        unsafeAtomicAdd(gp + a.gld + g, c1);
+                        unsafeAtomicAdd(gp + 2 * a.gld + g, c2);
                     }
                 }
             }
@@ -330,26 +433,26 @@ DMPM_DI void p2g_scatter(const KP<T>& a, const Win& win, T* sm, const Sten<T, OR
 }
 
 // Zero the window's accumulators (NA arrays).
-template <typename T, int NA>
-DMPM_DI void window_zero(const KP<T>& a, const Win& win, T* sm) {
+template <typename T, int NA, typename U>
+DMPM_DI void window_zero(const KP<T>& a, const Win& win, U* sm) {
     for (int t = threadIdx.x; t < win.vol; t += blockDim.x)
 #pragma unroll
-        for (int q = 0; q < NA; ++q) sm[q * a.cap + t] = T(0);
+        for (int q = 0; q < NA; ++q) sm[q * a.cap + t] = U(0);
 }
 
 // Flush the window to the grid: one atomic per touched node value.
 template <typename T, bool MASS_ONLY>
-DMPM_DI void p2g_flush(const KP<T>& a, const Win& win, const T* sm, T* gm, T* gp) {
+DMPM_DI void p2g_flush(const KP<T>& a, const Win& win, const Acc* sm, T* gm, T* gp) {
     const int e12 = win.e1 * win.e2;
     for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
         const int gi = win.lo0 + t / e12, gj = win.lo1 + (t / win.e2) % win.e1, gk = win.lo2 + t % win.e2;
         if (!in_grid(a, gi, gj, gk)) continue;
-        const T vm = sm[t];
+        const T vm = (T)sm[t];
         const long long g = gindex(a, gi, gj, gk);
         if (MASS_ONLY) {
             if (vm != T(0)) unsafeAtomicAdd(gm + g, vm);
         } else {
-            const T p0 = sm[a.cap + t], p1 = sm[2 * a.cap + t], p2 = sm[3 * a.cap + t];
+            const T p0 = (T)sm[a.cap + t], p1 = (T)sm[2 * a.cap + t], p2 = (T)sm[3 * a.cap + t];
             if (vm != T(0) || p0 != T(0) || p1 != T(0) || p2 != T(0)) {
                 unsafeAtomicAdd(gm + g, vm);
                 unsafeAtomicAdd(gp + g, p0);
@@ -366,7 +469,7 @@ __global__ void __launch_bounds__(256) k_p2g(KP<T> a) {
     constexpr int KS = Spline<ORDER>::K;
     constexpr int NA = MASS_ONLY ? 1 : 4;
     extern __shared__ double smem_d[];
-    T* sm = reinterpret_cast<T*>(smem_d);
+    Acc* sm = reinterpret_cast<Acc*>(smem_d);
     const int i = logical_block() * blockDim.x + threadIdx.x;
     const bool valid = i < a.n;
     Sten<T, ORDER> S;
@@ -378,7 +481,8 @@ __global__ void __launch_bounds__(256) k_p2g(KP<T> a) {
         S.b[0] = S.b[1] = S.b[2] = 0;
     }
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
-    if (win.lds) window_zero<T, NA>(a, win, sm);
+    count_window(a, win);
+    if (win.lds) window_zero<T, NA>(a, win, sm);  // Acc arrays
     T m = T(0);
     m3<T> A = mzero<T>();
     v3<T> mv = vzero<T>();
@@ -391,7 +495,7 @@ __global__ void __launch_bounds__(256) k_p2g(KP<T> a) {
         }
     }
     __syncthreads();
-    if (valid) p2g_scatter<T, ORDER, MASS_ONLY>(a, win, sm, S, m, A, mv, a.gm, a.gp);
+    p2g_scatter<T, ORDER, MASS_ONLY>(a, win, sm, S, valid, m, A, mv, a.gm, a.gp);
     if (win.lds && win.any) {
         __syncthreads();
         p2g_flush<T, MASS_ONLY>(a, win, sm, a.gm, a.gp);
@@ -519,7 +623,8 @@ template <typename T, int ORDER, int MODEL>
 __global__ void __launch_bounds__(256) k_step(KP<T> a) {
     constexpr int KS = Spline<ORDER>::K;
     extern __shared__ double smem_d[];
-    T* sm = reinterpret_cast<T*>(smem_d);
+    T* sm = reinterpret_cast<T*>(smem_d);     // staged velocities
+    Acc* sacc = reinterpret_cast<Acc*>(smem_d);  // then the P2G accumulators
     const int i = logical_block() * blockDim.x + threadIdx.x;
     const bool valid = i < a.n;
     Sten<T, ORDER> S;
@@ -547,7 +652,8 @@ __global__ void __launch_bounds__(256) k_step(KP<T> a) {
     if (valid) S1.init(a, xn);
     else S1.b[0] = S1.b[1] = S1.b[2] = 0;
     const Win win1 = block_window<KS>(valid, S1.b[0], S1.b[1], S1.b[2], a.cap);
-    if (win1.lds) window_zero<T, 4>(a, win1, sm);
+    count_window(a, win1);
+    if (win1.lds) window_zero<T, 4>(a, win1, sacc);
     T m = T(0);
     m3<T> A = mzero<T>();
     v3<T> mv = vzero<T>();
@@ -557,10 +663,10 @@ __global__ void __launch_bounds__(256) k_step(KP<T> a) {
         p2g_terms<T, ORDER, MODEL>(a, i, vn, Ft1, Cn, m, A, mv);
     }
     __syncthreads();
-    if (valid) p2g_scatter<T, ORDER, false>(a, win1, sm, S1, m, A, mv, a.gm1, a.gp1);
+    p2g_scatter<T, ORDER, false>(a, win1, sacc, S1, valid, m, A, mv, a.gm1, a.gp1);
     if (win1.lds && win1.any) {
         __syncthreads();
-        p2g_flush<T, false>(a, win1, sm, a.gm1, a.gp1);
+        p2g_flush<T, false>(a, win1, sacc, a.gm1, a.gp1);
     }
 }
 
@@ -635,7 +741,7 @@ __global__ void __launch_bounds__(256) k_adj_scatter(KP<T> a) {
     constexpr int KS = Spline<ORDER>::K;
     extern __shared__ double smem_d[];
     T* sv = reinterpret_cast<T*>(smem_d);  // 3 velocity arrays, then 3 accumulators
-    T* sg = sv + 3 * a.cap;
+    Acc* sg = reinterpret_cast<Acc*>(smem_d + (3 * a.cap * sizeof(T) + 7) / 8);
     const int i = logical_block() * blockDim.x + threadIdx.x;
     const bool valid = i < a.n;
     Sten<T, ORDER> S;
@@ -647,15 +753,18 @@ __global__ void __launch_bounds__(256) k_adj_scatter(KP<T> a) {
         S.b[0] = S.b[1] = S.b[2] = 0;
     }
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
+    count_window(a, win);
     if (win.lds) {
         stage_velocity(a, win, sv);
         for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
-            sg[t] = T(0);
-            sg[a.cap + t] = T(0);
-            sg[2 * a.cap + t] = T(0);
+            sg[t] = Acc(0);
+            sg[a.cap + t] = Acc(0);
+            sg[2 * a.cap + t] = Acc(0);
         }
         __syncthreads();
     }
+    v3<T> gvtot = vzero<T>();
+    m3<T> Gc = mzero<T>();
     if (valid) {
         v3<T> vn;
         m3<T> Craw;
@@ -664,8 +773,12 @@ __global__ void __launch_bounds__(256) k_adj_scatter(KP<T> a) {
         if (a.ctl) Ft = madd(Ft, load_m3(a.ctl, a.ld, 0, i));
         const Trust<T> tr = trust_region(a, Ft, Craw);
         const Phase1<T, REF> p1 = phase1<T, REF>(a, i, Ft, Craw, tr);
-        const T dinv = Spline<ORDER>::dinv(a.dx);
-        const m3<T> Gc = mscale(dinv, p1.gCraw);
+        gvtot = p1.gvtot;
+        Gc = mscale(Spline<ORDER>::dinv(a.dx), p1.gCraw);
+    }
+    {
+        const SegRun run = seg_run(valid, S.b);
+        const bool issue = valid && run.head;
 #pragma unroll
         for (int ia = 0; ia < KS; ++ia) {
             const int gi = S.b[0] + ia;
@@ -675,21 +788,23 @@ __global__ void __launch_bounds__(256) k_adj_scatter(KP<T> a) {
                 const int gj = S.b[1] + ib;
                 const T d1 = S.u[1][ib] * a.dx;
                 const T w01 = S.w[0][ia] * S.w[1][ib];
-                const T q0 = p1.gvtot.a[0] + Gc.a[0] * d0 + Gc.a[1] * d1;
-                const T q1 = p1.gvtot.a[1] + Gc.a[3] * d0 + Gc.a[4] * d1;
-                const T q2 = p1.gvtot.a[2] + Gc.a[6] * d0 + Gc.a[7] * d1;
+                const T q0 = gvtot.a[0] + Gc.a[0] * d0 + Gc.a[1] * d1;
+                const T q1 = gvtot.a[1] + Gc.a[3] * d0 + Gc.a[4] * d1;
+                const T q2 = gvtot.a[2] + Gc.a[6] * d0 + Gc.a[7] * d1;
 #pragma unroll
                 for (int ic = 0; ic < KS; ++ic) {
                     const int gk = S.b[2] + ic;
-                    if (!in_grid(a, gi, gj, gk)) continue;
                     const T d2 = S.u[2][ic] * a.dx;
-                    const T wt = w01 * S.w[2][ic];
-                    const T g0 = wt * (q0 + Gc.a[2] * d2), g1 = wt * (q1 + Gc.a[5] * d2), g2 = wt * (q2 + Gc.a[8] * d2);
+                    const T wt = valid ? w01 * S.w[2][ic] : T(0);
+                    const T g0 = seg_sum(wt * (q0 + Gc.a[2] * d2), run);
+                    const T g1 = seg_sum(wt * (q1 + Gc.a[5] * d2), run);
+                    const T g2 = seg_sum(wt * (q2 + Gc.a[8] * d2), run);
+                    if (!issue || !in_grid(a, gi, gj, gk)) continue;
                     if (win.lds) {
                         const int li = ((gi - win.lo0) * win.e1 + (gj - win.lo1)) * win.e2 + (gk - win.lo2);
-                        atomicAdd(&sg[li], g0);
-                        atomicAdd(&sg[a.cap + li], g1);
-                        atomicAdd(&sg[2 * a.cap + li], g2);
+                        atomicAdd(&sg[li], (Acc)g0);
+                        atomicAdd(&sg[a.cap + li], (Acc)g1);
+                        atomicAdd(&sg[2 * a.cap + li], (Acc)g2);
                     } else {
                         const long long g = gindex(a, gi, gj, gk);
                         unsafeAtomicAdd(a.ggv + g, g0);
@@ -706,7 +821,7 @@ __global__ void __launch_bounds__(256) k_adj_scatter(KP<T> a) {
         for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
             const int gi = win.lo0 + t / e12, gj = win.lo1 + (t / win.e2) % win.e1, gk = win.lo2 + t % win.e2;
             if (!in_grid(a, gi, gj, gk)) continue;
-            const T g0 = sg[t], g1 = sg[a.cap + t], g2 = sg[2 * a.cap + t];
+            const T g0 = (T)sg[t], g1 = (T)sg[a.cap + t], g2 = (T)sg[2 * a.cap + t];
             if (g0 != T(0) || g1 != T(0) || g2 != T(0)) {
                 const long long g = gindex(a, gi, gj, gk);
                 unsafeAtomicAdd(a.ggv + g, g0);
@@ -1292,9 +1407,9 @@ __global__ void k_adam(const T* dLdF, T* dFc, T* mom, T* vec, T* vmax, int n, lo
 template <typename T>
 static void launch_p2g(const Launch& L, bool mass_only, hipStream_t s) {
     const int na = mass_only ? 1 : 4;
-    KP<T> k = make_kp<T>(L, na);
+    KP<T> k = make_kp<T>(L, na * (int)sizeof(Acc));
     const unsigned nb = grid_blocks(L.n, L.block);
-    const size_t sh = (size_t)k.cap * na * sizeof(T);
+    const size_t sh = (size_t)k.cap * na * sizeof(Acc);
 #define P2G_CASE(O, M, MO) hipLaunchKernelGGL((k_p2g<T, O, M, MO>), dim3(nb), dim3(L.block), sh, s, k)
     const int o = L.ph.order, md = L.ph.model;
     if (mass_only) {
@@ -1309,7 +1424,7 @@ static void launch_p2g(const Launch& L, bool mass_only, hipStream_t s) {
 
 template <typename T>
 static void launch_g2p(const Launch& L, hipStream_t s) {
-    KP<T> k = make_kp<T>(L, 3);
+    KP<T> k = make_kp<T>(L, 3 * (int)sizeof(T));
     const unsigned nb = grid_blocks(L.n, L.block);
     const size_t sh = (size_t)k.cap * 3 * sizeof(T);
     if (L.ph.order == 3) hipLaunchKernelGGL((k_g2p<T, 3>), dim3(nb), dim3(L.block), sh, s, k);
@@ -1318,9 +1433,10 @@ static void launch_g2p(const Launch& L, hipStream_t s) {
 
 template <typename T>
 static void launch_step(const Launch& L, hipStream_t s) {
-    KP<T> k = make_kp<T>(L, 4);
+    // staging needs 3 T per node, the P2G accumulators 4 Acc per node
+    KP<T> k = make_kp<T>(L, 4 * (int)sizeof(Acc));
     const unsigned nb = grid_blocks(L.n, L.block);
-    const size_t sh = (size_t)k.cap * 4 * sizeof(T);
+    const size_t sh = (size_t)k.cap * 4 * sizeof(Acc);
     const int o = L.ph.order, md = L.ph.model;
 #define ST_CASE(O, M) hipLaunchKernelGGL((k_step<T, O, M>), dim3(nb), dim3(L.block), sh, s, k)
     if (o == 3) { if (md == 0) ST_CASE(3, 0); else ST_CASE(3, 1); }
@@ -1330,9 +1446,9 @@ static void launch_step(const Launch& L, hipStream_t s) {
 
 template <typename T>
 static void launch_adj_scatter(const Launch& L, hipStream_t s) {
-    KP<T> k = make_kp<T>(L, 6);
+    KP<T> k = make_kp<T>(L, 3 * (int)sizeof(T) + 3 * (int)sizeof(Acc) + 8);
     const unsigned nb = grid_blocks(L.n, L.block);
-    const size_t sh = (size_t)k.cap * 6 * sizeof(T);
+    const size_t sh = (size_t)k.cap * (3 * sizeof(T) + 3 * sizeof(Acc)) + 8;
     const bool ref = L.ph.adjoint == 1;
 #define AS_CASE(O, R) hipLaunchKernelGGL((k_adj_scatter<T, O, R>), dim3(nb), dim3(L.block), sh, s, k)
     if (L.ph.order == 3) { if (ref) AS_CASE(3, true); else AS_CASE(3, false); }
@@ -1342,7 +1458,7 @@ static void launch_adj_scatter(const Launch& L, hipStream_t s) {
 
 template <typename T>
 static void launch_adj_gather(const Launch& L, hipStream_t s) {
-    KP<T> k = make_kp<T>(L, 7);
+    KP<T> k = make_kp<T>(L, 7 * (int)sizeof(T));
     const unsigned nb = grid_blocks(L.n, L.block);
     const size_t sh = (size_t)k.cap * 7 * sizeof(T);
     const bool ref = L.ph.adjoint == 1;
@@ -1517,7 +1633,8 @@ size_t det_scratch_bytes(int n, const int* dims, int esize) {
     (void)hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned*)nullptr, (unsigned*)nullptr,
                                        (const int*)nullptr, (int*)nullptr, n, 0, 32, (hipStream_t)0);
     const long long pld = (n + 63) / 64 * 64;
-    return 4 * (size_t)pld * 4 + 2 * (size_t)ncells * 4 + 13 * (size_t)pld * esize + temp + 1024;
+    // 8 chunks, each rounded up to 256 bytes by det_layout
+    return 4 * (size_t)pld * 4 + 2 * (size_t)ncells * 4 + 13 * (size_t)pld * esize + temp + 8 * 256;
 }
 
 struct DetLayout {

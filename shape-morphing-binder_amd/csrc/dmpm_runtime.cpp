@@ -95,6 +95,7 @@ struct dmpm_sim {
     void* scratch[2] = {nullptr, nullptr};
     void* grid = nullptr;  // m, p (4 planes)
     void* grid2 = nullptr; // second grid for the fused forward step
+    unsigned long long* wstat = nullptr;  // scatter window counters (device, 2 entries)
     void* det = nullptr;   // deterministic-mode scratch
     void* det_buf() {
         if (!det) {
@@ -125,6 +126,7 @@ struct dmpm_sim {
     // per-kernel event timing (dmpm_kernel_timer): pairs of events per launch
     static constexpr int NKIND = 6;  // p2g, g2p, adj_scatter, adj_gather, grid clear, fused step
     bool ktimer = false;
+    bool capturing = false;  // inside dmpm_graph_capture: no timing events
     std::vector<hipEvent_t> kpool;
     size_t knext = 0;
     std::vector<std::pair<int, size_t>> krec;  // (kind, index of start event)
@@ -176,6 +178,7 @@ struct dmpm_sim {
         Lc.n = n;
         Lc.ld = ld;
         Lc.gld = gld;
+        Lc.wstat = wstat;
         Lc.lds_bytes = cfg.lds_bytes > 0 ? cfg.lds_bytes : 48 * 1024;
         Lc.block = cfg.block > 0 ? cfg.block : 256;
         return Lc;
@@ -308,7 +311,7 @@ struct dmpm_sim {
             for (int t = start + 1; t < L; ++t) dfree(ctl[t]);
         if (start + 1 < L) {
             // `cur` may live in a segment buffer, which this pass does not overwrite
-            HIPCHK(hipEventRecord(ev[0], stream));
+            if (!capturing) HIPCHK(hipEventRecord(ev[0], stream));
             void* ga = grid;
             void* gb = grid2;
             p2g_into(cur, start, ga);
@@ -325,7 +328,7 @@ struct dmpm_sim {
                 if (stored(t + 1)) valid[t + 1] = 1;
                 cur = out;
             }
-            HIPCHK(hipEventRecord(ev[1], stream));
+            if (!capturing) HIPCHK(hipEventRecord(ev[1], stream));
         }
         seg_base = -1;
     }
@@ -342,7 +345,7 @@ struct dmpm_sim {
         }
         copy_elems(pr, carry[carry_cur], adj_last, NSTATE * ld, stream);
         carry_layer = L - 1;
-        HIPCHK(hipEventRecord(ev[2], stream));
+        if (!capturing) HIPCHK(hipEventRecord(ev[2], stream));
         for (int t = L - 2; t >= stop; --t) {
             const void* st;
             void* g;
@@ -384,7 +387,7 @@ struct dmpm_sim {
             carry_cur ^= 1;
             carry_layer = t;
         }
-        HIPCHK(hipEventRecord(ev[3], stream));
+        if (!capturing) HIPCHK(hipEventRecord(ev[3], stream));
         if (stop == L - 1) {
             // no step: dL/d dFc of the last layer is dL/dF
             copy_elems(pr, gctl_stop, (char*)carry[carry_cur] + PF * ld * esz, 9 * ld, stream);
@@ -459,14 +462,38 @@ void validate(const dmpm_config& c) {
     if (c.lds_bytes < 0 || c.lds_bytes > 64 * 1024) throw Error(DMPM_ERR_ARG, "lds_bytes must be in [0, 65536]");
 }
 
-// Morton key of a cell (10 bits per axis).
-uint32_t spread10(uint32_t v) {
-    v &= 1023u;
-    v = (v | (v << 16)) & 0x030000FFu;
-    v = (v | (v << 8)) & 0x0300F00Fu;
-    v = (v | (v << 4)) & 0x030C30C3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
+// Position of a cell along a 3-D Hilbert curve over a 1024^3 lattice (10
+// bits per axis), after J. Skilling, "Programming the Hilbert curve" (AIP
+// Conf. Proc. 707, 2004): the coordinates are transformed in place into the
+// curve's "transposed" index, whose bits are then interleaved.  Unlike a
+// Morton (Z-order) curve the Hilbert curve has no jumps -- consecutive cells
+// are face neighbours -- so any run of ~60 cells lies inside two adjacent
+// 4x4x4 sub-cubes, which bounds the block windows of the particle kernels.
+uint32_t hilbert3(uint32_t x0, uint32_t x1, uint32_t x2) {
+    const int bits = 10;
+    uint32_t X[3] = {x0 & 1023u, x1 & 1023u, x2 & 1023u};
+    const uint32_t M = 1u << (bits - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    for (int i = 0; i < 3; ++i) X[i] ^= t;
+    uint32_t h = 0;
+    for (int b = bits - 1; b >= 0; --b)
+        for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
+    return h;
 }
 
 size_t dsize(int dtype) { return dtype == DMPM_DTYPE_F64 ? 8 : 4; }
@@ -615,6 +642,8 @@ int dmpm_create(const dmpm_config* cfg, dmpm_sim** out) {
         s->zero(s->target_m, s->gld);
         s->partial = (double*)s->dalloc(4 * 1024 * sizeof(double));
         s->red = (double*)s->dalloc(8 * sizeof(double));
+        s->wstat = (unsigned long long*)s->dalloc(2 * sizeof(unsigned long long));
+        s->zero(s->wstat, 4);  // 16 bytes
         *out = s.release();
     });
 }
@@ -650,20 +679,25 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
         if (n <= 0) throw Error(DMPM_ERR_ARG, "n must be positive");
         if (!x || !v || !F || !C || !m || !vol || !lam || !mu) throw Error(DMPM_ERR_ARG, "null particle array");
         const size_t es = dsize(dtype);
-        // Morton order of the cell of each particle (host)
+        // Hilbert order of the stencil base cell of each particle (host): the
+        // base cell is floor(x/dx - 1/2) for the quadratic B-spline and
+        // floor(x/dx) - 1 for the cubic one, so particles sharing a stencil
+        // end up adjacent and the scatter kernels can merge their sums
+        // across lanes before touching LDS.
         std::vector<char> xh((size_t)n * 3 * es);
         if (on_device) HIPCHK(hipMemcpy(xh.data(), x, xh.size(), hipMemcpyDeviceToHost));
         else std::memcpy(xh.data(), x, xh.size());
+        const double shift = s->cfg.order == 2 ? 0.5 : 0.0;
         std::vector<uint64_t> key(n);
         for (int i = 0; i < n; ++i) {
             uint32_t c[3];
             for (int a = 0; a < 3; ++a) {
                 double f = (host_get(xh.data(), dtype, 3 * (size_t)i + a) - s->cfg.grid_min[a]) / s->cfg.dx;
-                long long ci = (long long)std::floor(f);
+                long long ci = (long long)std::floor(f - shift) + 2;
                 ci = std::max(0LL, std::min(ci, 1023LL));
                 c[a] = (uint32_t)ci;
             }
-            key[i] = ((uint64_t)((spread10(c[0]) << 2) | (spread10(c[1]) << 1) | spread10(c[2])) << 32) | (uint32_t)i;
+            key[i] = ((uint64_t)hilbert3(c[0], c[1], c[2]) << 32) | (uint32_t)i;
         }
         std::sort(key.begin(), key.end());
         std::vector<int> perm(n);
@@ -1348,6 +1382,102 @@ int dmpm_last_timing(const dmpm_sim* s, double* fwd_ms, double* bwd_ms) {
     });
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// hipGraph capture of whole episodes.  A dmpm_graph holds one instantiated
+// graph with one branch per sim: each sim's forward pass (keeping controls)
+// from layer `start` and its backward pass to layer `stop`, recorded on the
+// sim's own stream, forked from and joined back to an origin stream.  The
+// branches are independent, so the GPU runs the instances concurrently.
+struct dmpm_graph {
+    hipStream_t origin = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> events;
+    int count = 0;
+    ~dmpm_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& e : events) (void)hipEventDestroy(e);
+        if (origin) (void)hipStreamDestroy(origin);
+    }
+};
+
+extern "C" {
+
+int dmpm_graph_capture(dmpm_sim** sims, int count, int start_layer, int stop_layer, dmpm_graph** out) {
+    return guard([&] {
+        if (!sims || count <= 0 || !out) throw Error(DMPM_ERR_ARG, "bad arguments");
+        auto g = std::make_unique<dmpm_graph>();
+        g->count = count;
+        HIPCHK(hipStreamCreateWithFlags(&g->origin, hipStreamNonBlocking));
+        g->events.resize(count + 1);
+        for (auto& e : g->events) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (int i = 0; i < count; ++i) {
+            dmpm_sim* s = sims[i];
+            check_sim(s);
+            if (s->n <= 0) throw Error(DMPM_ERR_STATE, "sim without particles");
+            if (s->cfg.deterministic) throw Error(DMPM_ERR_STATE, "deterministic mode cannot be captured");
+            for (int j = 0; j < i; ++j)
+                if (sims[j]->stream == s->stream) throw Error(DMPM_ERR_ARG, "sims must have distinct streams");
+            HIPCHK(hipStreamSynchronize(s->stream));
+            s->ktimer = false;
+            // everything the passes allocate lazily must exist before capture
+            for (int t = 0; t < s->L; ++t)
+                if (s->want_gctl[t] && !s->gctl[t]) s->gctl[t] = s->dalloc(s->ctl_bytes());
+        }
+        HIPCHK(hipStreamBeginCapture(g->origin, hipStreamCaptureModeRelaxed));
+        hipError_t cap_err = hipSuccess;
+        std::string msg;
+        try {
+            HIPCHK(hipEventRecord(g->events[count], g->origin));
+            for (int i = 0; i < count; ++i) {
+                dmpm_sim* s = sims[i];
+                HIPCHK(hipStreamWaitEvent(s->stream, g->events[count], 0));
+                s->capturing = true;
+                s->forward(start_layer, false);
+                s->backward(stop_layer);
+                s->capturing = false;
+                HIPCHK(hipEventRecord(g->events[i], s->stream));
+                HIPCHK(hipStreamWaitEvent(g->origin, g->events[i], 0));
+            }
+        } catch (const std::exception& e) {
+            msg = e.what();
+            cap_err = hipErrorUnknown;
+        }
+        for (int i = 0; i < count; ++i) sims[i]->capturing = false;
+        hipGraph_t graph = nullptr;
+        hipError_t end_err = hipStreamEndCapture(g->origin, &graph);
+        if (cap_err != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            throw Error(DMPM_ERR_HIP, "capture failed: " + msg);
+        }
+        HIPCHK(end_err);
+        g->graph = graph;
+        HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        *out = g.release();
+    });
+}
+
+int dmpm_graph_launch(dmpm_graph* g) {
+    return guard([&] {
+        if (!g || !g->exec) throw Error(DMPM_ERR_ARG, "null graph");
+        HIPCHK(hipGraphLaunch(g->exec, g->origin));
+    });
+}
+
+int dmpm_graph_synchronize(dmpm_graph* g) {
+    return guard([&] {
+        if (!g) throw Error(DMPM_ERR_ARG, "null graph");
+        HIPCHK(hipStreamSynchronize(g->origin));
+    });
+}
+
+int dmpm_graph_destroy(dmpm_graph* g) {
+    return guard([&] { delete g; });
+}
+
 int dmpm_kernel_timer(dmpm_sim* s, int capacity) {
     return guard([&] {
         check_sim(s);
@@ -1381,6 +1511,18 @@ int dmpm_kernel_stats(dmpm_sim* s, int kind, double* total_ms, int* count) {
         }
         if (total_ms) *total_ms = tot;
         if (count) *count = c;
+    });
+}
+
+int dmpm_window_stats(dmpm_sim* s, long long* lds_blocks, long long* fallback_blocks, int reset) {
+    return guard([&] {
+        check_sim(s);
+        unsigned long long h[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(h, s->wstat, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        if (lds_blocks) *lds_blocks = (long long)h[0];
+        if (fallback_blocks) *fallback_blocks = (long long)h[1];
+        if (reset) s->zero(s->wstat, 4);
     });
 }
 

@@ -124,9 +124,16 @@ def lib() -> C.CDLL:
         "dmpm_last_timing": (i, [vp, dp, dp]),
         "dmpm_memory": (i, [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
         "dmpm_kernel_timer": (i, [vp, i]),
+        "dmpm_window_stats": (i, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), i]),
+        "dmpm_graph_capture": (i, [C.POINTER(vp), i, i, i, C.POINTER(vp)]),
+        "dmpm_graph_launch": (i, [vp]),
+        "dmpm_graph_synchronize": (i, [vp]),
+        "dmpm_graph_destroy": (i, [vp]),
         "dmpm_kernel_stats": (i, [vp, i, dp, ip]),
     }
     for name, (res, args) in proto.items():
+        if not hasattr(L, name):
+            continue  # an older build; calling it raises AttributeError
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
@@ -144,7 +151,7 @@ EXPORTED = [
     "dmpm_set_control", "dmpm_request_control_grad", "dmpm_get_control_grad", "dmpm_control_grad_norm",
     "dmpm_descend_adam", "dmpm_optimize_single_timestep", "dmpm_run_optimization", "dmpm_p2g_mass",
     "dmpm_calculate_volumes", "dmpm_lame_parameters", "dmpm_last_timing", "dmpm_memory", "dmpm_kernel_timer",
-    "dmpm_kernel_stats",
+    "dmpm_kernel_stats", "dmpm_window_stats", "dmpm_graph_capture", "dmpm_graph_launch", "dmpm_graph_synchronize", "dmpm_graph_destroy",
 ]
 
 

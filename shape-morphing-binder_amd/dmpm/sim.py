@@ -296,6 +296,12 @@ class Simulator:
             out[name] = (ms.value, c.value)
         return out
 
+    def window_stats(self, reset: bool = False):
+        """(scatter blocks held in LDS, blocks that fell back to global atomics)."""
+        a, b = C.c_longlong(), C.c_longlong()
+        L.check(L.lib().dmpm_window_stats(self._h, C.byref(a), C.byref(b), 1 if reset else 0))
+        return a.value, b.value
+
     def memory(self):
         a, b = C.c_size_t(), C.c_size_t()
         L.check(L.lib().dmpm_memory(self._h, C.byref(a), C.byref(b)))
@@ -332,3 +338,35 @@ def lame_parameters(E, nu):
     lam, mu = C.c_double(), C.c_double()
     L.check(L.lib().dmpm_lame_parameters(float(E), float(nu), C.byref(lam), C.byref(mu)))
     return lam.value, mu.value
+
+
+class EpisodeGraph:
+    """One hipGraph holding a whole episode -- the forward pass from
+    `start` (keeping per-layer controls) and the backward pass to `stop` --
+    for each of `sims`, as concurrent branches.  launch() replays it; the
+    sims' inputs (particles, controls, last-layer seed) may be changed between
+    launches through the usual calls."""
+
+    def __init__(self, sims, start: int = 0, stop: int = 0):
+        arr = (C.c_void_p * len(sims))(*[s.handle for s in sims])
+        h = C.c_void_p()
+        L.check(L.lib().dmpm_graph_capture(arr, len(sims), int(start), int(stop), C.byref(h)))
+        self._h = h
+        self.sims = list(sims)
+
+    def launch(self):
+        L.check(L.lib().dmpm_graph_launch(self._h))
+
+    def synchronize(self):
+        L.check(L.lib().dmpm_graph_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().dmpm_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

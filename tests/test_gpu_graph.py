@@ -1,0 +1,59 @@
+"""hipGraph capture of whole episodes (one branch per instance): every replay
+must reproduce the eager episode, with the program's own eager work and
+synchronisations between replays."""
+import numpy as np
+import pytest
+
+from dmpm.scenes import Body, make_scene
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _instance(n, cells, seed, axes, E, nu, prec="f64", T=6, K=2):
+    from dmpm import Physics, Simulator
+    sc = make_scene(Body((0.5, 0.5, 0.5), axes), n, cells, seed, E, nu, velocity="sine", amplitude=0.3)
+    sim = Simulator(sc.dims, sc.dx, physics=Physics.plain(dt=5e-4), precision=prec, checkpoint_interval=K)
+    sim.set_state(sc.state())
+    sim.set_up(T + 1)
+    rng = np.random.default_rng(seed)
+    sim.set_control(1, 1e-3 * rng.normal(size=(sc.n, 3, 3)))
+    sim.request_control_grad(1)
+    sim.set_last_grad(x=rng.normal(size=(sc.n, 3)), F=1e-2 * rng.normal(size=(sc.n, 3, 3)))
+    return sim, T
+
+
+def _results(sim, T):
+    out = {k: sim.get(T, k, dtype=np.float64) for k in ("x", "F")}
+    out.update({"g" + k: sim.get_grad(0, k, dtype=np.float64) for k in ("x", "v", "F")})
+    out["gctl1"] = sim.get_control_grad(1, dtype=np.float64)
+    return out
+
+
+def test_graph_replays_match_eager():
+    from dmpm import EpisodeGraph
+    a, T = _instance(1500, 16, 1, (0.2, 0.22, 0.18), 1e4, 0.3)
+    b, _ = _instance(2600, 24, 2, (0.25, 0.2, 0.15), 3e4, 0.4)
+    c, _ = _instance(900, 12, 3, (0.2, 0.2, 0.2), 1e4, 0.3)
+    print("[graph] eager episodes", flush=True)
+    eager = []
+    for s in (a, b):
+        s.forward(0, keep_controls=True)
+        s.backward(0)
+        eager.append(_results(s, T))
+    g = EpisodeGraph([a, b], start=0, stop=0)
+    for rep in range(3):
+        print(f"[graph] replay {rep}", flush=True)
+        g.launch()
+        g.synchronize()
+        for s, e in zip((a, b), eager):
+            r = _results(s, T)
+            for k in e:
+                assert rel_err(r[k], e[k]) < 1e-12, (rep, k)
+        # eager work of the program between replays
+        c.forward(0, keep_controls=True)
+        c.backward(0)
+        c.synchronize()
+    g.close()
+    for s in (a, b, c):
+        s.close()
