@@ -133,6 +133,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-reference-physics", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="use the default LDS budget and block size")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (3 launches): the known-size kernel "
+                         "that scales the FETCH_SIZE/WRITE_SIZE counters of a --pmc run")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -140,8 +143,17 @@ def main():
               f"(value is the one GPU's throughput).", file=sys.stderr)
         sys.exit(2)
 
-    import torch  # noqa: F401  (one HIP runtime for torch and libdmpm)
+    import torch  # (one HIP runtime for torch and libdmpm)
     from dmpm import Physics
+
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 2.0, out=dst)  # one elementwise kernel: reads and writes MIB MiB
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: 3 launches, {src.numel() * 4} bytes read and written per launch")
+        del src, dst
     from dmpm.scenes import config3
 
     T, K = args.episode_steps, args.checkpoint

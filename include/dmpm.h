@@ -111,7 +111,7 @@ int dmpm_set_stream(dmpm_sim* sim, void* stream);
 int dmpm_synchronize(dmpm_sim* sim);
 
 /* Layer 0 particles (the PointCloud handed to CompGraph).  dFc may be NULL.
- * Particles are re-ordered internally (Morton order of their cell); every
+ * Particles are re-ordered internally (Hilbert order of their stencil base cell); every
  * export returns them in the caller's order. */
 int dmpm_set_particles(dmpm_sim* sim, int n, const void* x, const void* v, const void* F, const void* C,
                        const void* dFc, const void* m, const void* vol, const void* lam, const void* mu,

@@ -1,12 +1,13 @@
 // MPM step kernels for gfx950 (MI355X).
 //
 // One thread per particle, 256-thread blocks (4 wave64s).  Particles are kept
-// in Morton order of their grid cell, so a block's particles occupy a compact
-// box of the grid.  Every particle<->grid kernel first reduces that box (the
-// "window": the bounding box of the block's B-spline stencils) with wave
-// shuffles, then
-//   * scatters (P2G, adjoint of G2P) accumulate into the window in LDS with
-//     ds_add and flush it to HBM with one global atomic per touched node
+// in Hilbert order of their stencil base cell, so a block's particles occupy
+// a compact box of the grid.  Every particle<->grid kernel first reduces that
+// box (the "window": the bounding box of the block's B-spline stencils) with
+// wave shuffles, then
+//   * scatters (P2G, adjoint of G2P) sum each node value over runs of lanes
+//     that share a base cell, accumulate the run totals into the window in
+//     LDS (FP64) and flush it to HBM with one global atomic per touched node
 //     value, instead of one per particle-node pair;
 //   * gathers (G2P, adjoint of P2G) stage the window's node values in LDS
 //     once and read the stencil from there.
@@ -293,13 +294,6 @@ DMPM_DI void p2g_terms(const KP<T>& a, int i, const v3<T>& v, const m3<T>& Ft, c
 using Acc = double;
 
 // Select a[i] from a small register array without dynamic indexing.
-template <int KS, typename T>
-DMPM_DI T pick(const T* a, int i) {
-    T r = a[0];
-#pragma unroll
-    for (int q = 1; q < KS; ++q) r = (i == q) ? a[q] : r;
-    return r;
-}
 
 // ---------------------------------------------------------------------------
 // Segmented reduction across lanes that share a base cell.  Particles are in
@@ -325,9 +319,12 @@ DMPM_DI double dpp_down(double v) {
 }
 DMPM_DI int dpp_next(int k) { return __builtin_amdgcn_update_dpp(0, k, 0x101, 0xF, 0xF, true); }
 
-// Which Hillis-Steele steps a lane takes and whether it heads its run.
+// Which Hillis-Steele steps a lane takes (as 0/1 factors) and whether it
+// heads its run.
+template <typename T>
 struct SegRun {
-    bool a1, a2, a4, head;
+    T m1, m2, m4;
+    bool head;
 };
 
 // Runs are maximal sets of adjacent valid lanes inside one 8-lane chunk with
@@ -337,7 +334,8 @@ struct SegRun {
 // heads its run exactly when its left neighbour ends one, so the two views
 // always agree.  Values read across a 16-lane row boundary are never used:
 // such lanes sit at chunk ends.
-DMPM_DI SegRun seg_run(bool valid, const int* b) {
+template <typename T>
+DMPM_DI SegRun<T> seg_run(bool valid, const int* b) {
     const int lane = threadIdx.x & 63;
     const int vf = valid ? 1 : 0;
     const int nv = dpp_next(vf);
@@ -348,91 +346,109 @@ DMPM_DI SegRun seg_run(bool valid, const int* b) {
     // f: the lane's run ends within [lane, lane + d - 1]
     int f = (((lane & 7) == 7) | !same_next) ? 1 : 0;
     const int end_prev = __builtin_amdgcn_update_dpp(1, f, 0x111, 0xF, 0xF, false);
-    SegRun r;
+    SegRun<T> r;
     r.head = ((lane & 7) == 0) | (end_prev != 0);
-    r.a1 = !f;
+    r.m1 = f ? T(0) : T(1);
     f |= __builtin_amdgcn_update_dpp(1, f, 0x101, 0xF, 0xF, false);
-    r.a2 = !f;
+    r.m2 = f ? T(0) : T(1);
     f |= __builtin_amdgcn_update_dpp(1, f, 0x102, 0xF, 0xF, false);
-    r.a4 = !f;
+    r.m4 = f ? T(0) : T(1);
     return r;
 }
 
 // Sum v over the lane's run from this lane to the run's end; the head lane
-// ends with the run total.  Every lane of the wave must call this.
+// ends with the run total.  A step a lane does not take adds 0 * (the value
+// read), so every lane's v must be finite.  Every lane of the wave must call
+// this.
 template <typename T>
-DMPM_DI T seg_sum(T v, const SegRun& r) {
-    T o = dpp_down<1>(v);
-    v = r.a1 ? v + o : v;
-    o = dpp_down<2>(v);
-    v = r.a2 ? v + o : v;
-    o = dpp_down<4>(v);
-    v = r.a4 ? v + o : v;
+DMPM_DI T seg_sum(T v, const SegRun<T>& r) {
+    v = fma(dpp_down<1>(v), r.m1, v);
+    v = fma(dpp_down<2>(v), r.m2, v);
+    v = fma(dpp_down<4>(v), r.m4, v);
     return v;
 }
 
-// Scatter one particle's m and p into the LDS window (or the grid when the
-// window does not fit).  Called by every lane; lanes without a particle pass
-// valid = false and contribute nothing.
-template <typename T, int ORDER, bool MASS_ONLY>
-DMPM_DI void p2g_scatter(const KP<T>& a, const Win& win, Acc* sm, const Sten<T, ORDER>& S, bool valid, T m,
-                         const m3<T>& A, const v3<T>& mv, T* gm, T* gp) {
+// Window-relative index of stencil base node b (block_window guarantees that
+// every stencil node of a valid lane lies inside the window).
+DMPM_DI int window_base(const Win& w, const int* b) {
+    return ((b[0] - w.lo0) * w.e1 + (b[1] - w.lo1)) * w.e2 + (b[2] - w.lo2);
+}
+
+// Scatter NV values per stencil node: into the block's LDS window (NV Acc
+// arrays of a.cap) or, when the window did not fit, straight into the grid
+// planes dst[q] with global atomics.  val(ia, ib, ic, c) writes the lane's
+// contributions to node b + (ia, ib, ic); lanes without a particle must
+// produce finite values (they never issue).  Run sums merge the lanes that
+// share a base cell first.  Nodes outside the grid go to window slots that
+// the flush drops, or are skipped on the global path.
+template <typename T, int ORDER, int NV, typename Val>
+DMPM_DI void scatter_nodes(const KP<T>& a, const Win& win, Acc* sm, const Sten<T, ORDER>& S, bool valid,
+                           T* const* dst, Val&& val) {
     constexpr int KS = Spline<ORDER>::K;
-    Acc* lm = sm;
-    Acc* lp0 = sm + a.cap;
-    Acc* lp1 = sm + 2 * a.cap;
-    Acc* lp2 = sm + 3 * a.cap;
-    const SegRun run = seg_run(valid, S.b);
+    const SegRun<T> run = seg_run<T>(valid, S.b);
     const bool issue = valid && run.head;
+    if (win.lds) {
+        const int e2 = win.e2, e12 = win.e1 * win.e2;
+        const int lb = valid ? window_base(win, S.b) : 0;
 #pragma unroll
-    for (int ia = 0; ia < KS; ++ia) {
-        const int gi = S.b[0] + ia;
-        const T d0 = S.u[0][ia] * a.dx;
+        for (int ia = 0; ia < KS; ++ia) {
 #pragma unroll
-        for (int ib = 0; ib < KS; ++ib) {
-            const int gj = S.b[1] + ib;
-            const T d1 = S.u[1][ib] * a.dx;
-            const T w01 = S.w[0][ia] * S.w[1][ib];
-            // A dgp with dgp = (d0, d1, d2): the z-independent part first
-            const T q0 = mv.a[0] + A.a[0] * d0 + A.a[1] * d1;
-            const T q1 = mv.a[1] + A.a[3] * d0 + A.a[4] * d1;
-            const T q2 = mv.a[2] + A.a[6] * d0 + A.a[7] * d1;
+            for (int ib = 0; ib < KS; ++ib) {
+                const int row = lb + ia * e12 + ib * e2;
 #pragma unroll
-            for (int ic = 0; ic < KS; ++ic) {
-                const int gk = S.b[2] + ic;
-                const T d2 = S.u[2][ic] * a.dx;
-                const T wt = valid ? w01 * S.w[2][ic] : T(0);
-                const T cm = seg_sum(wt * m, run);
-                T c0 = T(0), c1 = T(0), c2 = T(0);
-                if (!MASS_ONLY) {
-                    c0 = seg_sum(wt * (q0 + A.a[2] * d2), run);
-                    c1 = seg_sum(wt * (q1 + A.a[5] * d2), run);
-                    c2 = seg_sum(wt * (q2 + A.a[8] * d2), run);
+                for (int ic = 0; ic < KS; ++ic) {
+                    T c[NV];
+                    val(ia, ib, ic, c);
+#pragma unroll
+                    for (int q = 0; q < NV; ++q) c[q] = seg_sum(c[q], run);
+                    if (issue) {
+#pragma unroll
+                        for (int q = 0; q < NV; ++q) atomicAdd(&sm[q * a.cap + row + ic], (Acc)c[q]);
+                    }
                 }
-                if (!issue || !in_grid(a, gi, gj, gk)) continue;
-                if (win.lds) {
-                    const int li = ((gi - win.lo0) * win.e1 + (gj - win.lo1)) * win.e2 + (gk - win.lo2);
-                    atomicAdd(&lm[li], (Acc)cm);
-                    if (!MASS_ONLY) {
-                        atomicAdd(&lp0[li], (Acc)c0);
-                        atomicAdd(&lp1[li], (Acc)c1);
-                        atomicAdd(&lp2[li], (Acc)c2);
-                    }
-                } else {
+            }
+        }
+    } else {
+#pragma unroll
+        for (int ia = 0; ia < KS; ++ia) {
+#pragma unroll
+            for (int ib = 0; ib < KS; ++ib) {
+#pragma unroll
+                for (int ic = 0; ic < KS; ++ic) {
+                    T c[NV];
+                    val(ia, ib, ic, c);
+#pragma unroll
+                    for (int q = 0; q < NV; ++q) c[q] = seg_sum(c[q], run);
+                    const int gi = S.b[0] + ia, gj = S.b[1] + ib, gk = S.b[2] + ic;
+                    if (!issue || !in_grid(a, gi, gj, gk)) continue;
                     const long long g = gindex(a, gi, gj, gk);
-                    unsafeAtomicAdd(gm + g, cm);
-                    if (!MASS_ONLY) {
-                        unsafeAtomicAdd(gp + g, c0);
-                        unsafeAtomicAdd(gp + a.gld + g, c1);
-                        unsafeAtomicAdd(gp + 2 * a.gld + g, c2);
-                    }
+#pragma unroll
+                    for (int q = 0; q < NV; ++q) unsafeAtomicAdd(dst[q] + g, c[q]);
                 }
             }
         }
     }
 }
 
-// Zero the window's accumulators (NA arrays).
+// Scatter one particle's m and p = m v + A (x_i - x_p) into the window (or
+// the grid).  Called by every lane; lanes without a particle pass m = 0,
+// A = 0, mv = 0.
+template <typename T, int ORDER, bool MASS_ONLY>
+DMPM_DI void p2g_scatter(const KP<T>& a, const Win& win, Acc* sm, const Sten<T, ORDER>& S, bool valid, T m,
+                         const m3<T>& A, const v3<T>& mv, T* gm, T* gp) {
+    T* const dst[4] = {gm, gp, gp + a.gld, gp + 2 * a.gld};
+    scatter_nodes<T, ORDER, MASS_ONLY ? 1 : 4>(a, win, sm, S, valid, dst, [&](int ia, int ib, int ic, T* c) {
+        const T d0 = S.u[0][ia] * a.dx, d1 = S.u[1][ib] * a.dx, d2 = S.u[2][ic] * a.dx;
+        const T wt = S.w[0][ia] * S.w[1][ib] * S.w[2][ic];
+        c[0] = wt * m;
+        if (!MASS_ONLY) {
+            c[1] = wt * (mv.a[0] + A.a[0] * d0 + A.a[1] * d1 + A.a[2] * d2);
+            c[2] = wt * (mv.a[1] + A.a[3] * d0 + A.a[4] * d1 + A.a[5] * d2);
+            c[3] = wt * (mv.a[2] + A.a[6] * d0 + A.a[7] * d1 + A.a[8] * d2);
+        }
+    });
+}
+
 template <typename T, int NA, typename U>
 DMPM_DI void window_zero(const KP<T>& a, const Win& win, U* sm) {
     for (int t = threadIdx.x; t < win.vol; t += blockDim.x)
@@ -474,12 +490,8 @@ __global__ void __launch_bounds__(256) k_p2g(KP<T> a) {
     const bool valid = i < a.n;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
-    if (valid) {
-        x = load_v3(a.st, a.ld, PX, i);
-        S.init(a, x);
-    } else {
-        S.b[0] = S.b[1] = S.b[2] = 0;
-    }
+    if (valid) x = load_v3(a.st, a.ld, PX, i);
+    S.init(a, x);  // lanes without a particle get a finite stencil at x = 0
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
     count_window(a, win);
     if (win.lds) window_zero<T, NA>(a, win, sm);  // Acc arrays
@@ -516,20 +528,6 @@ DMPM_DI void stage_velocity(const KP<T>& a, const Win& win, T* sv) {
     }
 }
 
-// Velocity of stencil node (gi,gj,gk), from LDS or from the grid.
-template <typename T>
-DMPM_DI void read_velocity(const KP<T>& a, const Win& win, const T* sv, int gi, int gj, int gk, T& v0, T& v1,
-                           T& v2) {
-    if (win.lds) {
-        const int li = ((gi - win.lo0) * win.e1 + (gj - win.lo1)) * win.e2 + (gk - win.lo2);
-        v0 = sv[li];
-        v1 = sv[a.cap + li];
-        v2 = sv[2 * a.cap + li];
-    } else {
-        node_velocity(a, gindex(a, gi, gj, gk), gi, gj, gk, v0, v1, v2);
-    }
-}
-
 // G2P gather (ForwardSimulation.cpp:89-123): v' = sum w v_i,
 // C' = D^{-1} sum w v_i (x_i - x_p)^T.
 template <typename T, int ORDER>
@@ -538,30 +536,41 @@ DMPM_DI void g2p_gather(const KP<T>& a, const Win& win, const T* sv, const Sten<
     constexpr int KS = Spline<ORDER>::K;
     vn = vzero<T>();
     Craw = mzero<T>();
+    auto add = [&](int ia, int ib, int ic, T v0, T v1, T v2) {
+        const T d0 = S.u[0][ia] * a.dx, d1 = S.u[1][ib] * a.dx, d2 = S.u[2][ic] * a.dx;
+        const T wt = S.w[0][ia] * S.w[1][ib] * S.w[2][ic];
+        const T w0 = wt * v0, w1 = wt * v1, w2 = wt * v2;
+        vn.a[0] += w0; vn.a[1] += w1; vn.a[2] += w2;
+        Craw.a[0] += w0 * d0; Craw.a[1] += w0 * d1; Craw.a[2] += w0 * d2;
+        Craw.a[3] += w1 * d0; Craw.a[4] += w1 * d1; Craw.a[5] += w1 * d2;
+        Craw.a[6] += w2 * d0; Craw.a[7] += w2 * d1; Craw.a[8] += w2 * d2;
+    };
+    if (win.lds) {
+        // window nodes outside the grid were staged with zero velocity
+        const int e2 = win.e2, e12 = win.e1 * win.e2;
+        const int lb = window_base(win, S.b);
 #pragma unroll
-    for (int ia = 0; ia < KS; ++ia) {
-        const int gi = S.b[0] + ia;
-        const T d0 = S.u[0][ia] * a.dx;
+        for (int ia = 0; ia < KS; ++ia)
 #pragma unroll
-        for (int ib = 0; ib < KS; ++ib) {
-            const int gj = S.b[1] + ib;
-            const T d1 = S.u[1][ib] * a.dx;
-            const T w01 = S.w[0][ia] * S.w[1][ib];
+            for (int ib = 0; ib < KS; ++ib) {
+                const int row = lb + ia * e12 + ib * e2;
 #pragma unroll
-            for (int ic = 0; ic < KS; ++ic) {
-                const int gk = S.b[2] + ic;
-                if (!in_grid(a, gi, gj, gk)) continue;
-                const T d2 = S.u[2][ic] * a.dx;
-                const T wt = w01 * S.w[2][ic];
-                T v0, v1, v2;
-                read_velocity(a, win, sv, gi, gj, gk, v0, v1, v2);
-                const T w0 = wt * v0, w1 = wt * v1, w2 = wt * v2;
-                vn.a[0] += w0; vn.a[1] += w1; vn.a[2] += w2;
-                Craw.a[0] += w0 * d0; Craw.a[1] += w0 * d1; Craw.a[2] += w0 * d2;
-                Craw.a[3] += w1 * d0; Craw.a[4] += w1 * d1; Craw.a[5] += w1 * d2;
-                Craw.a[6] += w2 * d0; Craw.a[7] += w2 * d1; Craw.a[8] += w2 * d2;
+                for (int ic = 0; ic < KS; ++ic)
+                    add(ia, ib, ic, sv[row + ic], sv[a.cap + row + ic], sv[2 * a.cap + row + ic]);
             }
-        }
+    } else {
+#pragma unroll
+        for (int ia = 0; ia < KS; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < KS; ++ib)
+#pragma unroll
+                for (int ic = 0; ic < KS; ++ic) {
+                    const int gi = S.b[0] + ia, gj = S.b[1] + ib, gk = S.b[2] + ic;
+                    if (!in_grid(a, gi, gj, gk)) continue;
+                    T v0, v1, v2;
+                    node_velocity(a, gindex(a, gi, gj, gk), gi, gj, gk, v0, v1, v2);
+                    add(ia, ib, ic, v0, v1, v2);
+                }
     }
     Craw = mscale(Spline<ORDER>::dinv(a.dx), Craw);
 }
@@ -597,12 +606,8 @@ __global__ void __launch_bounds__(256) k_g2p(KP<T> a) {
     const bool valid = i < a.n;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
-    if (valid) {
-        x = load_v3(a.st, a.ld, PX, i);
-        S.init(a, x);
-    } else {
-        S.b[0] = S.b[1] = S.b[2] = 0;
-    }
+    if (valid) x = load_v3(a.st, a.ld, PX, i);
+    S.init(a, x);  // lanes without a particle get a finite stencil at x = 0
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
     if (win.lds) {
         stage_velocity(a, win, sv);
@@ -629,12 +634,8 @@ __global__ void __launch_bounds__(256) k_step(KP<T> a) {
     const bool valid = i < a.n;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
-    if (valid) {
-        x = load_v3(a.st, a.ld, PX, i);
-        S.init(a, x);
-    } else {
-        S.b[0] = S.b[1] = S.b[2] = 0;
-    }
+    if (valid) x = load_v3(a.st, a.ld, PX, i);
+    S.init(a, x);  // lanes without a particle get a finite stencil at x = 0
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
     if (win.lds) {
         stage_velocity(a, win, sm);
@@ -649,8 +650,7 @@ __global__ void __launch_bounds__(256) k_step(KP<T> a) {
     // every wave is done with the staged velocities and the first window
     __syncthreads();
     Sten<T, ORDER> S1;
-    if (valid) S1.init(a, xn);
-    else S1.b[0] = S1.b[1] = S1.b[2] = 0;
+    S1.init(a, xn);
     const Win win1 = block_window<KS>(valid, S1.b[0], S1.b[1], S1.b[2], a.cap);
     count_window(a, win1);
     if (win1.lds) window_zero<T, 4>(a, win1, sacc);
@@ -746,12 +746,8 @@ __global__ void __launch_bounds__(256) k_adj_scatter(KP<T> a) {
     const bool valid = i < a.n;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
-    if (valid) {
-        x = load_v3(a.st, a.ld, PX, i);
-        S.init(a, x);
-    } else {
-        S.b[0] = S.b[1] = S.b[2] = 0;
-    }
+    if (valid) x = load_v3(a.st, a.ld, PX, i);
+    S.init(a, x);  // lanes without a particle get a finite stencil at x = 0
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
     count_window(a, win);
     if (win.lds) {
@@ -776,45 +772,16 @@ __global__ void __launch_bounds__(256) k_adj_scatter(KP<T> a) {
         gvtot = p1.gvtot;
         Gc = mscale(Spline<ORDER>::dinv(a.dx), p1.gCraw);
     }
-    {
-        const SegRun run = seg_run(valid, S.b);
-        const bool issue = valid && run.head;
-#pragma unroll
-        for (int ia = 0; ia < KS; ++ia) {
-            const int gi = S.b[0] + ia;
-            const T d0 = S.u[0][ia] * a.dx;
-#pragma unroll
-            for (int ib = 0; ib < KS; ++ib) {
-                const int gj = S.b[1] + ib;
-                const T d1 = S.u[1][ib] * a.dx;
-                const T w01 = S.w[0][ia] * S.w[1][ib];
-                const T q0 = gvtot.a[0] + Gc.a[0] * d0 + Gc.a[1] * d1;
-                const T q1 = gvtot.a[1] + Gc.a[3] * d0 + Gc.a[4] * d1;
-                const T q2 = gvtot.a[2] + Gc.a[6] * d0 + Gc.a[7] * d1;
-#pragma unroll
-                for (int ic = 0; ic < KS; ++ic) {
-                    const int gk = S.b[2] + ic;
-                    const T d2 = S.u[2][ic] * a.dx;
-                    const T wt = valid ? w01 * S.w[2][ic] : T(0);
-                    const T g0 = seg_sum(wt * (q0 + Gc.a[2] * d2), run);
-                    const T g1 = seg_sum(wt * (q1 + Gc.a[5] * d2), run);
-                    const T g2 = seg_sum(wt * (q2 + Gc.a[8] * d2), run);
-                    if (!issue || !in_grid(a, gi, gj, gk)) continue;
-                    if (win.lds) {
-                        const int li = ((gi - win.lo0) * win.e1 + (gj - win.lo1)) * win.e2 + (gk - win.lo2);
-                        atomicAdd(&sg[li], (Acc)g0);
-                        atomicAdd(&sg[a.cap + li], (Acc)g1);
-                        atomicAdd(&sg[2 * a.cap + li], (Acc)g2);
-                    } else {
-                        const long long g = gindex(a, gi, gj, gk);
-                        unsafeAtomicAdd(a.ggv + g, g0);
-                        unsafeAtomicAdd(a.ggv + a.gld + g, g1);
-                        unsafeAtomicAdd(a.ggv + 2 * a.gld + g, g2);
-                    }
-                }
-            }
-        }
-    }
+    // dL/dv_i += w (dL/dv' + D^{-1} dL/dC' (x_i - x_p)); lanes without a
+    // particle scatter zeros
+    T* const dst[3] = {a.ggv, a.ggv + a.gld, a.ggv + 2 * a.gld};
+    scatter_nodes<T, ORDER, 3>(a, win, sg, S, valid, dst, [&](int ia, int ib, int ic, T* c) {
+        const T d0 = S.u[0][ia] * a.dx, d1 = S.u[1][ib] * a.dx, d2 = S.u[2][ic] * a.dx;
+        const T wt = S.w[0][ia] * S.w[1][ib] * S.w[2][ic];
+        c[0] = wt * (gvtot.a[0] + Gc.a[0] * d0 + Gc.a[1] * d1 + Gc.a[2] * d2);
+        c[1] = wt * (gvtot.a[1] + Gc.a[3] * d0 + Gc.a[4] * d1 + Gc.a[5] * d2);
+        c[2] = wt * (gvtot.a[2] + Gc.a[6] * d0 + Gc.a[7] * d1 + Gc.a[8] * d2);
+    });
     if (win.lds && win.any) {
         __syncthreads();
         const int e12 = win.e1 * win.e2;
@@ -867,12 +834,8 @@ __global__ void __launch_bounds__(256) k_adj_gather(KP<T> a) {
     const bool valid = i < a.n;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
-    if (valid) {
-        x = load_v3(a.st, a.ld, PX, i);
-        S.init(a, x);
-    } else {
-        S.b[0] = S.b[1] = S.b[2] = 0;
-    }
+    if (valid) x = load_v3(a.st, a.ld, PX, i);
+    S.init(a, x);  // lanes without a particle get a finite stencil at x = 0
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
     if (win.lds) {
         const int e12 = win.e1 * win.e2;
@@ -922,58 +885,66 @@ __global__ void __launch_bounds__(256) k_adj_gather(KP<T> a) {
     m3<T> gA = mzero<T>();   // sum w h dgp^T
     v3<T> sh = vzero<T>();   // sum w h
     v3<T> dxa = vzero<T>();  // sum s_i grad w
+    const T invdx = T(-1) / a.dx;
+    auto add = [&](int ia, int ib, int ic, const T* v, const T* h, T gmass) {
+        const T d0 = S.u[0][ia] * a.dx, d1 = S.u[1][ib] * a.dx, d2 = S.u[2][ic] * a.dx;
+        const T wt = S.w[0][ia] * S.w[1][ib] * S.w[2][ic];
+        // grad_{x_p} w = -(1/dx) (N' N N, N N' N, N N N')
+        const T gw0 = invdx * S.dw[0][ia] * S.w[1][ib] * S.w[2][ic];
+        const T gw1 = invdx * S.w[0][ia] * S.dw[1][ib] * S.w[2][ic];
+        const T gw2 = invdx * S.w[0][ia] * S.w[1][ib] * S.dw[2][ic];
+        // q = mom v_p + Gt dgp ; (Gc dgp)
+        const T Gd0 = Gt.a[0] * d0 + Gt.a[1] * d1 + Gt.a[2] * d2;
+        const T Gd1 = Gt.a[3] * d0 + Gt.a[4] * d1 + Gt.a[5] * d2;
+        const T Gd2 = Gt.a[6] * d0 + Gt.a[7] * d1 + Gt.a[8] * d2;
+        const T Cd0 = Gc.a[0] * d0 + Gc.a[1] * d1 + Gc.a[2] * d2;
+        const T Cd1 = Gc.a[3] * d0 + Gc.a[4] * d1 + Gc.a[5] * d2;
+        const T Cd2 = Gc.a[6] * d0 + Gc.a[7] * d1 + Gc.a[8] * d2;
+        const T s = m * gmass + (mvp.a[0] + Gd0) * h[0] + (mvp.a[1] + Gd1) * h[1] + (mvp.a[2] + Gd2) * h[2] +
+                    v[0] * (p1.gvtot.a[0] + Cd0) + v[1] * (p1.gvtot.a[1] + Cd1) + v[2] * (p1.gvtot.a[2] + Cd2);
+        dxa.a[0] += s * gw0;
+        dxa.a[1] += s * gw1;
+        dxa.a[2] += s * gw2;
+        const T wh0 = wt * h[0], wh1 = wt * h[1], wh2 = wt * h[2];
+        sh.a[0] += wh0; sh.a[1] += wh1; sh.a[2] += wh2;
+        gA.a[0] += wh0 * d0; gA.a[1] += wh0 * d1; gA.a[2] += wh0 * d2;
+        gA.a[3] += wh1 * d0; gA.a[4] += wh1 * d1; gA.a[5] += wh1 * d2;
+        gA.a[6] += wh2 * d0; gA.a[7] += wh2 * d1; gA.a[8] += wh2 * d2;
+    };
+    if (win.lds) {
+        // window nodes outside the grid were staged as zeros
+        const int e2 = win.e2, e12 = win.e1 * win.e2;
+        const int lb = window_base(win, S.b);
 #pragma unroll
-    for (int ia = 0; ia < KS; ++ia) {
-        const int gi = S.b[0] + ia;
-        const T d0 = S.u[0][ia] * a.dx;
+        for (int ia = 0; ia < KS; ++ia)
 #pragma unroll
-        for (int ib = 0; ib < KS; ++ib) {
-            const int gj = S.b[1] + ib;
-            const T d1 = S.u[1][ib] * a.dx;
-            const T w01 = S.w[0][ia] * S.w[1][ib];
+            for (int ib = 0; ib < KS; ++ib) {
+                const int row = lb + ia * e12 + ib * e2;
 #pragma unroll
-            for (int ic = 0; ic < KS; ++ic) {
-                const int gk = S.b[2] + ic;
-                if (!in_grid(a, gi, gj, gk)) continue;
-                const T d2 = S.u[2][ic] * a.dx;
-                const T wt = w01 * S.w[2][ic];
-                T v[3], h[3], gmass;
-                if (win.lds) {
-                    const int li = ((gi - win.lo0) * win.e1 + (gj - win.lo1)) * win.e2 + (gk - win.lo2);
+                for (int ic = 0; ic < KS; ++ic) {
+                    const int li = row + ic;
+                    T v[3], h[3];
 #pragma unroll
                     for (int q = 0; q < 3; ++q) {
                         v[q] = sv[q * a.cap + li];
                         h[q] = sv[(3 + q) * a.cap + li];
                     }
-                    gmass = sv[6 * a.cap + li];
-                } else {
-                    node_adjoint<T, REF>(a, gindex(a, gi, gj, gk), gi, gj, gk, v, h, gmass);
+                    add(ia, ib, ic, v, h, sv[6 * a.cap + li]);
                 }
-                // grad_{x_p} w = -(1/dx) (N' N N, N N' N, N N N')
-                const T invdx = T(-1) / a.dx;
-                const T gw0 = invdx * S.dw[0][ia] * S.w[1][ib] * S.w[2][ic];
-                const T gw1 = invdx * S.w[0][ia] * S.dw[1][ib] * S.w[2][ic];
-                const T gw2 = invdx * S.w[0][ia] * S.w[1][ib] * S.dw[2][ic];
-                // q = mom v_p + Gt dgp ; (Gc dgp)
-                const T Gd0 = Gt.a[0] * d0 + Gt.a[1] * d1 + Gt.a[2] * d2;
-                const T Gd1 = Gt.a[3] * d0 + Gt.a[4] * d1 + Gt.a[5] * d2;
-                const T Gd2 = Gt.a[6] * d0 + Gt.a[7] * d1 + Gt.a[8] * d2;
-                const T Cd0 = Gc.a[0] * d0 + Gc.a[1] * d1 + Gc.a[2] * d2;
-                const T Cd1 = Gc.a[3] * d0 + Gc.a[4] * d1 + Gc.a[5] * d2;
-                const T Cd2 = Gc.a[6] * d0 + Gc.a[7] * d1 + Gc.a[8] * d2;
-                const T s = m * gmass + (mvp.a[0] + Gd0) * h[0] + (mvp.a[1] + Gd1) * h[1] + (mvp.a[2] + Gd2) * h[2] +
-                            v[0] * (p1.gvtot.a[0] + Cd0) + v[1] * (p1.gvtot.a[1] + Cd1) +
-                            v[2] * (p1.gvtot.a[2] + Cd2);
-                dxa.a[0] += s * gw0;
-                dxa.a[1] += s * gw1;
-                dxa.a[2] += s * gw2;
-                const T wh0 = wt * h[0], wh1 = wt * h[1], wh2 = wt * h[2];
-                sh.a[0] += wh0; sh.a[1] += wh1; sh.a[2] += wh2;
-                gA.a[0] += wh0 * d0; gA.a[1] += wh0 * d1; gA.a[2] += wh0 * d2;
-                gA.a[3] += wh1 * d0; gA.a[4] += wh1 * d1; gA.a[5] += wh1 * d2;
-                gA.a[6] += wh2 * d0; gA.a[7] += wh2 * d1; gA.a[8] += wh2 * d2;
             }
-        }
+    } else {
+#pragma unroll
+        for (int ia = 0; ia < KS; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < KS; ++ib)
+#pragma unroll
+                for (int ic = 0; ic < KS; ++ic) {
+                    const int gi = S.b[0] + ia, gj = S.b[1] + ib, gk = S.b[2] + ic;
+                    if (!in_grid(a, gi, gj, gk)) continue;
+                    T v[3], h[3], gmass;
+                    node_adjoint<T, REF>(a, gindex(a, gi, gj, gk), gi, gj, gk, v, h, gmass);
+                    add(ia, ib, ic, v, h, gmass);
+                }
     }
     // dL/dx_p: sum s_i grad w - Gt^T sum w h - D^{-1} gCraw^T v' + dL/dx'
     const v3<T> gx1 = load_v3(a.adj1, a.ld, PX, i);
