@@ -120,6 +120,101 @@ def run_episodes(sim, count):
         sim.backward(0)
 
 
+def batched_workload(dt: float = 2.5e-4, T: int = 100, K: int = 10, reps: int = 2) -> dict:
+    """64 independent instances of uneven size (particle counts log-uniform
+    in [4,096, 131,072], grids of 32/48/64/96 cells, random ellipsoids and
+    materials), each a T-step forward + backward episode seeded at its last
+    layer: one at a time (eager calls per instance) against one hipGraph
+    holding every instance's episode as a concurrent branch."""
+    from dmpm import EpisodeGraph, Physics, Simulator
+    from dmpm.scenes import config4_instances, instance_scene
+    sims, total = [], 0
+    for spec in config4_instances():
+        sc = instance_scene(spec)
+        sim = Simulator(sc.dims, sc.dx, sc.grid_min, physics=Physics.plain(dt=dt), precision="f32",
+                        checkpoint_interval=K)
+        sim.set_state(sc.state())
+        sim.set_up(T + 1)
+        rng = np.random.default_rng(spec["seed"])
+        sim.set_last_grad(x=1e-3 * rng.normal(size=(sc.n, 3)), F=1e-4 * rng.normal(size=(sc.n, 3, 3)))
+        sims.append(sim)
+        total += sc.n
+
+    def sequential():
+        for s in sims:
+            run_episodes(s, 1)
+            s.synchronize()
+
+    log(f"[bench] batched: {len(sims)} instances set up, {total} particles")
+    sequential()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sequential()
+    w_seq = (time.perf_counter() - t0) / reps
+    log(f"[bench] batched: one at a time {1e3 * w_seq:.1f} ms")
+    t0 = time.perf_counter()
+    g = EpisodeGraph(sims, start=0, stop=0)
+    log(f"[bench] batched: graph captured and instantiated ({time.perf_counter() - t0:.2f}s)")
+    g.launch()
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.launch()
+    g.synchronize()
+    w_graph = (time.perf_counter() - t0) / reps
+    g.close()
+    peak = sum(s.memory()[1] for s in sims)
+    for s in sims:
+        s.close()
+    out = {"instances": len(sims), "particles_total": total, "episode_steps": T, "checkpoint_interval": K,
+           "dt": dt, "one_at_a_time": {"value": total * T / w_seq, "ms_per_step": 1e3 * w_seq},
+           "graph": {"value": total * T / w_graph, "ms_per_step": 1e3 * w_graph},
+           "speedup": w_seq / w_graph, "peak_device_bytes": peak, "unit": "particle-steps/s"}
+    log(f"[bench] batched: {len(sims)} instances, {total} particles: one at a time {1e3 * w_seq:.1f} ms, "
+        f"graph {1e3 * w_graph:.1f} ms per episode set ({out['speedup']:.2f}x)")
+    return out
+
+
+def full_scale_episode(T: int = 500, K: int = 25, controls: int = 20) -> dict:
+    """4,194,304 particles in a sphere of radius 0.3 on a 256^3 grid, FP32
+    with FP64 det/log J/F^-T, dt = 5e-5, a 500-step episode with checkpoint
+    interval 25, Gaussian dL/dx (sigma 1e-3) and dL/dF (sigma 1e-4) injected
+    at the last layer and backpropagated to a per-particle control at 20
+    control layers (every 25th)."""
+    from dmpm import Physics, Simulator
+    from dmpm.scenes import config5
+    sc = config5()
+    sim = Simulator(sc.dims, sc.dx, sc.grid_min, physics=Physics.plain(dt=5e-5), precision="f32",
+                    checkpoint_interval=K)
+    sim.set_state(sc.state())
+    sim.set_up(T + 1)
+    zero = np.zeros((sc.n, 3, 3), np.float32)
+    layers = list(range(0, T, T // controls))[:controls]
+    for t in layers:
+        sim.set_control(t, zero)
+        sim.request_control_grad(t)
+    rng = np.random.default_rng(5)
+    sim.set_last_grad(x=1e-3 * rng.normal(size=(sc.n, 3)), F=1e-4 * rng.normal(size=(sc.n, 3, 3)))
+    log(f"[bench] full scale: {sc.n} particles set up")
+    run_episodes(sim, 1)
+    sim.synchronize()
+    log("[bench] full scale: warm-up episode done")
+    t0 = time.perf_counter()
+    run_episodes(sim, 1)
+    sim.synchronize()
+    w = time.perf_counter() - t0
+    g = sim.get_control_grad(layers[0], dtype=np.float64)
+    _, peak = sim.memory()
+    sim.close()
+    out = {"particles": sc.n, "grid_nodes": list(sc.dims), "episode_steps": T, "checkpoint_interval": K,
+           "control_layers": len(layers), "value": sc.n * T / w, "unit": "particle-steps/s",
+           "ms_per_step": 1e3 * w, "peak_device_bytes": peak,
+           "control_grad_finite": bool(np.isfinite(g).all()), "control_grad_rms": float(np.sqrt((g ** 2).mean()))}
+    log(f"[bench] full scale: {1e3 * w:.0f} ms/episode, {out['value']:.4g} particle-steps/s, "
+        f"peak {peak / 1e9:.1f} GB")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +228,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-reference-physics", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="use the default LDS budget and block size")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-instance batched workload")
+    ap.add_argument("--no-full-scale", action="store_true", help="skip the 4M-particle, 256^3, 500-step episode")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (3 launches): the known-size kernel "
                          "that scales the FETCH_SIZE/WRITE_SIZE counters of a --pmc run")
@@ -177,21 +274,26 @@ def main():
             + ", ".join(f"{r['lds_bytes'] // 1024}K/{r['block']}: {r['ms']:.1f}ms fb {r['fallback_frac']:.3f}"
                         for r in tuned["table"]))
 
-    # ---- FP32 (FP64 det/logJ/F^-T), checkpointed: the measured configuration
+    # ---- FP32 (FP64 det/logJ/F^-T), checkpointed: the measured configuration.
+    # Eager episodes (the library queues a whole episode asynchronously, so a
+    # replay of the captured episode graph measures the same), then one more
+    # episode with the per-kernel HIP-event timers on.
     sim = episode_sim(sc, "f32", K, T, seed_gx, seed_gF, lds_bytes=lds_bytes, block=block)
     run_episodes(sim, args.warmup)
     sim.synchronize()
-    sim.window_stats(reset=True)
-    sim.kernel_timer(args.steps * 12 * T + 64)
     t0 = time.perf_counter()
     run_episodes(sim, args.steps)
     sim.synchronize()
     wall = time.perf_counter() - t0
+    value = sc.n * T * args.steps / wall
+    ms_per_step = 1e3 * wall / args.steps
+    sim.window_stats(reset=True)
+    sim.kernel_timer(12 * T + 64)
+    run_episodes(sim, 1)
+    sim.synchronize()
     kstats = sim.kernel_stats()
     sim.kernel_timer(0)
     win_lds, win_fb = sim.window_stats()
-    value = sc.n * T * args.steps / wall
-    ms_per_step = 1e3 * wall / args.steps
     mem_cur, mem_peak = sim.memory()
     log(f"[bench] f32 K={K}: {ms_per_step:.1f} ms/episode, {value:.4g} particle-steps/s, peak {mem_peak / 1e9:.2f} GB")
 
@@ -261,6 +363,11 @@ def main():
                                       "physics": "cubic B-spline, fixed-corotated, bulk viscosity 0.25, "
                                                  "trust region 0.6, reference adjoint"}
         log(f"[bench] reference physics: {1e3 * wr:.1f} ms/episode")
+
+    if not args.no_batched:
+        extra["batched"] = batched_workload()
+    if not args.no_full_scale:
+        extra["full_scale"] = full_scale_episode()
 
     cpu = None
     if not args.no_cpu:

@@ -181,6 +181,8 @@ def config4_instances(seed: int = 4, count: int = 64, nmin: int = 4096, nmax: in
     return out
 
 
-def instance_scene(spec: dict) -> Scene:
+def instance_scene(spec: dict, amplitude: float = 0.3) -> Scene:
+    """One instance of the batched workload, set in motion by a seeded sine
+    velocity field (so the episode is not a rest state)."""
     return make_scene(Body((0.5, 0.5, 0.5), spec["axes"]), spec["n"], spec["cells"], spec["seed"],
-                      spec["E"], spec["nu"])
+                      spec["E"], spec["nu"], velocity="sine", amplitude=amplitude)
