@@ -50,6 +50,20 @@ BYTES = {
 }
 
 
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of the default workload (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "k_step_1M_128cubed_hbm_traffic.md")
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+                return int(f[2])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -311,8 +325,10 @@ def main():
         kernels[name] = entry
     dom = max((k for k in kernels if k in BYTES), key=lambda k: kernels[k]["total_ms"])
     d = kernels[dom]
+    default_workload = (args.particles, args.cells, T, K) == (1048576, 128, 300, 25)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": d["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                "frac": d["GBps"] / HBM_PEAK_GBS,
+                "traffic": profiled_traffic("k_" + dom) if default_workload else None,
                 "bytes_per_launch": d["bytes_per_launch"], "avg_launch_us": d["avg_us"]}
 
     # whole-step algorithmic roofline: 532 B per particle-step + 128 B per active node
