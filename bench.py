@@ -111,6 +111,25 @@ def cpu_baseline(sc, seed_gx, steps: int = 2):
             "sample": f"{n} particles, {steps} fwd+bwd steps of the oracle (plain physics)"}
 
 
+def stream_copy_GBps(mib: int = 1024, reps: int = 5) -> float:
+    """Achievable HBM bandwidth: a plain elementwise copy (read + write of
+    `mib` MiB each) timed with HIP events on torch's stream."""
+    import torch
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        torch.mul(src, 2.0, out=dst)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        torch.mul(src, 2.0, out=dst)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2 * (mib << 20) / (ms * 1e-3) / 1e9
+
+
 def active_nodes(sc) -> int:
     from dmpm import p2g_mass
     gm = p2g_mass(sc.dims, sc.dx, sc.grid_min, sc.x, sc.m, order=2)
@@ -267,6 +286,8 @@ def main():
         del src, dst
     from dmpm.scenes import config3
 
+    copy_bw = stream_copy_GBps()
+    log(f"[bench] plain copy: {copy_bw:.0f} GB/s")
     T, K = args.episode_steps, args.checkpoint
     t0 = time.perf_counter()
     sc = config3(n=args.particles, cells=args.cells)
@@ -329,7 +350,8 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": d["GBps"] / HBM_PEAK_GBS,
                 "traffic": profiled_traffic("k_" + dom) if default_workload else None,
-                "bytes_per_launch": d["bytes_per_launch"], "avg_launch_us": d["avg_us"]}
+                "bytes_per_launch": d["bytes_per_launch"], "avg_launch_us": d["avg_us"],
+                "achievable_copy_GBps": copy_bw}
 
     # whole-step algorithmic roofline: 532 B per particle-step + 128 B per active node
     step_bytes = 532 * sc.n + 128 * gact

@@ -63,6 +63,10 @@ extern "C" {
 #define DMPM_FIELD_VOL 7
 #define DMPM_FIELD_LAM 8
 #define DMPM_FIELD_MU 9
+#define DMPM_FIELD_P 10         /* PK1 stress P(F + dFc) (MaterialPoint::P after P_op_1) */
+#define DMPM_FIELD_ADAM_M 11    /* Adam state of the layer's control (MaterialPoint::momentum) */
+#define DMPM_FIELD_ADAM_V 12    /* (MaterialPoint::vector)     */
+#define DMPM_FIELD_ADAM_VMAX 13 /* (MaterialPoint::vector_max) */
 
 typedef struct dmpm_sim dmpm_sim;
 
@@ -99,6 +103,8 @@ typedef struct {
 void dmpm_config_init(dmpm_config* cfg, int nx, int ny, int nz, double dx, const double grid_min[3]);
 
 const char* dmpm_last_error(void);
+/* The configuration a sim was created with. */
+int dmpm_get_config(const dmpm_sim* sim, dmpm_config* out);
 const char* dmpm_version(void);
 
 /* --- lifetime -------------------------------------------------------------- */
@@ -171,6 +177,11 @@ int dmpm_set_control(dmpm_sim* sim, int layer, const void* dFc, int dtype, int o
 /* Keep dL/d dFc of `layer` from the next backward passes (1) or not (0);
  * read it with dmpm_get_control_grad. */
 int dmpm_request_control_grad(dmpm_sim* sim, int layer, int on);
+/* The Adam state of `layer`'s control (MaterialPoint::momentum, vector,
+ * vector_max, PointCloud.cpp:83-147), e.g. when resuming an optimisation;
+ * read it back with dmpm_get_layer(..., DMPM_FIELD_ADAM_*, ...). */
+int dmpm_set_adam_state(dmpm_sim* sim, int layer, const void* m, const void* v, const void* vmax, int dtype,
+                        int on_device);
 int dmpm_get_control_grad(dmpm_sim* sim, int layer, void* out, int dtype, int on_device);
 /* PointCloud::Compute_dLdF_Norm of the control layer's gradient (PointCloud.cpp:24-35). */
 int dmpm_control_grad_norm(dmpm_sim* sim, double* norm);
@@ -207,6 +218,47 @@ int dmpm_lame_parameters(double E, double nu, double* lam, double* mu);
  * the sims' buffers: set controls / seeds / particles through the usual calls
  * (outside a launch) and relaunch; changing the number of layers, the
  * particle count or the checkpoint interval needs a new capture. */
+/* --- on-disk particle state ------------------------------------------------
+ * DMPM_FILE_CEREAL: the reference's PointCloud binary file
+ *   (PointCloud::WriteEntirePointCloudToBinaryFile / ReadEntirePointCloudFromBinaryFile,
+ *   PointCloud.cpp:382-410, cereal BinaryOutputArchive of MaterialPoint.h:55-71):
+ *   uint64 n, then per point x, v, F, C, P, m, vol, lam, mu, dFc, dLdx, dLdv,
+ *   dLdF, momentum, vector, vector_max; a 3-vector is int32 3 + 3 float32, a
+ *   3x3 matrix int32 3, int32 3 + 9 float32 row by row, scalars float32
+ *   (432 bytes per point, little endian).
+ * DMPM_FILE_NATIVE: this library's checkpoint format, little endian:
+ *   header (64 bytes): char magic[8] = "DMPMSTAT"; uint32 version = 1;
+ *     uint32 dtype (DMPM_DTYPE_*); uint64 n; uint32 nfields; int32 layer the
+ *     file was written from (-1: from arrays); 32 reserved bytes (0);
+ *   field table: nfields x { char name[16] (NUL padded); uint32 ncomp;
+ *     uint32 reserved };
+ *   data: per field in table order, n x ncomp values of dtype, particle-major,
+ *     3x3 matrices row by row.
+ *   Field names as in the cereal record (x v F C P m vol lam mu dFc dLdx dLdv
+ *   dLdF momentum vector vector_max); readers require x v F C m vol lam mu and
+ *   take absent optional fields as zeros.
+ * Arrays are host memory in the caller's particle order: vectors (n,3),
+ * matrices (n,3,3), scalars (n,). */
+#define DMPM_FILE_NATIVE 0
+#define DMPM_FILE_CEREAL 1
+typedef struct {
+    void *x, *v, *F, *C, *P, *m, *vol, *lam, *mu, *dFc, *dLdx, *dLdv, *dLdF, *adam_m, *adam_v, *adam_vmax;
+} dmpm_particle_arrays;
+/* Particle count (and value type) of a file. */
+int dmpm_file_info(const char* path, int format, int* n, int* dtype);
+/* Write arrays (x v F C m vol lam mu required; NULL optional fields are
+ * written as zeros).  A cereal file always holds float32. */
+int dmpm_write_file(const char* path, int format, int n, const dmpm_particle_arrays* arrays, int dtype);
+/* Read a file of exactly n particles into the non-NULL arrays. */
+int dmpm_read_file(const char* path, int format, int n, dmpm_particle_arrays* arrays, int dtype);
+/* Write one layer of a sim: its state, control, statics, stress P(F + dFc),
+ * gradients where the sim keeps them (last layer, control layer of the last
+ * backward pass; zeros elsewhere) and Adam state (PointCloud.cpp:382-394). */
+int dmpm_write_layer(dmpm_sim* sim, int layer, const char* path, int format);
+/* dmpm_set_particles from a file, plus layer 0's Adam state when the file
+ * holds a non-zero one (PointCloud.cpp:396-410). */
+int dmpm_read_particles(dmpm_sim* sim, const char* path, int format);
+
 typedef struct dmpm_graph dmpm_graph;
 int dmpm_graph_capture(dmpm_sim** sims, int count, int start_layer, int stop_layer, dmpm_graph** out);
 int dmpm_graph_launch(dmpm_graph* graph);

@@ -461,3 +461,27 @@ class Ref:
         lam, mu = C.c_float(), C.c_float()
         self._lib.ref_lame(C.c_float(E), C.c_float(nu), C.byref(lam), C.byref(mu))
         return lam.value, mu.value
+
+    CEREAL_FIELDS = ("x", "v", "F", "C", "P", "m", "vol", "lam", "mu", "dFc", "dLdx", "dLdv", "dLdF",
+                     "momentum", "vector", "vector_max")
+
+    def write_cereal(self, path: str, a: dict):
+        """PointCloud::WriteEntirePointCloudToBinaryFile of a point cloud
+        holding the arrays `a` (keys CEREAL_FIELDS)."""
+        args = [self._f3(a[k]) for k in self.CEREAL_FIELDS]
+        n = args[0].shape[0]
+        self._lib.ref_write_cereal(int(n), *[_ptr(x) for x in args], str(path).encode())
+
+    def read_cereal(self, path: str) -> dict:
+        """PointCloud::ReadEntirePointCloudFromBinaryFile: x, F, P, m and
+        vector_max of the points read."""
+        self._lib.ref_read_cereal.restype = C.c_int
+        n = self._lib.ref_read_cereal(str(path).encode(), 0, None, None, None, None, None)
+        if n < 0:
+            raise RuntimeError(f"the reference could not read {path}")
+        o = dict(x=np.zeros((n, 3), np.float32), F=np.zeros((n, 3, 3), np.float32),
+                 P=np.zeros((n, 3, 3), np.float32), m=np.zeros(n, np.float32),
+                 vector_max=np.zeros((n, 3, 3), np.float32))
+        self._lib.ref_read_cereal(str(path).encode(), int(n), _ptr(o["x"]), _ptr(o["F"]), _ptr(o["P"]),
+                                  _ptr(o["m"]), _ptr(o["vector_max"]))
+        return o

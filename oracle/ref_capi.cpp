@@ -362,6 +362,57 @@ void ref_polar(int n, const float* F, float* R, float* S) {
         store_m3(s, S + 9 * p);
     }
 }
+// PointCloud::WriteEntirePointCloudToBinaryFile (PointCloud.cpp:382-394) of
+// a point cloud holding exactly the given fields (MaterialPoint.h:55-71):
+// vectors (N,3), matrices (N,3,3), scalars (N,).  Returns 0 on success.
+int ref_write_cereal(int n, const float* x, const float* v, const float* F, const float* C, const float* P,
+                     const float* m, const float* vol, const float* lam, const float* mu, const float* dFc,
+                     const float* dLdx, const float* dLdv, const float* dLdF, const float* mom,
+                     const float* vec, const float* vmax, const char* path) {
+    PointCloud pc;
+    pc.points.resize(n);
+    for (int p = 0; p < n; ++p) {
+        MaterialPoint& mp = pc.points[p];
+        mp.x = load_v3(x + 3 * p);
+        mp.v = load_v3(v + 3 * p);
+        mp.F = load_m3(F + 9 * p);
+        mp.C = load_m3(C + 9 * p);
+        mp.P = load_m3(P + 9 * p);
+        mp.m = m[p];
+        mp.vol = vol[p];
+        mp.lam = lam[p];
+        mp.mu = mu[p];
+        mp.dFc = load_m3(dFc + 9 * p);
+        mp.dLdx = load_v3(dLdx + 3 * p);
+        mp.dLdv = load_v3(dLdv + 3 * p);
+        mp.dLdF = load_m3(dLdF + 9 * p);
+        mp.momentum = load_m3(mom + 9 * p);
+        mp.vector = load_m3(vec + 9 * p);
+        mp.vector_max = load_m3(vmax + 9 * p);
+    }
+    pc.WriteEntirePointCloudToBinaryFile(path);
+    return 0;
+}
+
+// PointCloud::ReadEntirePointCloudFromBinaryFile (PointCloud.cpp:396-410):
+// the number of points of the file, then (second call, arrays non-null) its
+// x, F, P, m and vector_max fields.
+int ref_read_cereal(const char* path, int n_max, float* x, float* F, float* P, float* m, float* vmax) {
+    PointCloud pc;
+    if (!pc.ReadEntirePointCloudFromBinaryFile(path)) return -1;
+    const int n = (int)pc.points.size();
+    if (!x) return n;
+    for (int p = 0; p < n && p < n_max; ++p) {
+        const MaterialPoint& mp = pc.points[p];
+        store_v3(mp.x, x + 3 * p);
+        store_m3(mp.F, F + 9 * p);
+        store_m3(mp.P, P + 9 * p);
+        m[p] = mp.m;
+        store_m3(mp.vector_max, vmax + 9 * p);
+    }
+    return n;
+}
+
 void ref_lame(float E, float nu, float* lam, float* mu) {
     CalculateLameParameters(E, nu, *lam, *mu);
 }

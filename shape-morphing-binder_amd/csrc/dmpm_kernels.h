@@ -10,6 +10,8 @@
 //                  gv0..2 (adjoint dL/dv_i).
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <string>
 #include <cstddef>
 #include <cstdint>
 
@@ -51,6 +53,10 @@ struct Launch {
     int block;         // threads per block (multiple of 64)
 };
 
+// Record the message dmpm_last_error() returns (for library parts outside
+// the runtime's own error guard).
+void set_last_error(const std::string& msg);
+
 // Forward: P2G of layer `st` (stress computed inside) into `grid`.
 // mass_only: scatter m only (the dt = 0 P2G used by the loss and volumes).
 void p2g(Prec pr, const Launch& L, bool mass_only, hipStream_t s);
@@ -83,6 +89,10 @@ void import_planes(Prec pr, const void* src, int src_f64, void* dst, const int* 
 void export_planes(Prec pr, const void* src, void* dst, int dst_f64, const int* perm, int n, int nc,
                    long long ld, hipStream_t s);
 // export of F + dFc
+// PK1 stress P(F + dFc) of every particle (model 0: fixed-corotated, 1:
+// Neo-Hookean), 9 values per particle in caller order.
+void export_stress(Prec pr, int model, const void* st, const void* ctl, const void* stat, void* dst, int dst_f64,
+                   const int* perm, int n, long long ld, hipStream_t s);
 void export_F_total(Prec pr, const void* st, const void* ctl, void* dst, int dst_f64, const int* perm, int n,
                     long long ld, hipStream_t s);
 // Copy between same-layout plane buffers (`count` elements).

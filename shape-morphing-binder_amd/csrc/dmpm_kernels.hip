@@ -1191,6 +1191,21 @@ __global__ void k_export_Ftot(const T* st, const T* ctl, D* dst, const int* perm
     for (int c = 0; c < 9; ++c) dst[d + c] = (D)(st[(PF + c) * ld + i] + (ctl ? ctl[c * ld + i] : T(0)));
 }
 
+// First Piola-Kirchhoff stress of F + dFc under MODEL (the reference keeps it
+// as MaterialPoint::P after P_op_1, ForwardSimulation.cpp:16-24), exported in
+// caller order.
+template <typename T, typename D, int MODEL>
+__global__ void k_export_stress(const T* st, const T* ctl, const T* stat, D* dst, const int* perm, int n,
+                                long long ld) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    m3<T> F = load_m3(st, ld, PF, i);
+    if (ctl) F = madd(F, load_m3(ctl, ld, 0, i));
+    const m3<T> P = Model<MODEL>::pk1(F, stat[SLAM * ld + i], stat[SMU * ld + i]);
+    const long long d = (long long)(perm ? perm[i] : i) * 9;
+    for (int c = 0; c < 9; ++c) dst[d + c] = (D)P.a[c];
+}
+
 template <typename T>
 __global__ void k_copy(T* dst, const T* src, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1517,6 +1532,30 @@ void export_planes(Prec pr, const void* src, void* dst, int dst_f64, const int* 
     } else {
         if (dst_f64) hipLaunchKernelGGL((k_export<double, double>), dim3(nb), dim3(256), 0, s, (const double*)src, (double*)dst, perm, n, nc, ld);
         else hipLaunchKernelGGL((k_export<double, float>), dim3(nb), dim3(256), 0, s, (const double*)src, (float*)dst, perm, n, nc, ld);
+    }
+}
+
+template <typename T, typename D>
+static void launch_export_stress(int model, const void* st, const void* ctl, const void* stat, void* dst,
+                                 const int* perm, int n, long long ld, hipStream_t s) {
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (model == 0)
+        hipLaunchKernelGGL((k_export_stress<T, D, 0>), dim3(nb), dim3(256), 0, s, (const T*)st, (const T*)ctl,
+                           (const T*)stat, (D*)dst, perm, n, ld);
+    else
+        hipLaunchKernelGGL((k_export_stress<T, D, 1>), dim3(nb), dim3(256), 0, s, (const T*)st, (const T*)ctl,
+                           (const T*)stat, (D*)dst, perm, n, ld);
+}
+
+void export_stress(Prec pr, int model, const void* st, const void* ctl, const void* stat, void* dst, int dst_f64,
+                   const int* perm, int n, long long ld, hipStream_t s) {
+    if (n <= 0) return;
+    if (pr == PREC_F32) {
+        if (dst_f64) launch_export_stress<float, double>(model, st, ctl, stat, dst, perm, n, ld, s);
+        else launch_export_stress<float, float>(model, st, ctl, stat, dst, perm, n, ld, s);
+    } else {
+        if (dst_f64) launch_export_stress<double, double>(model, st, ctl, stat, dst, perm, n, ld, s);
+        else launch_export_stress<double, float>(model, st, ctl, stat, dst, perm, n, ld, s);
     }
 }
 

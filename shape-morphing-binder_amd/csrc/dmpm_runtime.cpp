@@ -63,6 +63,8 @@ long long round_up(long long n, long long a) { return (n + a - 1) / a * a; }
 
 }  // namespace
 
+void dmpm::set_last_error(const std::string& msg) { g_err = msg; }
+
 struct dmpm_sim {
     dmpm_config cfg{};
     Phys ph{};
@@ -617,6 +619,14 @@ void dmpm_config_init(dmpm_config* c, int nx, int ny, int nz, double dx, const d
 }
 
 const char* dmpm_last_error(void) { return g_err.c_str(); }
+
+int dmpm_get_config(const dmpm_sim* s, dmpm_config* out) {
+    return guard([&] {
+        check_sim(s);
+        if (!out) throw Error(DMPM_ERR_ARG, "null output");
+        *out = s->cfg;
+    });
+}
 const char* dmpm_version(void) { return "dmpm 0.1 (gfx950)"; }
 
 int dmpm_create(const dmpm_config* cfg, dmpm_sim** out) {
@@ -901,12 +911,22 @@ int dmpm_get_layer(dmpm_sim* s, int layer, int field, void* out, int dtype, int 
             case DMPM_FIELD_VOL: nc = 1; plane = SVOL; base = s->stat; break;
             case DMPM_FIELD_LAM: nc = 1; plane = SLAM; base = s->stat; break;
             case DMPM_FIELD_MU: nc = 1; plane = SMU; base = s->stat; break;
+            case DMPM_FIELD_P: nc = 9; break;
+            case DMPM_FIELD_ADAM_M: nc = 9; base = s->am[layer]; break;
+            case DMPM_FIELD_ADAM_V: nc = 9; base = s->av[layer]; break;
+            case DMPM_FIELD_ADAM_VMAX: nc = 9; base = s->avmax[layer]; break;
             default: throw Error(DMPM_ERR_ARG, "unknown field");
         }
         const size_t bytes = (size_t)s->n * nc * es;
         void* dst = on_device ? out : s->dalloc(bytes);
         const int f64 = dtype == DMPM_DTYPE_F64;
-        if (field == DMPM_FIELD_DFC) {
+        if (field == DMPM_FIELD_ADAM_M || field == DMPM_FIELD_ADAM_V || field == DMPM_FIELD_ADAM_VMAX) {
+            if (base) export_planes(s->pr, base, dst, f64, s->perm, s->n, 9, s->ld, s->stream);
+            else fill_zero(f64 ? PREC_F64 : PREC_F32, dst, (long long)s->n * 9, s->stream);
+        } else if (field == DMPM_FIELD_P) {
+            export_stress(s->pr, s->cfg.model, s->state(layer), s->ctl[layer], s->stat, dst, f64, s->perm, s->n,
+                          s->ld, s->stream);
+        } else if (field == DMPM_FIELD_DFC) {
             if (s->ctl[layer]) export_planes(s->pr, s->ctl[layer], dst, f64, s->perm, s->n, 9, s->ld, s->stream);
             else fill_zero(dtype == DMPM_DTYPE_F64 ? PREC_F64 : PREC_F32, dst, (long long)s->n * 9, s->stream);
         } else if (field == DMPM_FIELD_FTOTAL) {
@@ -1107,6 +1127,25 @@ int dmpm_set_control(dmpm_sim* s, int layer, const void* dFc, int dtype, int on_
         import_planes(s->pr, d, dtype == DMPM_DTYPE_F64, s->ctl[layer], s->perm, s->n, 9, s->ld, s->stream);
         s->unstage();
         s->seg_base = -1;
+    });
+}
+
+int dmpm_set_adam_state(dmpm_sim* s, int layer, const void* m, const void* v, const void* vmax, int dtype,
+                        int on_device) {
+    return guard([&] {
+        check_sim(s);
+        if (layer < 0 || layer >= s->L) throw Error(DMPM_ERR_INDEX, "Layer index out of range.");
+        if (!m || !v || !vmax) throw Error(DMPM_ERR_ARG, "null Adam state");
+        ensure_adam(s, layer);
+        const size_t bytes = (size_t)s->n * 9 * dsize(dtype);
+        const int f64 = dtype == DMPM_DTYPE_F64;
+        const void* src[3] = {m, v, vmax};
+        void* dst[3] = {s->am[layer], s->av[layer], s->avmax[layer]};
+        for (int q = 0; q < 3; ++q) {
+            const void* d = s->stage(src[q], bytes, on_device);
+            import_planes(s->pr, d, f64, dst[q], s->perm, s->n, 9, s->ld, s->stream);
+        }
+        s->unstage();
     });
 }
 

@@ -197,6 +197,27 @@ class PointCloud:
                 t = t.to(device)
         return t.requires_grad_(requires_grad)
 
+    # -- on-disk state (superset) --------------------------------------------
+    def write_entire_point_cloud_to_binary_file(self, file_path: str):
+        """The reference's cereal PointCloud file
+        (PointCloud::WriteEntirePointCloudToBinaryFile, PointCloud.cpp:382-394).
+        A CompGraph layer is written with its stress P, kept gradients and
+        Adam state; a host PointCloud with those fields zero."""
+        from dmpm.io import write_file
+        if self._arr is None:
+            self._sim._sim.write_layer(self._layer, file_path, "cereal")
+        else:
+            write_file(file_path, dict(self._arr), "cereal")
+
+    @classmethod
+    def read_entire_point_cloud_from_binary_file(cls, file_path: str) -> "PointCloud":
+        """A host PointCloud from a cereal PointCloud file
+        (PointCloud::ReadEntirePointCloudFromBinaryFile, PointCloud.cpp:396-410);
+        raises RuntimeError where the reference prints and returns false."""
+        from dmpm.io import read_file
+        a = read_file(file_path, "cereal")
+        return cls._host(a["x"], a["v"], a["F"], a["C"], a["m"], a["vol"], a["lam"], a["mu"], a["dFc"])
+
     # -- superset ------------------------------------------------------------
     def get_velocities(self) -> np.ndarray:
         return np.array(self._get("v"), np.float32)

@@ -19,7 +19,7 @@ ORDER_QUADRATIC, ORDER_CUBIC = 2, 3
 MODEL_FCR, MODEL_NEOHOOKEAN = 0, 1
 ADJOINT_EXACT, ADJOINT_REFERENCE = 0, 1
 (FIELD_X, FIELD_V, FIELD_F, FIELD_C, FIELD_DFC, FIELD_FTOTAL, FIELD_M, FIELD_VOL, FIELD_LAM,
- FIELD_MU) = range(10)
+ FIELD_MU, FIELD_P, FIELD_ADAM_M, FIELD_ADAM_V, FIELD_ADAM_VMAX) = range(14)
 
 KERNEL_P2G, KERNEL_G2P, KERNEL_ADJ_SCATTER, KERNEL_ADJ_GATHER, KERNEL_GRID_CLEAR, KERNEL_STEP = range(6)
 KERNEL_NAMES = ["p2g", "g2p", "adj_scatter", "adj_gather", "grid_clear", "step"]
@@ -50,6 +50,16 @@ class Config(C.Structure):
         ("unfused", C.c_int),
         ("deterministic", C.c_int),
     ]
+
+
+class ParticleArrays(C.Structure):
+    """dmpm_particle_arrays: host arrays of a particle file (NULL = absent)."""
+    FIELDS = ("x", "v", "F", "C", "P", "m", "vol", "lam", "mu", "dFc", "dLdx", "dLdv", "dLdF",
+              "adam_m", "adam_v", "adam_vmax")
+    _fields_ = [(f, C.c_void_p) for f in FIELDS]
+
+
+FILE_NATIVE, FILE_CEREAL = 0, 1
 
 
 class DmpmError(RuntimeError):
@@ -130,6 +140,13 @@ def lib() -> C.CDLL:
         "dmpm_graph_synchronize": (i, [vp]),
         "dmpm_graph_destroy": (i, [vp]),
         "dmpm_kernel_stats": (i, [vp, i, dp, ip]),
+        "dmpm_get_config": (i, [vp, cfgp]),
+        "dmpm_set_adam_state": (i, [vp, i, vp, vp, vp, i, i]),
+        "dmpm_file_info": (i, [C.c_char_p, i, ip, ip]),
+        "dmpm_write_file": (i, [C.c_char_p, i, i, C.POINTER(ParticleArrays), i]),
+        "dmpm_read_file": (i, [C.c_char_p, i, i, C.POINTER(ParticleArrays), i]),
+        "dmpm_write_layer": (i, [vp, i, C.c_char_p, i]),
+        "dmpm_read_particles": (i, [vp, C.c_char_p, i]),
     }
     for name, (res, args) in proto.items():
         if not hasattr(L, name):
@@ -152,6 +169,8 @@ EXPORTED = [
     "dmpm_descend_adam", "dmpm_optimize_single_timestep", "dmpm_run_optimization", "dmpm_p2g_mass",
     "dmpm_calculate_volumes", "dmpm_lame_parameters", "dmpm_last_timing", "dmpm_memory", "dmpm_kernel_timer",
     "dmpm_kernel_stats", "dmpm_window_stats", "dmpm_graph_capture", "dmpm_graph_launch", "dmpm_graph_synchronize", "dmpm_graph_destroy",
+    "dmpm_get_config", "dmpm_set_adam_state", "dmpm_file_info", "dmpm_write_file", "dmpm_read_file",
+    "dmpm_write_layer", "dmpm_read_particles",
 ]
 
 

@@ -109,10 +109,12 @@ class _Arr:
 
 
 _NCOMP = {L.FIELD_X: 3, L.FIELD_V: 3, L.FIELD_F: 9, L.FIELD_C: 9, L.FIELD_DFC: 9, L.FIELD_FTOTAL: 9,
-          L.FIELD_M: 1, L.FIELD_VOL: 1, L.FIELD_LAM: 1, L.FIELD_MU: 1}
+          L.FIELD_M: 1, L.FIELD_VOL: 1, L.FIELD_LAM: 1, L.FIELD_MU: 1, L.FIELD_P: 9, L.FIELD_ADAM_M: 9,
+          L.FIELD_ADAM_V: 9, L.FIELD_ADAM_VMAX: 9}
 _SHAPE = {3: (3,), 9: (3, 3), 1: ()}
 FIELDS = {"x": L.FIELD_X, "v": L.FIELD_V, "F": L.FIELD_F, "C": L.FIELD_C, "dFc": L.FIELD_DFC,
-          "F_total": L.FIELD_FTOTAL, "m": L.FIELD_M, "vol": L.FIELD_VOL, "lam": L.FIELD_LAM, "mu": L.FIELD_MU}
+          "F_total": L.FIELD_FTOTAL, "m": L.FIELD_M, "vol": L.FIELD_VOL, "lam": L.FIELD_LAM, "mu": L.FIELD_MU,
+          "P": L.FIELD_P, "adam_m": L.FIELD_ADAM_M, "adam_v": L.FIELD_ADAM_V, "adam_vmax": L.FIELD_ADAM_VMAX}
 
 
 class Simulator:
@@ -257,6 +259,29 @@ class Simulator:
         code = L.DTYPE_F64 if self.precision == "f64" else L.DTYPE_F32
         a = _Arr(dFc, code)
         L.check(L.lib().dmpm_set_control(self._h, int(layer), a.ptr, a.code, a.dev))
+
+    def set_adam_state(self, layer: int, m, v, vmax):
+        code = L.DTYPE_F64 if self.precision == "f64" else L.DTYPE_F32
+        a = [_Arr(t, code) for t in (m, v, vmax)]
+        L.check(L.lib().dmpm_set_adam_state(self._h, int(layer), a[0].ptr, a[1].ptr, a[2].ptr, a[0].code,
+                                            a[0].dev))
+
+    # -- on-disk state -------------------------------------------------------
+    def write_layer(self, layer: int, path: str, format: str = "native"):
+        """Write one layer (state, control, statics, stress, kept gradients,
+        Adam state) to `path`: "native" (this library's checkpoint format) or
+        "cereal" (the reference's PointCloud binary file)."""
+        from .io import FORMATS
+        L.check(L.lib().dmpm_write_layer(self._h, int(layer), str(path).encode(), FORMATS[format]))
+
+    def read_particles(self, path: str, format: str = "native"):
+        """Layer-0 particles (and Adam state) from a file written by
+        write_layer or by the reference."""
+        from .io import FORMATS
+        L.check(L.lib().dmpm_read_particles(self._h, str(path).encode(), FORMATS[format]))
+        n = C.c_int()
+        L.check(L.lib().dmpm_num_particles(self._h, C.byref(n)))
+        self.n = n.value
 
     def request_control_grad(self, layer: int, on: bool = True):
         L.check(L.lib().dmpm_request_control_grad(self._h, int(layer), 1 if on else 0))

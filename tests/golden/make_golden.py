@@ -110,7 +110,28 @@ def main():
                         mu=np.float64(mu), R=R, S=S, P=ref.pk_fcr(F, lam, mu), hvp=ref.pk_fcr_hvp(F, lam, mu, dF),
                         psi=ref.energy_fcr(F, lam, mu))
     print("wrote golden vectors to", HERE)
+    io_fixture()
+
+
+def io_fixture():
+    """8. The reference's cereal PointCloud file (PointCloud::
+    WriteEntirePointCloudToBinaryFile, PointCloud.cpp:382-394) of 5 points
+    with distinct values in every serialised field, and the inputs."""
+    ref = Ref(1)
+    rng = np.random.default_rng(808)
+    n = 5
+    shapes = {3: (n, 3), 9: (n, 3, 3), 1: (n,)}
+    width = dict(x=3, v=3, F=9, C=9, P=9, m=1, vol=1, lam=1, mu=1, dFc=9, dLdx=3, dLdv=3, dLdF=9, momentum=9,
+                 vector=9, vector_max=9)
+    a = {k: rng.normal(size=shapes[width[k]]).astype(np.float32) for k in Ref.CEREAL_FIELDS}
+    path = os.path.join(HERE, "ref_pointcloud.cereal")
+    ref.write_cereal(path, a)
+    np.savez_compressed(os.path.join(HERE, "ref_pointcloud_inputs.npz"), **a)
+    print("wrote", path)
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["io"]:
+        io_fixture()
+    else:
+        main()
