@@ -259,6 +259,17 @@ int dmpm_write_layer(dmpm_sim* sim, int layer, const char* path, int format);
  * holds a non-zero one (PointCloud.cpp:396-410). */
 int dmpm_read_particles(dmpm_sim* sim, const char* path, int format);
 
+/* --- particle initialisation -------------------------------------------------
+ * Generalised winding numbers w[np] of points P (np x 3) with respect to the
+ * triangle mesh V (nv x 3), F (nf x 3, 0-based vertex indices), computed on
+ * the device (host arrays in and out).  A lattice point is inside where
+ * w >= 1/2: the test of GeometryLoading::GeneratePointCloudFromWatertightTriangleMesh
+ * (GeometryLoading.cpp:10-83, libigl signed distance with sign type
+ * SIGNED_DISTANCE_TYPE_WINDING_NUMBER), which load_point_cloud_from_obj
+ * (bind.cpp:531-538) runs on the lattice of spacing grid_dx /
+ * points_per_cell_cuberoot over the mesh's bounding box. */
+int dmpm_winding_numbers(const double* V, int nv, const int* F, int nf, const double* P, int np, double* w);
+
 typedef struct dmpm_graph dmpm_graph;
 int dmpm_graph_capture(dmpm_sim** sims, int count, int start_layer, int stop_layer, dmpm_graph** out);
 int dmpm_graph_launch(dmpm_graph* graph);

@@ -284,6 +284,29 @@ class Oracle:
         return adj, gctl
 
 
+def winding_numbers(P, V, F, chunk: int = 2048) -> np.ndarray:
+    """Generalised winding number of points P (n,3) with respect to the
+    triangle mesh (V, F): the sum of the triangles' signed solid angles over
+    4 pi, each from tan(Omega/2) = a.(b x c) / (|a||b||c| + (a.b)|c| + (b.c)|a|
+    + (c.a)|b|) (Van Oosterom & Strackee, 1983), in float64.  The reference
+    gets the same quantity from libigl (GeometryLoading.cpp:61-77, sign type
+    SIGNED_DISTANCE_TYPE_WINDING_NUMBER; inside <=> w >= 1/2)."""
+    P = np.asarray(P, np.float64).reshape(-1, 3)
+    V = np.asarray(V, np.float64)
+    F = np.asarray(F, np.int64)
+    out = np.zeros(P.shape[0])
+    A, B, Cc = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    for s in range(0, P.shape[0], chunk):
+        p = P[s:s + chunk, None, :]
+        a, b, c = A[None] - p, B[None] - p, Cc[None] - p
+        la, lb, lc = (np.linalg.norm(t, axis=2) for t in (a, b, c))
+        num = np.einsum("ijk,ijk->ij", a, np.cross(b, c))
+        den = la * lb * lc + np.einsum("ijk,ijk->ij", a, b) * lc + np.einsum("ijk,ijk->ij", b, c) * la + \
+            np.einsum("ijk,ijk->ij", c, a) * lb
+        out[s:s + chunk] = (2 * np.arctan2(num, den)).sum(axis=1) / (4 * np.pi)
+    return out
+
+
 class Ref:
     """The reference's own code (float32), through oracle/ref_capi.cpp."""
 
@@ -461,6 +484,19 @@ class Ref:
         lam, mu = C.c_float(), C.c_float()
         self._lib.ref_lame(C.c_float(E), C.c_float(nu), C.byref(lam), C.byref(mu))
         return lam.value, mu.value
+
+    def load_obj(self, path: str, point_dx: float, density: float, lam: float, mu: float) -> dict:
+        """GeometryLoading::LoadMPMPointCloudFromObj: positions, masses and
+        volumes of the points it samples from the OBJ mesh."""
+        f = self._lib.ref_load_obj
+        f.restype = C.c_int
+        args = (str(path).encode(), C.c_float(point_dx), C.c_float(density), C.c_float(lam), C.c_float(mu))
+        n = f(*args, 0, None, None, None)
+        if n < 0:
+            raise RuntimeError(f"the reference could not load {path}")
+        o = dict(x=np.zeros((n, 3), np.float32), m=np.zeros(n, np.float32), vol=np.zeros(n, np.float32))
+        f(*args, int(n), _ptr(o["x"]), _ptr(o["m"]), _ptr(o["vol"]))
+        return o
 
     CEREAL_FIELDS = ("x", "v", "F", "C", "P", "m", "vol", "lam", "mu", "dFc", "dLdx", "dLdv", "dLdF",
                      "momentum", "vector", "vector_max")

@@ -20,6 +20,7 @@
 #include "BackPropagation.h"
 #include "CompGraph.h"
 #include "qrsvd/ImplicitQRSVD.h"
+#include "GeometryLoading.h"
 
 #include <omp.h>
 #include <cstring>
@@ -409,6 +410,23 @@ int ref_read_cereal(const char* path, int n_max, float* x, float* F, float* P, f
         store_m3(mp.P, P + 9 * p);
         m[p] = mp.m;
         store_m3(mp.vector_max, vmax + 9 * p);
+    }
+    return n;
+}
+
+// GeometryLoading::LoadMPMPointCloudFromObj (GeometryLoading.cpp:85-143): the
+// number of points it generates from the OBJ mesh, and (arrays non-null, at
+// most n_max) their positions, masses and volumes.  -1 if it fails.
+int ref_load_obj(const char* path, float point_dx, float density, float lam, float mu, int n_max, float* x,
+                 float* m, float* vol) {
+    std::shared_ptr<PointCloud> pc;
+    if (!GeometryLoading::LoadMPMPointCloudFromObj(path, pc, point_dx, density, lam, mu)) return -1;
+    const int n = (int)pc->points.size();
+    if (!x) return n;
+    for (int p = 0; p < n && p < n_max; ++p) {
+        store_v3(pc->points[p].x, x + 3 * p);
+        m[p] = pc->points[p].m;
+        vol[p] = pc->points[p].vol;
     }
     return n;
 }

@@ -53,6 +53,13 @@ struct Launch {
     int block;         // threads per block (multiple of 64)
 };
 
+// Generalised winding numbers w[np] of points P (np x 3) w.r.t. the
+// triangles tri (nf x 9: the three vertices of each), all device memory;
+// the triangles are split into `chunks` ranges summed through partial
+// (chunks x np doubles).
+void winding_numbers(const double* tri, int nf, const double* P, int np, double* partial, int chunks, double* w,
+                     hipStream_t s);
+
 // Record the message dmpm_last_error() returns (for library parts outside
 // the runtime's own error guard).
 void set_last_error(const std::string& msg);

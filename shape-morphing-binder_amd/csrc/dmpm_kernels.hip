@@ -1206,6 +1206,61 @@ __global__ void k_export_stress(const T* st, const T* ctl, const T* stat, D* dst
     for (int c = 0; c < 9; ++c) dst[d + c] = (D)P.a[c];
 }
 
+// Generalised winding number of points with respect to a triangle mesh (the
+// inside test of the mesh sampler, GeometryLoading.cpp:61-77):
+// w(p) = (1/4pi) sum_t Omega_t(p), the signed solid angle of triangle
+// (A, B, C) seen from p following A. Van Oosterom and J. Strackee, "The solid
+// angle of a plane triangle", IEEE Trans. Biomed. Eng. 30(2), 1983:
+// tan(Omega/2) = a.(b x c) / (|a||b||c| + (a.b)|c| + (b.c)|a| + (c.a)|b|),
+// a = A - p, b = B - p, c = C - p.  Block (x, y) sums the triangles
+// [y per, (y+1) per) for 256 points, staging WIND_TILE triangles at a time in
+// LDS (every lane reads the same triangle, a broadcast); partial[y np + i]
+// holds its sum and k_winding_sum adds the chunks in a fixed order.
+constexpr int WIND_TILE = 256;
+__global__ void __launch_bounds__(256) k_winding(const double* tri, int nf, int per, const double* P, int np,
+                                                 double* partial) {
+    __shared__ double st[WIND_TILE * 9];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < np;
+    double px = 0, py = 0, pz = 0;
+    if (valid) {
+        px = P[3 * (long long)i];
+        py = P[3 * (long long)i + 1];
+        pz = P[3 * (long long)i + 2];
+    }
+    const int t0 = blockIdx.y * per, t1 = min(nf, t0 + per);
+    double acc = 0;
+    for (int base = t0; base < t1; base += WIND_TILE) {
+        const int cnt = min(WIND_TILE, t1 - base);
+        __syncthreads();
+        for (int q = threadIdx.x; q < cnt * 9; q += blockDim.x) st[q] = tri[(long long)base * 9 + q];
+        __syncthreads();
+        if (!valid) continue;
+        for (int k = 0; k < cnt; ++k) {
+            const double* T = st + 9 * k;
+            const double ax = T[0] - px, ay = T[1] - py, az = T[2] - pz;
+            const double bx = T[3] - px, by = T[4] - py, bz = T[5] - pz;
+            const double cx = T[6] - px, cy = T[7] - py, cz = T[8] - pz;
+            const double la = sqrt(ax * ax + ay * ay + az * az);
+            const double lb = sqrt(bx * bx + by * by + bz * bz);
+            const double lc = sqrt(cx * cx + cy * cy + cz * cz);
+            const double det = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
+            const double den = la * lb * lc + (ax * bx + ay * by + az * bz) * lc + (bx * cx + by * cy + bz * cz) * la +
+                               (cx * ax + cy * ay + cz * az) * lb;
+            acc += 2.0 * atan2(det, den);
+        }
+    }
+    if (valid) partial[(long long)blockIdx.y * np + i] = acc;
+}
+
+__global__ void k_winding_sum(const double* partial, int chunks, int np, double* w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    double s = 0;
+    for (int c = 0; c < chunks; ++c) s += partial[(long long)c * np + i];
+    w[i] = s / (4.0 * 3.14159265358979323846);
+}
+
 template <typename T>
 __global__ void k_copy(T* dst, const T* src, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1486,6 +1541,15 @@ void adj_scatter(Prec pr, const Launch& L, hipStream_t s) {
 void adj_gather(Prec pr, const Launch& L, hipStream_t s) {
     if (pr == PREC_F32) launch_adj_gather<float>(L, s);
     else launch_adj_gather<double>(L, s);
+}
+
+void winding_numbers(const double* tri, int nf, const double* P, int np, double* partial, int chunks, double* w,
+                     hipStream_t s) {
+    if (np <= 0) return;
+    const int per = chunks > 0 ? (nf + chunks - 1) / chunks : 0;
+    const unsigned nbx = (unsigned)((np + 255) / 256);
+    hipLaunchKernelGGL(k_winding, dim3(nbx, (unsigned)chunks), dim3(256), 0, s, tri, nf, per, P, np, partial);
+    hipLaunchKernelGGL(k_winding_sum, dim3(nbx), dim3(256), 0, s, partial, chunks, np, w);
 }
 
 void fill_zero(Prec pr, void* p, long long n, hipStream_t s) {

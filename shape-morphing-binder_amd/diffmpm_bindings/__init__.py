@@ -383,60 +383,21 @@ def get_positions_from_pc(pc: PointCloud):
     return pc.get_positions()
 
 
-def _read_obj(path: str):
-    V, F = [], []
-    with open(path) as f:
-        for line in f:
-            p = line.split()
-            if not p:
-                continue
-            if p[0] == "v":
-                V.append([float(a) for a in p[1:4]])
-            elif p[0] == "f":
-                idx = [int(a.split("/")[0]) for a in p[1:]]
-                idx = [i - 1 if i > 0 else len(V) + i for i in idx]
-                for k in range(1, len(idx) - 1):
-                    F.append([idx[0], idx[k], idx[k + 1]])
-    return np.asarray(V, np.float64), np.asarray(F, np.int64)
-
-
-def _winding_number(P: np.ndarray, V: np.ndarray, F: np.ndarray, chunk: int = 4096) -> np.ndarray:
-    """Generalised winding number of points P w.r.t. the triangle mesh
-    (V, F): sum of the triangles' signed solid angles / 4 pi (Van Oosterom &
-    Strackee), the inside test of the reference's sign type
-    SIGNED_DISTANCE_TYPE_WINDING_NUMBER (GeometryLoading.cpp:61-62)."""
-    out = np.zeros(P.shape[0])
-    A, B, Cc = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
-    for s in range(0, P.shape[0], chunk):
-        p = P[s:s + chunk, None, :]
-        a, b, c = A[None] - p, B[None] - p, Cc[None] - p
-        la, lb, lc = (np.linalg.norm(t, axis=2) for t in (a, b, c))
-        num = np.einsum("ijk,ijk->ij", a, np.cross(b, c))
-        den = la * lb * lc + np.einsum("ijk,ijk->ij", a, b) * lc + np.einsum("ijk,ijk->ij", b, c) * la + \
-            np.einsum("ijk,ijk->ij", c, a) * lb
-        out[s:s + chunk] = (2 * np.arctan2(num, den)).sum(axis=1) / (4 * np.pi)
-    return out
-
-
 def load_point_cloud_from_obj(obj_path: str, opt: OptInput) -> PointCloud:
     """Particles on a lattice of spacing grid_dx / points_per_cell_cuberoot
-    over the mesh's bounding box, kept where the winding number says inside;
-    m = rho h^3, vol = h^3, F = I (GeometryLoading.cpp:10-83, 85-143,
-    bind.cpp:531-538)."""
+    over the mesh's bounding box, kept where the generalised winding number
+    (computed on the device) is >= 1/2; m = rho h^3, vol = h^3, F = I
+    (GeometryLoading.cpp:10-83, 85-143, bind.cpp:531-538)."""
+    from dmpm.mesh import read_obj, sample_inside
     if not os.path.exists(obj_path):
         raise RuntimeError("Failed to load PointCloud from: " + obj_path)
-    V, F = _read_obj(obj_path)
+    V, F = read_obj(obj_path)
     if V.size == 0 or F.size == 0:
         raise RuntimeError("Failed to load PointCloud from: " + obj_path)
-    h = float(np.float32(opt.grid_dx) / np.float32(opt.points_per_cell_cuberoot))
-    lo, hi = V.min(axis=0).astype(np.float32), V.max(axis=0).astype(np.float32)
-    dims = [int(np.ceil((hi[i] - lo[i]) / h)) + 1 for i in range(3)]
-    g = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), axis=-1).reshape(-1, 3)
-    P = lo[None].astype(np.float32) + np.float32(h) * g.astype(np.float32)
-    w = _winding_number(P.astype(np.float64), V, F)
-    x = P[w >= 0.5]
+    h = np.float32(opt.grid_dx) / np.float32(opt.points_per_cell_cuberoot)
+    x = sample_inside(V, F, float(h))
     n = x.shape[0]
-    m = np.float32(h) ** 3 * np.float32(opt.p_density)
+    m = h * h * h * np.float32(opt.p_density)
     return PointCloud._host(x, np.zeros((n, 3)), np.broadcast_to(np.eye(3), (n, 3, 3)), np.zeros((n, 3, 3)),
                             np.full(n, m), np.full(n, m / np.float32(opt.p_density)), np.full(n, opt.lam),
                             np.full(n, opt.mu))

@@ -147,6 +147,7 @@ def lib() -> C.CDLL:
         "dmpm_read_file": (i, [C.c_char_p, i, i, C.POINTER(ParticleArrays), i]),
         "dmpm_write_layer": (i, [vp, i, C.c_char_p, i]),
         "dmpm_read_particles": (i, [vp, C.c_char_p, i]),
+        "dmpm_winding_numbers": (i, [vp, i, vp, i, vp, i, vp]),
     }
     for name, (res, args) in proto.items():
         if not hasattr(L, name):
@@ -170,7 +171,7 @@ EXPORTED = [
     "dmpm_calculate_volumes", "dmpm_lame_parameters", "dmpm_last_timing", "dmpm_memory", "dmpm_kernel_timer",
     "dmpm_kernel_stats", "dmpm_window_stats", "dmpm_graph_capture", "dmpm_graph_launch", "dmpm_graph_synchronize", "dmpm_graph_destroy",
     "dmpm_get_config", "dmpm_set_adam_state", "dmpm_file_info", "dmpm_write_file", "dmpm_read_file",
-    "dmpm_write_layer", "dmpm_read_particles",
+    "dmpm_write_layer", "dmpm_read_particles", "dmpm_winding_numbers",
 ]
 
 

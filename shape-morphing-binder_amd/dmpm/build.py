@@ -20,7 +20,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libdmpm.so")
 
-SOURCES = ["dmpm_kernels.hip", "dmpm_runtime.cpp", "dmpm_io.cpp"]
+SOURCES = ["dmpm_kernels.hip", "dmpm_runtime.cpp", "dmpm_io.cpp", "dmpm_mesh.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

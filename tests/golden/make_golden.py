@@ -111,6 +111,7 @@ def main():
                         psi=ref.energy_fcr(F, lam, mu))
     print("wrote golden vectors to", HERE)
     io_fixture()
+    mesh_fixture()
 
 
 def io_fixture():
@@ -130,8 +131,49 @@ def io_fixture():
     print("wrote", path)
 
 
+def torus_obj(path: str, R=0.3, r=0.12, nu=24, nv=12, noise=0.01, seed=909):
+    """A closed, irregular triangle mesh (a torus with seeded vertex noise),
+    outward-oriented, centred at (0.5, 0.5, 0.5), written as OBJ text."""
+    rng = np.random.default_rng(seed)
+    V = []
+    for i in range(nu):
+        for j in range(nv):
+            u, v = 2 * np.pi * i / nu, 2 * np.pi * j / nv
+            p = np.array([(R + r * np.cos(v)) * np.cos(u), (R + r * np.cos(v)) * np.sin(u), r * np.sin(v)])
+            V.append(p + noise * rng.normal(size=3) + 0.5)
+    F = []
+    for i in range(nu):
+        for j in range(nv):
+            a, b = i * nv + j, ((i + 1) % nu) * nv + j
+            c, d = ((i + 1) % nu) * nv + (j + 1) % nv, i * nv + (j + 1) % nv
+            F += [(a, b, c), (a, c, d)]
+    with open(path, "w") as f:
+        for p in V:
+            f.write("v %.6f %.6f %.6f\n" % tuple(p))
+        for t in F:
+            f.write("f %d %d %d\n" % (t[0] + 1, t[1] + 1, t[2] + 1))
+
+
+def mesh_fixture():
+    """9. GeometryLoading::LoadMPMPointCloudFromObj (GeometryLoading.cpp:85-143)
+    on an irregular torus: the points, masses and volumes the reference
+    samples at point_dx = 0.025 (grid_dx 0.05, 2 points per cell)."""
+    ref = Ref(1)
+    obj = os.path.join(HERE, "mesh_torus.obj")
+    torus_obj(obj)
+    grid_dx, ppc, rho, E, nu = 0.05, 2, 1000.0, 1e4, 0.3
+    lam, mu = ref.lame(E, nu)
+    point_dx = float(np.float32(grid_dx) / np.float32(ppc))
+    o = ref.load_obj(obj, point_dx, rho, lam, mu)
+    np.savez_compressed(os.path.join(HERE, "ref_mesh_torus.npz"), grid_dx=np.float32(grid_dx), ppc=np.int32(ppc),
+                        rho=np.float32(rho), lam=np.float32(lam), mu=np.float32(mu), **o)
+    print("wrote", obj, "and", o["x"].shape[0], "reference points")
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["io"]:
         io_fixture()
+    elif sys.argv[1:] == ["mesh"]:
+        mesh_fixture()
     else:
         main()
