@@ -19,7 +19,9 @@
 // or on a wall node) is recomputed from (m, p) wherever it is read, which is
 // the reference's grid update (ForwardSimulation.cpp:82-87, 165-170) fused
 // into its readers.
+#include <algorithm>
 #include <climits>
+#include <cstdint>
 
 #include <hipcub/hipcub.hpp>
 
@@ -1197,6 +1199,15 @@ __global__ void k_fill_zero(T* p, long long n) {
     for (; i < n; i += stride) p[i] = T(0);
 }
 
+// Zero n16 16-byte words (grid planes: 16-byte aligned, sizes a multiple of
+// 64 elements) with one dwordx4 store per word.
+__global__ void k_fill_zero16(uint4* p, long long n16) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for (; i < n16; i += stride) p[i] = z;
+}
+
 template <typename T, typename S>
 __global__ void k_import(const S* src, T* dst, const int* perm, int n, int nc, long long ld) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1592,6 +1603,13 @@ void winding_numbers(const double* tri, int nf, const double* P, int np, double*
 
 void fill_zero(Prec pr, void* p, long long n, hipStream_t s) {
     if (n <= 0) return;
+    const long long bytes = n * (pr == PREC_F32 ? 4 : 8);
+    if (((uintptr_t)p & 15) == 0 && (bytes & 15) == 0) {
+        const long long n16 = bytes / 16;
+        const long long nb = std::min<long long>((n16 + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_fill_zero16, dim3((unsigned)nb), dim3(256), 0, s, (uint4*)p, n16);
+        return;
+    }
     if (pr == PREC_F32) hipLaunchKernelGGL(k_fill_zero<float>, dim3(ew_blocks(n)), dim3(256), 0, s, (float*)p, n);
     else hipLaunchKernelGGL(k_fill_zero<double>, dim3(ew_blocks(n)), dim3(256), 0, s, (double*)p, n);
 }

@@ -112,6 +112,7 @@ def main():
     print("wrote golden vectors to", HERE)
     io_fixture()
     mesh_fixture()
+    assets_fixture()
 
 
 def io_fixture():
@@ -170,8 +171,42 @@ def mesh_fixture():
     print("wrote", obj, "and", o["x"].shape[0], "reference points")
 
 
+REF_ASSETS = "/root/reference/assets"
+
+
+def assets_fixture():
+    """10. The reference's own binding test (tests/test.py:157-245) on the
+    meshes it ships: isosphere.obj -> input, bunny.obj -> target; grid_dx 1,
+    grid 33^3 from -16, 2 points per cell axis, density 75, lam 38888.89,
+    mu 58333.3.  Copies the two meshes here (data files of the reference's
+    test) and records the points, volumes, target grid masses and the
+    end-layer mass loss (with its dL/dx) that the reference computes."""
+    import shutil
+    ref = Ref(1)
+    for name in ("isosphere.obj", "bunny.obj"):
+        shutil.copyfile(os.path.join(REF_ASSETS, name), os.path.join(HERE, "asset_" + name))
+    point_dx = float(np.float32(1.0) / np.float32(2))
+    lam, mu, rho = 38888.89, 58333.3, 75.0
+    inp = ref.load_obj(os.path.join(HERE, "asset_isosphere.obj"), point_dx, rho, lam, mu)
+    tgt = ref.load_obj(os.path.join(HERE, "asset_bunny.obj"), point_dx, rho, lam, mu)
+    prm = reference_params(dims=(33, 33, 33), dx=1.0, grid_min=(-16.0, -16.0, -16.0))
+    vin, _ = ref.volumes(prm, inp["x"], inp["m"])
+    vt, gt = ref.volumes(prm, tgt["x"], tgt["m"])
+    n = inp["x"].shape[0]
+    s = dict(x=inp["x"], v=np.zeros((n, 3), np.float32), F=np.broadcast_to(np.eye(3, dtype=np.float32), (n, 3, 3)),
+             C=np.zeros((n, 3, 3), np.float32), m=inp["m"], vol=vin, lam=np.full(n, lam, np.float32),
+             mu=np.full(n, mu, np.float32))
+    loss, gx, _, _ = ref.mass_loss(prm, s, gt)
+    np.savez_compressed(os.path.join(HERE, "ref_assets.npz"), in_x=inp["x"], in_m=inp["m"], in_vol=vin,
+                        tgt_x=tgt["x"], tgt_m=tgt["m"], tgt_vol=vt, target_grid_m=gt, loss=np.float64(loss),
+                        dLdx=gx, lam=np.float32(lam), mu=np.float32(mu), rho=np.float32(rho))
+    print("assets:", n, "input points,", tgt["x"].shape[0], "target points, loss", loss)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["io"]:
+    if sys.argv[1:] == ["assets"]:
+        assets_fixture()
+    elif sys.argv[1:] == ["io"]:
         io_fixture()
     elif sys.argv[1:] == ["mesh"]:
         mesh_fixture()
