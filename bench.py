@@ -157,9 +157,10 @@ def batched_workload(dt: float = 2.5e-4, T: int = 100, K: int = 10, reps: int = 
     """64 independent instances of uneven size (particle counts log-uniform
     in [4,096, 131,072], grids of 32/48/64/96 cells, random ellipsoids and
     materials), each a T-step forward + backward episode seeded at its last
-    layer: one at a time (eager calls per instance) against one hipGraph
-    holding every instance's episode as a concurrent branch."""
-    from dmpm import EpisodeGraph, Physics, Simulator
+    layer: one at a time (eager calls per instance), one hipGraph holding
+    every instance's episode as a concurrent branch, and the batched episode
+    (each kernel one launch over all instances, the episode one hipGraph)."""
+    from dmpm import BatchedEpisode, EpisodeGraph, Physics, Simulator
     from dmpm.scenes import config4_instances, instance_scene
     sims, total = [], 0
     for spec in config4_instances():
@@ -196,15 +197,30 @@ def batched_workload(dt: float = 2.5e-4, T: int = 100, K: int = 10, reps: int = 
     g.synchronize()
     w_graph = (time.perf_counter() - t0) / reps
     g.close()
+    log(f"[bench] batched: graph of branches {1e3 * w_graph:.1f} ms")
+    t0 = time.perf_counter()
+    b = BatchedEpisode(sims, start=0, stop=0)
+    log(f"[bench] batched: one-launch episode recorded and captured ({time.perf_counter() - t0:.2f}s)")
+    b.launch()
+    b.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.launch()
+    b.synchronize()
+    w_batch = (time.perf_counter() - t0) / reps
+    b.close()
     peak = sum(s.memory()[1] for s in sims)
     for s in sims:
         s.close()
     out = {"instances": len(sims), "particles_total": total, "episode_steps": T, "checkpoint_interval": K,
            "dt": dt, "one_at_a_time": {"value": total * T / w_seq, "ms_per_step": 1e3 * w_seq},
            "graph": {"value": total * T / w_graph, "ms_per_step": 1e3 * w_graph},
-           "speedup": w_seq / w_graph, "peak_device_bytes": peak, "unit": "particle-steps/s"}
+           "one_launch": {"value": total * T / w_batch, "ms_per_step": 1e3 * w_batch},
+           "speedup_graph": w_seq / w_graph, "speedup_one_launch": w_seq / w_batch,
+           "peak_device_bytes": peak, "unit": "particle-steps/s"}
     log(f"[bench] batched: {len(sims)} instances, {total} particles: one at a time {1e3 * w_seq:.1f} ms, "
-        f"graph {1e3 * w_graph:.1f} ms per episode set ({out['speedup']:.2f}x)")
+        f"graph of branches {1e3 * w_graph:.1f} ms ({out['speedup_graph']:.2f}x), one launch per kernel "
+        f"{1e3 * w_batch:.1f} ms ({out['speedup_one_launch']:.2f}x)")
     return out
 
 

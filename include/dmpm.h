@@ -270,6 +270,25 @@ int dmpm_read_particles(dmpm_sim* sim, const char* path, int format);
  * points_per_cell_cuberoot over the mesh's bounding box. */
 int dmpm_winding_numbers(const double* V, int nv, const int* F, int nf, const double* P, int np, double* w);
 
+/* --- batched episodes ----------------------------------------------------------
+ * Many independent sims (different particle counts, grids and materials; the
+ * same precision, B-spline order, model, adjoint mode, launch shape and
+ * episode structure) stepped together: every kernel of the episode runs as
+ * one launch over all instances (each thread block belongs to one instance),
+ * and the whole episode -- forward from start_layer keeping per-layer
+ * controls, backward to stop_layer -- is one hipGraph.  No counterpart in the
+ * reference, which runs one CompGraph at a time (CompGraph.cpp:131-211).
+ * Capture records each sim's episode without running it (host bookkeeping
+ * such as which layers are valid is updated as for a run); launch replays it
+ * on the batch's own stream.  Synchronise the sims before a launch and the
+ * batch before reading the sims.  Inputs may change between launches through
+ * the usual calls; particle counts and layer counts may not. */
+typedef struct dmpm_batch dmpm_batch;
+int dmpm_batch_capture(dmpm_sim** sims, int count, int start_layer, int stop_layer, dmpm_batch** out);
+int dmpm_batch_launch(dmpm_batch* batch);
+int dmpm_batch_synchronize(dmpm_batch* batch);
+int dmpm_batch_destroy(dmpm_batch* batch);
+
 typedef struct dmpm_graph dmpm_graph;
 int dmpm_graph_capture(dmpm_sim** sims, int count, int start_layer, int stop_layer, dmpm_graph** out);
 int dmpm_graph_launch(dmpm_graph* graph);

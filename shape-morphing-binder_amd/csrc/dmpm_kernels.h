@@ -60,6 +60,44 @@ struct Launch {
 void winding_numbers(const double* tri, int nf, const double* P, int np, double* partial, int chunks, double* w,
                      hipStream_t s);
 
+// ---- batched launches: one kernel launch over many instances ---------------
+// Operations of a recorded episode.
+enum OpKind {
+    OP_ZERO = 0,     // dst[0..n) = 0 (n elements)
+    OP_COPY,         // dst[0..n) = src[0..n)
+    OP_ACCUM,        // dst[0..n) += src[0..n)
+    OP_P2G,          // Launch: P2G of m and p
+    OP_P2G_MASS,     // Launch: P2G of m only
+    OP_G2P,
+    OP_STEP,
+    OP_ADJ_SCATTER,
+    OP_ADJ_GATHER,
+};
+inline bool op_is_kernel(int kind) { return kind >= OP_P2G; }
+// Size of one kernel parameter block for precision pr, and the block of
+// kernel op `kind` for launch L, written to host memory dst.
+size_t kp_bytes(Prec pr);
+void build_kp(Prec pr, int kind, const Launch& L, void* dst);
+// Thread blocks one instance of n particles takes.
+int instance_blocks(int n, int block);
+// One launch of kernel op `kind` for `count` instances: tab (device) holds
+// their parameter blocks, bstart (device, count + 1 entries) the prefix sums
+// of their block counts, total = bstart[count].  The instances share the
+// precision, physics templates (order, model, adjoint mode), block size and
+// LDS budget.
+void launch_batch(Prec pr, int kind, const Phys& ph, int lds_bytes, int block, const void* tab, const int* bstart,
+                  int count, int total, hipStream_t s);
+// Element ranges of an OP_ZERO / OP_COPY / OP_ACCUM over many buffers (one
+// launch): segs (device) holds `count` ranges; n counts elements of pr and
+// max_n is the largest n.  Zero and copy ranges must be 16-byte aligned and a
+// multiple of 16 bytes long.
+struct Seg {
+    void* dst;
+    const void* src;
+    long long n;
+};
+void launch_segs(Prec pr, int kind, const Seg* segs, int count, long long max_n, hipStream_t s);
+
 // Record the message dmpm_last_error() returns (for library parts outside
 // the runtime's own error guard).
 void set_last_error(const std::string& msg);

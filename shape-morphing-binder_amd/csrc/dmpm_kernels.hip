@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <cstring>
 
 #include <hipcub/hipcub.hpp>
 
@@ -105,6 +106,24 @@ DMPM_DI int logical_block() {
     const int nb = gridDim.x;
     const int per = nb >> 3;
     return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+// Blocks of a batched launch: instance `inst` owns the logical blocks
+// [bstart[inst], bstart[inst + 1]) (logical order as in logical_block(); the
+// grid is padded to a multiple of 8 and blocks past bstart[count] have no
+// work).  Block-uniform, so the search runs on the scalar unit.
+DMPM_DI bool batch_block(const int* bstart, int count, int& inst, int& lblock) {
+    const int lb = logical_block();
+    if (lb >= bstart[count]) return false;
+    int lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (bstart[mid] <= lb) lo = mid;
+        else hi = mid;
+    }
+    inst = lo;
+    lblock = lb - bstart[lo];
+    return true;
 }
 
 static inline unsigned grid_blocks(int n, int block) {
@@ -488,12 +507,11 @@ DMPM_DI void p2g_flush(const KP<T>& a, const Win& win, const Acc* sm, T* gm, T* 
 
 // P2G of layer `st` into the grid.
 template <typename T, int ORDER, int MODEL, bool MASS_ONLY>
-__global__ void __launch_bounds__(256) k_p2g(KP<T> a) {
+DMPM_DI void p2g_block(const KP<T>& a, int lblock, double* smem_d) {
     constexpr int KS = Spline<ORDER>::K;
     constexpr int NA = MASS_ONLY ? 1 : 4;
-    extern __shared__ double smem_d[];
     Acc* sm = reinterpret_cast<Acc*>(smem_d);
-    const int i = logical_block() * blockDim.x + threadIdx.x;
+    const int i = lblock * blockDim.x + threadIdx.x;
     const bool valid = i < a.n;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
@@ -519,6 +537,21 @@ __global__ void __launch_bounds__(256) k_p2g(KP<T> a) {
         __syncthreads();
         p2g_flush<T, MASS_ONLY>(a, win, sm, a.gm, a.gp);
     }
+}
+
+template <typename T, int ORDER, int MODEL, bool MASS_ONLY>
+__global__ void __launch_bounds__(256) k_p2g(KP<T> a) {
+    extern __shared__ double smem_d[];
+    p2g_block<T, ORDER, MODEL, MASS_ONLY>(a, logical_block(), smem_d);
+}
+
+template <typename T, int ORDER, int MODEL, bool MASS_ONLY>
+__global__ void __launch_bounds__(256) k_p2g_batch(const KP<T>* tab, const int* bstart, int count) {
+    extern __shared__ double smem_d[];
+    int inst, lblock;
+    if (!batch_block(bstart, count, inst, lblock)) return;
+    const KP<T> a = tab[inst];
+    p2g_block<T, ORDER, MODEL, MASS_ONLY>(a, lblock, smem_d);
 }
 
 // Stage the window's grid velocities in LDS (3 arrays of cap).
@@ -638,11 +671,10 @@ DMPM_DI void particle_update(const KP<T>& a, int i, const v3<T>& x, const v3<T>&
 
 // G2P + P_op_2 (ForwardSimulation.cpp:89-157): writes layer t+1.
 template <typename T, int ORDER>
-__global__ void __launch_bounds__(256) k_g2p(KP<T> a) {
+DMPM_DI void g2p_block(const KP<T>& a, int lblock, double* smem_d) {
     constexpr int KS = Spline<ORDER>::K;
-    extern __shared__ double smem_d[];
     T* sv = reinterpret_cast<T*>(smem_d);
-    const int i = logical_block() * blockDim.x + threadIdx.x;
+    const int i = lblock * blockDim.x + threadIdx.x;
     const bool valid = i < a.n;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
@@ -660,17 +692,31 @@ __global__ void __launch_bounds__(256) k_g2p(KP<T> a) {
     particle_update(a, i, x, vn, Craw, xn, Fn, Cn);
 }
 
+template <typename T, int ORDER>
+__global__ void __launch_bounds__(256) k_g2p(KP<T> a) {
+    extern __shared__ double smem_d[];
+    g2p_block<T, ORDER>(a, logical_block(), smem_d);
+}
+
+template <typename T, int ORDER>
+__global__ void __launch_bounds__(256) k_g2p_batch(const KP<T>* tab, const int* bstart, int count) {
+    extern __shared__ double smem_d[];
+    int inst, lblock;
+    if (!batch_block(bstart, count, inst, lblock)) return;
+    const KP<T> a = tab[inst];
+    g2p_block<T, ORDER>(a, lblock, smem_d);
+}
+
 // One fused forward step: G2P of step t from grid t (a.gm/a.gp), the
 // particle update writing layer t+1, then the stress of layer t+1 (with its
 // control a.ctl1) and its P2G into grid t+1 (a.gm1/a.gp1, zeroed by the
 // caller).  Layer t+1 is never re-read from HBM for its own P2G.
 template <typename T, int ORDER, int MODEL>
-__global__ void __launch_bounds__(256) k_step(KP<T> a) {
+DMPM_DI void step_block(const KP<T>& a, int lblock, double* smem_d) {
     constexpr int KS = Spline<ORDER>::K;
-    extern __shared__ double smem_d[];
     T* sm = reinterpret_cast<T*>(smem_d);     // staged velocities
     Acc* sacc = reinterpret_cast<Acc*>(smem_d);  // then the P2G accumulators
-    const int i = logical_block() * blockDim.x + threadIdx.x;
+    const int i = lblock * blockDim.x + threadIdx.x;
     const bool valid = i < a.n;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
@@ -708,6 +754,21 @@ __global__ void __launch_bounds__(256) k_step(KP<T> a) {
         __syncthreads();
         p2g_flush<T, false>(a, win1, sacc, a.gm1, a.gp1);
     }
+}
+
+template <typename T, int ORDER, int MODEL>
+__global__ void __launch_bounds__(256) k_step(KP<T> a) {
+    extern __shared__ double smem_d[];
+    step_block<T, ORDER, MODEL>(a, logical_block(), smem_d);
+}
+
+template <typename T, int ORDER, int MODEL>
+__global__ void __launch_bounds__(256) k_step_batch(const KP<T>* tab, const int* bstart, int count) {
+    extern __shared__ double smem_d[];
+    int inst, lblock;
+    if (!batch_block(bstart, count, inst, lblock)) return;
+    const KP<T> a = tab[inst];
+    step_block<T, ORDER, MODEL>(a, lblock, smem_d);
 }
 
 // Phase 1 of the adjoint (BackPropagation.cpp:22-32).  REF reproduces the
@@ -777,12 +838,11 @@ DMPM_DI Phase1<T, REF> phase1(const KP<T>& a, int i, const m3<T>& Ft, const m3<T
 // Adjoint phases 1 + 2: dL/dv_i += w gvtot + D^{-1} w gCraw (x_i - x_p)
 // (BackPropagation.cpp:36-55), accumulated in an LDS window.
 template <typename T, int ORDER, bool REF>
-__global__ void __launch_bounds__(256) k_adj_scatter(KP<T> a) {
+DMPM_DI void adj_scatter_block(const KP<T>& a, int lblock, double* smem_d) {
     constexpr int KS = Spline<ORDER>::K;
-    extern __shared__ double smem_d[];
     T* sv = reinterpret_cast<T*>(smem_d);  // 3 velocity arrays, then 3 accumulators
     Acc* sg = reinterpret_cast<Acc*>(smem_d + (3 * a.cap * sizeof(T) + 7) / 8);
-    const int i = logical_block() * blockDim.x + threadIdx.x;
+    const int i = lblock * blockDim.x + threadIdx.x;
     const bool valid = i < a.n;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
@@ -839,6 +899,21 @@ __global__ void __launch_bounds__(256) k_adj_scatter(KP<T> a) {
     }
 }
 
+template <typename T, int ORDER, bool REF>
+__global__ void __launch_bounds__(256) k_adj_scatter(KP<T> a) {
+    extern __shared__ double smem_d[];
+    adj_scatter_block<T, ORDER, REF>(a, logical_block(), smem_d);
+}
+
+template <typename T, int ORDER, bool REF>
+__global__ void __launch_bounds__(256) k_adj_scatter_batch(const KP<T>* tab, const int* bstart, int count) {
+    extern __shared__ double smem_d[];
+    int inst, lblock;
+    if (!batch_block(bstart, count, inst, lblock)) return;
+    const KP<T> a = tab[inst];
+    adj_scatter_block<T, ORDER, REF>(a, lblock, smem_d);
+}
+
 // Grid adjoint (BackPropagation.cpp:59-72) for one node: h = dL/dp_i,
 // gm = dL/dm_i.  Exact: dm uses v_i - dt f_ext; REF uses v_i (as written).
 template <typename T, bool REF>
@@ -866,11 +941,10 @@ DMPM_DI void node_adjoint(const KP<T>& a, long long g, int i, int j, int k, T* v
 // Adjoint phases 3 + 4 and the stress VJP (BackPropagation.cpp:59-143).
 // The window holds per node v_i (3), h_i = dL/dp_i (3), dL/dm_i (1).
 template <typename T, int ORDER, int MODEL, bool REF>
-__global__ void __launch_bounds__(256) k_adj_gather(KP<T> a) {
+DMPM_DI void adj_gather_block(const KP<T>& a, int lblock, double* smem_d) {
     constexpr int KS = Spline<ORDER>::K;
-    extern __shared__ double smem_d[];
     T* sv = reinterpret_cast<T*>(smem_d);  // v (3 arrays), h (3), gm (1)
-    const int i = logical_block() * blockDim.x + threadIdx.x;
+    const int i = lblock * blockDim.x + threadIdx.x;
     const bool valid = i < a.n;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
@@ -1022,6 +1096,21 @@ __global__ void __launch_bounds__(256) k_adj_gather(KP<T> a) {
     store_m3(a.adj0, a.ld, PF, i, gF);
     store_m3(a.adj0, a.ld, PC, i, gC);
     if (a.gctl) store_m3(a.gctl, a.ld, 0, i, REF ? gF : gFt);
+}
+
+template <typename T, int ORDER, int MODEL, bool REF>
+__global__ void __launch_bounds__(256) k_adj_gather(KP<T> a) {
+    extern __shared__ double smem_d[];
+    adj_gather_block<T, ORDER, MODEL, REF>(a, logical_block(), smem_d);
+}
+
+template <typename T, int ORDER, int MODEL, bool REF>
+__global__ void __launch_bounds__(256) k_adj_gather_batch(const KP<T>* tab, const int* bstart, int count) {
+    extern __shared__ double smem_d[];
+    int inst, lblock;
+    if (!batch_block(bstart, count, inst, lblock)) return;
+    const KP<T> a = tab[inst];
+    adj_gather_block<T, ORDER, MODEL, REF>(a, lblock, smem_d);
 }
 
 // ---------------------------------------------------------------------------
@@ -1562,6 +1651,129 @@ static void launch_adj_gather(const Launch& L, hipStream_t s) {
         else { if (ref) AG_CASE(2, 1, true); else AG_CASE(2, 1, false); }
     }
 #undef AG_CASE
+}
+
+// ---- batched launches --------------------------------------------------------
+// Bytes of LDS per window node of each kernel op (the window capacity is the
+// LDS budget divided by this), and the dynamic LDS a block of it takes.
+template <typename T>
+static int node_bytes(int kind) {
+    switch (kind) {
+        case OP_P2G: return 4 * (int)sizeof(Acc);
+        case OP_P2G_MASS: return (int)sizeof(Acc);
+        case OP_G2P: return 3 * (int)sizeof(T);
+        case OP_STEP: return 4 * (int)sizeof(Acc);
+        case OP_ADJ_SCATTER: return 3 * (int)sizeof(T) + 3 * (int)sizeof(Acc) + 8;
+        case OP_ADJ_GATHER: return 7 * (int)sizeof(T);
+        default: return 0;
+    }
+}
+template <typename T>
+static size_t block_lds(int kind, int cap) {
+    if (kind == OP_ADJ_SCATTER) return (size_t)cap * (3 * sizeof(T) + 3 * sizeof(Acc)) + 8;
+    return (size_t)cap * node_bytes<T>(kind);
+}
+
+size_t kp_bytes(Prec pr) { return pr == PREC_F32 ? sizeof(KP<float>) : sizeof(KP<double>); }
+
+void build_kp(Prec pr, int kind, const Launch& L, void* dst) {
+    if (pr == PREC_F32) {
+        const KP<float> k = make_kp<float>(L, node_bytes<float>(kind));
+        std::memcpy(dst, &k, sizeof(k));
+    } else {
+        const KP<double> k = make_kp<double>(L, node_bytes<double>(kind));
+        std::memcpy(dst, &k, sizeof(k));
+    }
+}
+
+int instance_blocks(int n, int block) { return n > 0 ? (n + block - 1) / block : 0; }
+
+template <typename T>
+static void launch_batch_t(int kind, const Phys& ph, int lds_bytes, int block, const void* tab_, const int* bstart,
+                           int count, int total, hipStream_t s) {
+    const KP<T>* tab = (const KP<T>*)tab_;
+    const unsigned nb = (unsigned)std::max(8, (total + 7) & ~7);
+    const int cap = lds_bytes / node_bytes<T>(kind);
+    const size_t sh = block_lds<T>(kind, cap);
+    const int o = ph.order, md = ph.model;
+    const bool ref = ph.adjoint == 1;
+#define BATCH(KERNEL) hipLaunchKernelGGL(KERNEL, dim3(nb), dim3(block), sh, s, tab, bstart, count)
+    switch (kind) {
+        case OP_P2G:
+            if (o == 3) { if (md == 0) BATCH((k_p2g_batch<T, 3, 0, false>)); else BATCH((k_p2g_batch<T, 3, 1, false>)); }
+            else { if (md == 0) BATCH((k_p2g_batch<T, 2, 0, false>)); else BATCH((k_p2g_batch<T, 2, 1, false>)); }
+            break;
+        case OP_P2G_MASS:
+            if (o == 3) BATCH((k_p2g_batch<T, 3, 0, true>)); else BATCH((k_p2g_batch<T, 2, 0, true>));
+            break;
+        case OP_G2P:
+            if (o == 3) BATCH((k_g2p_batch<T, 3>)); else BATCH((k_g2p_batch<T, 2>));
+            break;
+        case OP_STEP:
+            if (o == 3) { if (md == 0) BATCH((k_step_batch<T, 3, 0>)); else BATCH((k_step_batch<T, 3, 1>)); }
+            else { if (md == 0) BATCH((k_step_batch<T, 2, 0>)); else BATCH((k_step_batch<T, 2, 1>)); }
+            break;
+        case OP_ADJ_SCATTER:
+            if (o == 3) { if (ref) BATCH((k_adj_scatter_batch<T, 3, true>)); else BATCH((k_adj_scatter_batch<T, 3, false>)); }
+            else { if (ref) BATCH((k_adj_scatter_batch<T, 2, true>)); else BATCH((k_adj_scatter_batch<T, 2, false>)); }
+            break;
+        case OP_ADJ_GATHER:
+            if (o == 3) {
+                if (md == 0) { if (ref) BATCH((k_adj_gather_batch<T, 3, 0, true>)); else BATCH((k_adj_gather_batch<T, 3, 0, false>)); }
+                else { if (ref) BATCH((k_adj_gather_batch<T, 3, 1, true>)); else BATCH((k_adj_gather_batch<T, 3, 1, false>)); }
+            } else {
+                if (md == 0) { if (ref) BATCH((k_adj_gather_batch<T, 2, 0, true>)); else BATCH((k_adj_gather_batch<T, 2, 0, false>)); }
+                else { if (ref) BATCH((k_adj_gather_batch<T, 2, 1, true>)); else BATCH((k_adj_gather_batch<T, 2, 1, false>)); }
+            }
+            break;
+        default: break;
+    }
+#undef BATCH
+}
+
+void launch_batch(Prec pr, int kind, const Phys& ph, int lds_bytes, int block, const void* tab, const int* bstart,
+                  int count, int total, hipStream_t s) {
+    if (count <= 0 || total <= 0) return;
+    if (pr == PREC_F32) launch_batch_t<float>(kind, ph, lds_bytes, block, tab, bstart, count, total, s);
+    else launch_batch_t<double>(kind, ph, lds_bytes, block, tab, bstart, count, total, s);
+}
+
+// Segment kernels (blockIdx.y = range); zero and copy move 16-byte words.
+__global__ void k_zero_segs(const Seg* segs, int esz) {
+    const Seg g = segs[blockIdx.y];
+    const long long n16 = g.n * esz / 16;
+    uint4* d = (uint4*)g.dst;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long long)gridDim.x * blockDim.x)
+        d[i] = z;
+}
+__global__ void k_copy_segs(const Seg* segs, int esz) {
+    const Seg g = segs[blockIdx.y];
+    const long long n16 = g.n * esz / 16;
+    uint4* d = (uint4*)g.dst;
+    const uint4* src = (const uint4*)g.src;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long long)gridDim.x * blockDim.x)
+        d[i] = src[i];
+}
+template <typename T>
+__global__ void k_accum_segs(const Seg* segs) {
+    const Seg g = segs[blockIdx.y];
+    T* d = (T*)g.dst;
+    const T* src = (const T*)g.src;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += (long long)gridDim.x * blockDim.x)
+        d[i] += src[i];
+}
+
+void launch_segs(Prec pr, int kind, const Seg* segs, int count, long long max_n, hipStream_t s) {
+    if (count <= 0 || max_n <= 0) return;
+    const int esz = pr == PREC_F32 ? 4 : 8;
+    const long long words = kind == OP_ACCUM ? max_n : max_n * esz / 16;
+    const unsigned gx = (unsigned)std::max(1LL, std::min((words + 1023) / 1024, 256LL));
+    const dim3 grid(gx, (unsigned)count);
+    if (kind == OP_ZERO) hipLaunchKernelGGL(k_zero_segs, grid, dim3(256), 0, s, segs, esz);
+    else if (kind == OP_COPY) hipLaunchKernelGGL(k_copy_segs, grid, dim3(256), 0, s, segs, esz);
+    else if (pr == PREC_F32) hipLaunchKernelGGL(k_accum_segs<float>, grid, dim3(256), 0, s, segs);
+    else hipLaunchKernelGGL(k_accum_segs<double>, grid, dim3(256), 0, s, segs);
 }
 
 static inline unsigned ew_blocks(long long n, int bs = 256) {

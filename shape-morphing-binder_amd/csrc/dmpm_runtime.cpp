@@ -206,6 +206,56 @@ struct dmpm_sim {
         krec.emplace_back(kind, i);
     }
 
+    // ---- GPU operations of the forward and backward passes ----
+    // Each either runs now or, while an episode is being recorded for a
+    // batched launch (rec != null), is appended to rec instead.
+    struct Op {
+        int kind;
+        Launch L;
+        void* dst;
+        const void* src;
+        long long n;
+    };
+    std::vector<Op>* rec = nullptr;
+
+    void op_zero(void* p, long long elems) {
+        if (rec) {
+            rec->push_back(Op{OP_ZERO, Launch{}, p, nullptr, elems});
+            return;
+        }
+        timed(4, [&] { zero(p, elems); });
+    }
+    void op_copy(void* dst, const void* src, long long elems) {
+        if (rec) {
+            rec->push_back(Op{OP_COPY, Launch{}, dst, src, elems});
+            return;
+        }
+        copy_elems(pr, dst, src, elems, stream);
+    }
+    // dst += src over `elems` elements of the sim's precision
+    void op_accum(void* dst, const void* src, long long elems) {
+        if (rec) {
+            rec->push_back(Op{OP_ACCUM, Launch{}, dst, src, elems});
+            return;
+        }
+        accumulate_carry(pr, src, pr == PREC_F64, dst, 0, 1, nullptr, (int)elems, elems, stream);
+    }
+    void op_kernel(int kind, const Launch& Lc) {
+        if (rec) {
+            rec->push_back(Op{kind, Lc, nullptr, nullptr, 0});
+            return;
+        }
+        switch (kind) {
+            case OP_P2G: timed(0, [&] { p2g(pr, Lc, false, stream); }); break;
+            case OP_P2G_MASS: timed(0, [&] { p2g(pr, Lc, true, stream); }); break;
+            case OP_G2P: timed(1, [&] { g2p(pr, Lc, stream); }); break;
+            case OP_STEP: timed(5, [&] { dmpm::step(pr, Lc, stream); }); break;
+            case OP_ADJ_SCATTER: timed(2, [&] { adj_scatter(pr, Lc, stream); }); break;
+            case OP_ADJ_GATHER: timed(3, [&] { adj_gather(pr, Lc, stream); }); break;
+            default: throw Error(DMPM_ERR_ARG, "unknown kernel op");
+        }
+    }
+
     bool stored(int t) const { return K <= 1 || t % K == 0 || t == L - 1; }
 
     // (Re)allocate layer storage for L layers.
@@ -267,11 +317,12 @@ struct dmpm_sim {
         Lc.ctl = ctl[t];
         Lc.grid = g;
         if (cfg.deterministic) {
+            if (rec) throw Error(DMPM_ERR_STATE, "deterministic mode cannot be batched");
             timed(0, [&] { det_p2g(pr, Lc, false, det_buf(), stream); });
             return;
         }
-        timed(4, [&] { zero(g, 4 * gld); });
-        timed(0, [&] { p2g(pr, Lc, false, stream); });
+        op_zero(g, 4 * gld);
+        op_kernel(OP_P2G, Lc);
     }
 
     // G2P of layer t (`in`, grid `g`) -> `out`; no P2G of the result.
@@ -282,7 +333,7 @@ struct dmpm_sim {
         Lc.ctl = ctl[t];
         Lc.grid = g;
         Lc.out = out;
-        timed(1, [&] { g2p(pr, Lc, stream); });
+        op_kernel(OP_G2P, Lc);
     }
 
     // Layer t (`in`, grid `g`) -> layer t+1 (`out`) and its grid `g1`.
@@ -296,11 +347,11 @@ struct dmpm_sim {
         Lc.ctl1 = t + 1 < L ? ctl[t + 1] : nullptr;
         Lc.grid1 = g1;
         if (cfg.unfused || cfg.deterministic) {
-            timed(1, [&] { g2p(pr, Lc, stream); });
+            op_kernel(OP_G2P, Lc);
             p2g_into(out, t + 1, g1);
         } else {
-            timed(4, [&] { zero(g1, 4 * gld); });
-            timed(5, [&] { dmpm::step(pr, Lc, stream); });
+            op_zero(g1, 4 * gld);
+            op_kernel(OP_STEP, Lc);
         }
     }
 
@@ -313,7 +364,7 @@ struct dmpm_sim {
             for (int t = start + 1; t < L; ++t) dfree(ctl[t]);
         if (start + 1 < L) {
             // `cur` may live in a segment buffer, which this pass does not overwrite
-            if (!capturing) HIPCHK(hipEventRecord(ev[0], stream));
+            if (!capturing && !rec) HIPCHK(hipEventRecord(ev[0], stream));
             void* ga = grid;
             void* gb = grid2;
             p2g_into(cur, start, ga);
@@ -330,7 +381,7 @@ struct dmpm_sim {
                 if (stored(t + 1)) valid[t + 1] = 1;
                 cur = out;
             }
-            if (!capturing) HIPCHK(hipEventRecord(ev[1], stream));
+            if (!capturing && !rec) HIPCHK(hipEventRecord(ev[1], stream));
         }
         seg_base = -1;
     }
@@ -345,9 +396,9 @@ struct dmpm_sim {
             fprintf(stderr, "[WARN] Render gradient size mismatch: stored %d but layer has %d points. Skipping injection.\n",
                     rg_n, n);
         }
-        copy_elems(pr, carry[carry_cur], adj_last, NSTATE * ld, stream);
+        op_copy(carry[carry_cur], adj_last, NSTATE * ld);
         carry_layer = L - 1;
-        if (!capturing) HIPCHK(hipEventRecord(ev[2], stream));
+        if (!capturing && !rec) HIPCHK(hipEventRecord(ev[2], stream));
         for (int t = L - 2; t >= stop; --t) {
             const void* st;
             void* g;
@@ -376,23 +427,24 @@ struct dmpm_sim {
             }
             Lc.gctl = gout;
             if (cfg.deterministic) {
+                if (rec) throw Error(DMPM_ERR_STATE, "deterministic mode cannot be batched");
                 timed(2, [&] { det_adj_scatter(pr, Lc, det_buf(), stream); });
             } else {
-                timed(4, [&] { zero(ggv, 3 * gld); });
-                timed(2, [&] { adj_scatter(pr, Lc, stream); });
+                op_zero(ggv, 3 * gld);
+                op_kernel(OP_ADJ_SCATTER, Lc);
             }
-            timed(3, [&] { adj_gather(pr, Lc, stream); });
+            op_kernel(OP_ADJ_GATHER, Lc);
             if (t == stop && want_gctl[t]) {
                 if (!gctl[t]) gctl[t] = dalloc(ctl_bytes());
-                copy_elems(pr, gctl[t], gctl_stop, 9 * ld, stream);
+                op_copy(gctl[t], gctl_stop, 9 * ld);
             }
             carry_cur ^= 1;
             carry_layer = t;
         }
-        if (!capturing) HIPCHK(hipEventRecord(ev[3], stream));
+        if (!capturing && !rec) HIPCHK(hipEventRecord(ev[3], stream));
         if (stop == L - 1) {
             // no step: dL/d dFc of the last layer is dL/dF
-            copy_elems(pr, gctl_stop, (char*)carry[carry_cur] + PF * ld * esz, 9 * ld, stream);
+            op_copy(gctl_stop, (char*)carry[carry_cur] + PF * ld * esz, 9 * ld);
         }
     }
 
@@ -400,8 +452,7 @@ struct dmpm_sim {
     void accumulate_planes(const void* src, int plane_off, int nc) {
         // src holds internal-order planes; add them plane by plane (identity permutation)
         for (int c = 0; c < nc; ++c)
-            accumulate_carry(pr, (const char*)src + (size_t)c * ld * esz, pr == PREC_F64, adj_last,
-                             plane_off + c, 1, nullptr, n, ld, stream);
+            op_accum((char*)adj_last + (size_t)(plane_off + c) * ld * esz, (const char*)src + (size_t)c * ld * esz, n);
     }
 
     // Upload a caller array (n x nc, caller order) into a device staging
@@ -1515,6 +1566,160 @@ int dmpm_graph_synchronize(dmpm_graph* g) {
 
 int dmpm_graph_destroy(dmpm_graph* g) {
     return guard([&] { delete g; });
+}
+
+// ---- batched episodes ---------------------------------------------------------
+struct dmpm_batch {
+    int count = 0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void* dev = nullptr;  // parameter blocks, block prefix sums and ranges of every op
+    ~dmpm_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (dev) (void)hipFree(dev);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+int dmpm_batch_capture(dmpm_sim** sims, int count, int start_layer, int stop_layer, dmpm_batch** out) {
+    return guard([&] {
+        if (!sims || count <= 0 || !out) throw Error(DMPM_ERR_ARG, "bad arguments");
+        for (int i = 0; i < count; ++i) {
+            check_sim(sims[i]);
+            if (sims[i]->n <= 0) throw Error(DMPM_ERR_STATE, "sim without particles");
+            if (sims[i]->cfg.deterministic) throw Error(DMPM_ERR_STATE, "deterministic mode cannot be batched");
+        }
+        dmpm_sim* s0 = sims[0];
+        const Launch L0 = s0->launch();
+        for (int i = 1; i < count; ++i) {
+            const dmpm_sim* s = sims[i];
+            const Launch Li = s->launch();
+            if (s->pr != s0->pr || s->cfg.order != s0->cfg.order || s->cfg.model != s0->cfg.model ||
+                s->cfg.adjoint != s0->cfg.adjoint || Li.block != L0.block || Li.lds_bytes != L0.lds_bytes ||
+                s->cfg.unfused != s0->cfg.unfused)
+                throw Error(DMPM_ERR_ARG, "batched sims must share precision, B-spline order, model, adjoint mode, "
+                                          "block size, LDS budget and fusion");
+        }
+        // record each sim's episode (host bookkeeping runs as for an eager episode)
+        std::vector<std::vector<dmpm_sim::Op>> ops(count);
+        for (int i = 0; i < count; ++i) {
+            dmpm_sim* s = sims[i];
+            HIPCHK(hipStreamSynchronize(s->stream));
+            s->rec = &ops[i];
+            try {
+                s->forward(start_layer, false);
+                s->backward(stop_layer);
+            } catch (...) {
+                s->rec = nullptr;
+                throw;
+            }
+            s->rec = nullptr;
+        }
+        const size_t nops = ops[0].size();
+        for (int i = 1; i < count; ++i) {
+            if (ops[i].size() != nops) throw Error(DMPM_ERR_ARG, "batched sims must run episodes of the same structure");
+            for (size_t k = 0; k < nops; ++k)
+                if (ops[i][k].kind != ops[0][k].kind)
+                    throw Error(DMPM_ERR_ARG, "batched sims must run episodes of the same structure");
+        }
+        // lay out every op's table in one buffer
+        const Prec pr = s0->pr;
+        // the kernels index the parameter table as an array of the kernels'
+        // parameter struct, so the host stride must be exactly its size;
+        // only each table's start is aligned
+        const size_t kpb = kp_bytes(pr);
+        const size_t esz = s0->esz;
+        struct Item {
+            size_t off, off2;
+            int total;
+            long long max_n;
+        };
+        std::vector<Item> items(nops);
+        size_t bytes = 0;
+        for (size_t k = 0; k < nops; ++k) {
+            items[k].off = bytes;
+            if (op_is_kernel(ops[0][k].kind)) {
+                bytes += kpb * count;
+                bytes = (bytes + 15) & ~size_t(15);
+                items[k].off2 = bytes;
+                bytes += ((count + 1) * sizeof(int) + 15) & ~size_t(15);
+            } else {
+                bytes += sizeof(Seg) * count;
+                bytes = (bytes + 15) & ~size_t(15);
+            }
+        }
+        std::vector<char> host(std::max<size_t>(bytes, 16), 0);
+        for (size_t k = 0; k < nops; ++k) {
+            const int kind = ops[0][k].kind;
+            if (op_is_kernel(kind)) {
+                int* bs = (int*)(host.data() + items[k].off2);
+                bs[0] = 0;
+                for (int i = 0; i < count; ++i) {
+                    const Launch& Li = ops[i][k].L;
+                    build_kp(pr, kind, Li, host.data() + items[k].off + kpb * i);
+                    bs[i + 1] = bs[i] + instance_blocks(Li.n, Li.block);
+                }
+                items[k].total = bs[count];
+            } else {
+                Seg* sg = (Seg*)(host.data() + items[k].off);
+                long long mx = 0;
+                for (int i = 0; i < count; ++i) {
+                    const auto& o = ops[i][k];
+                    if (kind != OP_ACCUM && ((((uintptr_t)o.dst | (uintptr_t)o.src) & 15) || (o.n * (long long)esz) % 16))
+                        throw Error(DMPM_ERR_STATE, "unaligned buffer range in a batched episode");
+                    sg[i] = Seg{o.dst, o.src, o.n};
+                    mx = std::max(mx, o.n);
+                }
+                items[k].max_n = mx;
+            }
+        }
+        auto b = std::make_unique<dmpm_batch>();
+        b->count = count;
+        HIPCHK(hipMalloc(&b->dev, host.size()));
+        HIPCHK(hipMemcpy(b->dev, host.data(), host.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+        hipError_t launch_err = hipSuccess;
+        for (size_t k = 0; k < nops; ++k) {
+            const int kind = ops[0][k].kind;
+            const char* base = (const char*)b->dev;
+            if (op_is_kernel(kind))
+                launch_batch(pr, kind, s0->ph, L0.lds_bytes, L0.block, base + items[k].off,
+                             (const int*)(base + items[k].off2), count, items[k].total, b->stream);
+            else
+                launch_segs(pr, kind, (const Seg*)(base + items[k].off), count, items[k].max_n, b->stream);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess && launch_err == hipSuccess) launch_err = e;
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t end_err = hipStreamEndCapture(b->stream, &graph);
+        b->graph = graph;
+        HIPCHK(launch_err);
+        HIPCHK(end_err);
+        HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        *out = b.release();
+    });
+}
+
+int dmpm_batch_launch(dmpm_batch* b) {
+    return guard([&] {
+        if (!b || !b->exec) throw Error(DMPM_ERR_ARG, "null batch");
+        HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    });
+}
+
+int dmpm_batch_synchronize(dmpm_batch* b) {
+    return guard([&] {
+        if (!b) throw Error(DMPM_ERR_ARG, "null batch");
+        HIPCHK(hipStreamSynchronize(b->stream));
+    });
+}
+
+int dmpm_batch_destroy(dmpm_batch* b) {
+    return guard([&] { delete b; });
 }
 
 int dmpm_kernel_timer(dmpm_sim* s, int capacity) {

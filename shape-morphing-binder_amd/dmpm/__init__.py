@@ -7,6 +7,7 @@ The drop-in module for the reference's Python API is `diffmpm_bindings`
 (next to this package); `dmpm.rollout` holds the torch.autograd.Function.
 """
 from . import _lib
-from .sim import EpisodeGraph, Physics, Simulator, calculate_volumes, lame_parameters, make_config, p2g_mass
+from .sim import (BatchedEpisode, EpisodeGraph, Physics, Simulator, calculate_volumes, lame_parameters, make_config,
+                  p2g_mass)
 
-__all__ = ["EpisodeGraph", "Physics", "Simulator", "calculate_volumes", "lame_parameters", "make_config", "p2g_mass", "_lib"]
+__all__ = ["BatchedEpisode", "EpisodeGraph", "Physics", "Simulator", "calculate_volumes", "lame_parameters", "make_config", "p2g_mass", "_lib"]

@@ -148,6 +148,10 @@ def lib() -> C.CDLL:
         "dmpm_write_layer": (i, [vp, i, C.c_char_p, i]),
         "dmpm_read_particles": (i, [vp, C.c_char_p, i]),
         "dmpm_winding_numbers": (i, [vp, i, vp, i, vp, i, vp]),
+        "dmpm_batch_capture": (i, [C.POINTER(vp), i, i, i, C.POINTER(vp)]),
+        "dmpm_batch_launch": (i, [vp]),
+        "dmpm_batch_synchronize": (i, [vp]),
+        "dmpm_batch_destroy": (i, [vp]),
     }
     for name, (res, args) in proto.items():
         if not hasattr(L, name):
@@ -172,6 +176,7 @@ EXPORTED = [
     "dmpm_kernel_stats", "dmpm_window_stats", "dmpm_graph_capture", "dmpm_graph_launch", "dmpm_graph_synchronize", "dmpm_graph_destroy",
     "dmpm_get_config", "dmpm_set_adam_state", "dmpm_file_info", "dmpm_write_file", "dmpm_read_file",
     "dmpm_write_layer", "dmpm_read_particles", "dmpm_winding_numbers",
+    "dmpm_batch_capture", "dmpm_batch_launch", "dmpm_batch_synchronize", "dmpm_batch_destroy",
 ]
 
 

@@ -395,3 +395,37 @@ class EpisodeGraph:
             self.close()
         except Exception:
             pass
+
+
+class BatchedEpisode:
+    """Many sims' episodes (forward from `start` keeping per-layer controls,
+    backward to `stop`) as one hipGraph of batched launches: each kernel runs
+    once over all instances.  The sims must share precision, physics
+    templates, launch shape and episode structure.  Synchronise the sims
+    before launch() and call synchronize() before reading them."""
+
+    def __init__(self, sims, start: int = 0, stop: int = 0):
+        for sm in sims:
+            sm.synchronize()
+        arr = (C.c_void_p * len(sims))(*[sm.handle for sm in sims])
+        h = C.c_void_p()
+        L.check(L.lib().dmpm_batch_capture(arr, len(sims), int(start), int(stop), C.byref(h)))
+        self._h = h
+        self.sims = list(sims)
+
+    def launch(self):
+        L.check(L.lib().dmpm_batch_launch(self._h))
+
+    def synchronize(self):
+        L.check(L.lib().dmpm_batch_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().dmpm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
