@@ -19,6 +19,7 @@ wall time of K episodes.
 from __future__ import annotations
 
 import argparse
+import contextlib
 import json
 import os
 import sys
@@ -76,6 +77,22 @@ def cpu_threads() -> int:
     return max(1, min(n, 16))
 
 
+@contextlib.contextmanager
+def stdout_to_stderr():
+    """Send what native code writes to file descriptor 1 (the reference
+    prints its grid size there) to stderr, so that stdout carries only the
+    result line."""
+    sys.stdout.flush()
+    saved = os.dup(1)
+    try:
+        os.dup2(2, 1)
+        yield
+    finally:
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
 def cpu_baseline(sc, seed_gx, steps: int = 2):
     """The reference's own OpenMP C++ path (compiled from its sources into
     oracle/_ref) on a bounded sample of the same workload: all particles,
@@ -90,7 +107,8 @@ def cpu_baseline(sc, seed_gx, steps: int = 2):
         ref = Ref(threads)
         prm = reference_params(dims=sc.dims, dx=sc.dx, dt=1e-4, drag=0.0, smoothing=0.0)
         t0 = time.perf_counter()
-        fwd, bwd = ref.time_steps(prm, s, steps, seed_gx.astype(np.float32))
+        with stdout_to_stderr():
+            fwd, bwd = ref.time_steps(prm, s, steps, seed_gx.astype(np.float32))
         log(f"[cpu] reference: {steps} steps fwd {fwd:.2f}s bwd {bwd:.2f}s ({time.perf_counter() - t0:.1f}s total)")
         return {"value": sc.n * steps / (fwd + bwd), "unit": "particle-steps/s", "cores": threads,
                 "kind": "reference",
@@ -153,7 +171,8 @@ def run_episodes(sim, count):
         sim.backward(0)
 
 
-def batched_workload(dt: float = 2.5e-4, T: int = 100, K: int = 10, reps: int = 2) -> dict:
+def batched_workload(dt: float = 2.5e-4, T: int = 100, K: int = 10, reps: int = 2, lds_bytes: int = 0,
+                     block: int = 0) -> dict:
     """64 independent instances of uneven size (particle counts log-uniform
     in [4,096, 131,072], grids of 32/48/64/96 cells, random ellipsoids and
     materials), each a T-step forward + backward episode seeded at its last
@@ -166,7 +185,7 @@ def batched_workload(dt: float = 2.5e-4, T: int = 100, K: int = 10, reps: int = 
     for spec in config4_instances():
         sc = instance_scene(spec)
         sim = Simulator(sc.dims, sc.dx, sc.grid_min, physics=Physics.plain(dt=dt), precision="f32",
-                        checkpoint_interval=K)
+                        checkpoint_interval=K, lds_bytes=lds_bytes, block=block)
         sim.set_state(sc.state())
         sim.set_up(T + 1)
         rng = np.random.default_rng(spec["seed"])
@@ -419,7 +438,7 @@ def main():
         log(f"[bench] reference physics: {1e3 * wr:.1f} ms/episode")
 
     if not args.no_batched:
-        extra["batched"] = batched_workload()
+        extra["batched"] = batched_workload(lds_bytes=lds_bytes, block=block)
     if not args.no_full_scale:
         extra["full_scale"] = full_scale_episode()
 

@@ -362,19 +362,34 @@ struct dmpm_sim {
         // a rollout with per-step controls keeps them.
         if (zero_controls)
             for (int t = start + 1; t < L; ++t) dfree(ctl[t]);
+        // The segment the backward pass enters first (holding layer L-2) is
+        // written straight into the segment buffers -- states and grids --
+        // when this pass computes all of it, so the backward pass need not
+        // recompute it.  `cur` may live in a segment buffer of an earlier
+        // segment; it is read by the first step only, before any layer of the
+        // last segment is written.
+        const int c_last = K > 1 && L >= 2 ? ((L - 2) / K) * K : -1;
+        const bool keep = c_last >= 0 && start <= c_last;
         if (start + 1 < L) {
-            // `cur` may live in a segment buffer, which this pass does not overwrite
             if (!capturing && !rec) HIPCHK(hipEventRecord(ev[0], stream));
-            void* ga = grid;
-            void* gb = grid2;
+            void* ga = keep && start == c_last ? seg_grid[0] : grid;
             p2g_into(cur, start, ga);
             int sc = 0;
             for (int t = start; t < L - 1; ++t) {
-                void* out = stored(t + 1) ? layer[t + 1] : scratch[sc];
-                if (!stored(t + 1)) sc ^= 1;
+                const bool in_last = keep && t + 1 >= c_last;
+                void* out;
+                if (stored(t + 1)) {
+                    out = layer[t + 1];
+                } else if (in_last) {
+                    out = seg[t + 1 - c_last];
+                } else {
+                    out = scratch[sc];
+                    sc ^= 1;
+                }
                 if (t + 1 < L - 1) {
+                    void* gb = in_last ? seg_grid[t + 1 - c_last] : (ga == grid ? grid2 : grid);
                     fused_step(cur, t, ga, out, gb);
-                    std::swap(ga, gb);
+                    ga = gb;
                 } else {
                     g2p_only(cur, t, ga, out);
                 }
@@ -384,6 +399,10 @@ struct dmpm_sim {
             if (!capturing && !rec) HIPCHK(hipEventRecord(ev[1], stream));
         }
         seg_base = -1;
+        if (keep) {
+            seg[0] = layer[c_last];
+            seg_base = c_last;
+        }
     }
 
     void backward(int stop) {
