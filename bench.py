@@ -9,10 +9,16 @@ sphere of radius 0.3 (jittered lattice), 128^3 cells (129^3 nodes,
 dx = 1/128), E = 5e4, nu = 0.45, rho = 1e3, dt = 1e-4, an 8-mode sine
 velocity field of amplitude 0.5, 300 steps, plain physics (quadratic
 B-spline, Neo-Hookean, sticky walls at 3 cells).  A "step" of this benchmark
-is one episode: 300 forward steps plus the 300-step adjoint back to layer 0
-(with checkpoint recomputation), seeded at the last layer with a Gaussian
-dL/dx (sigma 1e-3) and dL/dF (sigma 1e-4).  value = particles * 300 * K /
-wall time of K episodes.
+is one episode: 300 forward steps plus the 300-step adjoint back to layer 0,
+seeded at the last layer with a Gaussian dL/dx (sigma 1e-3) and dL/dF
+(sigma 1e-4).  value = particles * 300 * K / wall time of K episodes.
+
+Tape: the whole episode's tape (301 layer states and 300 P2G grids, about
+41 GB in FP32) fits in HBM, so by default every layer's state and grid is kept
+(checkpoint interval 1) and the backward pass recomputes nothing -- the
+reference's CompGraph keeps every layer and its grid too.  The checkpointed
+form (checkpoint interval 25: about 5 GB, one extra forward pass of
+recomputation) is measured beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -291,7 +297,10 @@ def main():
     ap.add_argument("--particles", type=int, default=1048576)
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--episode-steps", type=int, default=300)
-    ap.add_argument("--checkpoint", type=int, default=25, help="checkpoint interval K")
+    ap.add_argument("--checkpoint", type=int, default=1,
+                    help="checkpoint interval K (1: every layer and its P2G grid kept in HBM)")
+    ap.add_argument("--no-checkpointed", action="store_true",
+                    help="skip the checkpointed (K = 25) comparison run")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-reference-physics", action="store_true")
@@ -323,7 +332,7 @@ def main():
 
     copy_bw = stream_copy_GBps()
     log(f"[bench] plain copy: {copy_bw:.0f} GB/s")
-    T, K = args.episode_steps, args.checkpoint
+    T, K = args.episode_steps, max(1, args.checkpoint)
     t0 = time.perf_counter()
     sc = config3(n=args.particles, cells=args.cells)
     log(f"[bench] scene: {sc.n} particles, {sc.dims} nodes, h={sc.h:.5f} ({time.perf_counter() - t0:.1f}s)")
@@ -381,7 +390,7 @@ def main():
         kernels[name] = entry
     dom = max((k for k in kernels if k in BYTES), key=lambda k: kernels[k]["total_ms"])
     d = kernels[dom]
-    default_workload = (args.particles, args.cells, T, K) == (1048576, 128, 300, 25)
+    default_workload = (args.particles, args.cells, T) == (1048576, 128, 300)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": d["GBps"] / HBM_PEAK_GBS,
                 "traffic": profiled_traffic("k_" + dom) if default_workload else None,
@@ -397,6 +406,20 @@ def main():
     sim.close()
 
     extra = {}
+    if not args.no_checkpointed and K != 25:
+        simc = episode_sim(sc, "f32", 25, T, seed_gx, seed_gF, lds_bytes=lds_bytes, block=block)
+        run_episodes(simc, args.warmup)
+        simc.synchronize()
+        t0 = time.perf_counter()
+        run_episodes(simc, args.steps)
+        simc.synchronize()
+        wc = (time.perf_counter() - t0) / args.steps
+        _, peak_c = simc.memory()
+        simc.close()
+        extra["checkpointed"] = {"checkpoint_interval": 25, "value": sc.n * T / wc, "ms_per_step": 1e3 * wc,
+                                 "peak_device_bytes": peak_c}
+        log(f"[bench] f32 K=25: {1e3 * wc:.1f} ms/episode, {sc.n * T / wc:.4g} particle-steps/s, "
+            f"peak {peak_c / 1e9:.2f} GB")
     if not args.no_fp64:
         sim64 = episode_sim(sc, "f64", K, T, seed_gx, seed_gF)
         run_episodes(sim64, 1)
@@ -464,6 +487,8 @@ def main():
         "config": {
             "workload": "precision-stress sphere: 1,048,576 particles, 128^3 grid, 300-step fwd+bwd episode",
             "particles": sc.n, "grid_nodes": list(sc.dims), "episode_steps": T, "checkpoint_interval": K,
+            "tape": ("every layer's state and P2G grid kept in HBM (no recomputation)" if K == 1
+                     else f"checkpoints every {K} layers, segments recomputed"),
             "physics": "plain: quadratic B-spline, Neo-Hookean, sticky walls (3 cells)",
             "precision": "FP32 state/arithmetic, FP64 det/log J/F^-T", "E": 5e4, "nu": 0.45, "dt": 1e-4,
             "active_nodes": gact, "peak_device_bytes": mem_peak,

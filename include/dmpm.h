@@ -95,6 +95,10 @@ typedef struct {
     int unfused;            /* 1: separate G2P and P2G kernels instead of the fused forward step */
     int deterministic;      /* 1: two-pass scatters (sort by cell, node-centric sums in a fixed
                                order): bitwise reproducible, slower */
+    int recompute_grids;    /* with checkpoint_interval 1: 0 keeps every layer's P2G grid (m, p)
+                               for the backward pass, as CompGraph keeps each layer's Grid
+                               (CompGraph.h:13-30), at 4 values per node per layer; 1 recomputes
+                               each layer's P2G in the backward pass instead */
 } dmpm_config;
 
 /* Fill `cfg` with the reference's physics (cubic, fixed-corotated, zeta

@@ -94,6 +94,20 @@ struct dmpm_sim {
     std::vector<void*> seg;    // K segment buffers (recomputed states)
     std::vector<void*> seg_grid;  // grids (m, p) of the segment's layers
     int seg_base = -1;
+    // With K = 1 (and recompute_grids off) the forward pass keeps every
+    // layer's P2G grid (m, p) for the backward pass, as the reference's
+    // CompGraph keeps each layer's Grid: lgrid[t] is layer t's grid,
+    // gvalid[t] whether it matches layer t's state and control.
+    std::vector<void*> lgrid;
+    std::vector<char> gvalid;
+    bool keep_grids() const { return K <= 1 && !cfg.recompute_grids; }
+    // A control (or a parameter the P2G reads) changed: layer t's grid (all
+    // grids for t < 0) and any recomputed segment no longer hold.
+    void grids_stale(int t) {
+        seg_base = -1;
+        if (t < 0) std::fill(gvalid.begin(), gvalid.end(), 0);
+        else if (t < (int)gvalid.size()) gvalid[t] = 0;
+    }
     void* scratch[2] = {nullptr, nullptr};
     void* grid = nullptr;  // m, p (4 planes)
     void* grid2 = nullptr; // second grid for the fused forward step
@@ -264,8 +278,11 @@ struct dmpm_sim {
         // seg[0] aliases a stored layer; only seg[1..] are owned
         for (size_t j = 1; j < seg.size(); ++j) dfree(seg[j]);
         for (auto& g : seg_grid) dfree(g);
+        for (auto& g : lgrid) dfree(g);
         seg.clear();
         seg_grid.clear();
+        lgrid.clear();
+        gvalid.clear();
         seg_base = -1;
         layer.resize(L, nullptr);
         valid.assign(L, 0);
@@ -278,6 +295,11 @@ struct dmpm_sim {
             seg_grid.resize(k, nullptr);
             for (int j = 1; j < k; ++j) seg[j] = dalloc(state_bytes());
             for (int j = 0; j < k; ++j) seg_grid[j] = dalloc(4 * gld * esz);
+        }
+        if (keep_grids() && L > 1) {
+            lgrid.resize(L - 1, nullptr);
+            gvalid.assign(L - 1, 0);
+            for (auto& g : lgrid) g = dalloc(4 * gld * esz);
         }
         for (auto& p : scratch)
             if (!p) p = dalloc(state_bytes());
@@ -370,10 +392,12 @@ struct dmpm_sim {
         // last segment is written.
         const int c_last = K > 1 && L >= 2 ? ((L - 2) / K) * K : -1;
         const bool keep = c_last >= 0 && start <= c_last;
+        const bool kg = keep_grids();
         if (start + 1 < L) {
             if (!capturing && !rec) HIPCHK(hipEventRecord(ev[0], stream));
-            void* ga = keep && start == c_last ? seg_grid[0] : grid;
+            void* ga = kg ? lgrid[start] : keep && start == c_last ? seg_grid[0] : grid;
             p2g_into(cur, start, ga);
+            if (kg) gvalid[start] = 1;
             int sc = 0;
             for (int t = start; t < L - 1; ++t) {
                 const bool in_last = keep && t + 1 >= c_last;
@@ -387,8 +411,11 @@ struct dmpm_sim {
                     sc ^= 1;
                 }
                 if (t + 1 < L - 1) {
-                    void* gb = in_last ? seg_grid[t + 1 - c_last] : (ga == grid ? grid2 : grid);
+                    void* gb = kg        ? lgrid[t + 1]
+                               : in_last ? seg_grid[t + 1 - c_last]
+                                         : (ga == grid ? grid2 : grid);
                     fused_step(cur, t, ga, out, gb);
+                    if (kg) gvalid[t + 1] = 1;
                     ga = gb;
                 } else {
                     g2p_only(cur, t, ga, out);
@@ -425,6 +452,13 @@ struct dmpm_sim {
                 ensure_segment(t);
                 st = t == seg_base ? layer[t] : seg[t - seg_base];
                 g = seg_grid[t - seg_base];
+            } else if (keep_grids()) {
+                st = state(t);
+                g = lgrid[t];
+                if (!gvalid[t]) {
+                    p2g_into(st, t, g);
+                    gvalid[t] = 1;
+                }
             } else {
                 st = state(t);
                 g = grid;
@@ -634,6 +668,7 @@ void ensure_adam(dmpm_sim* s, int t) {
 
 void descend_adam(dmpm_sim* s, int t, double alpha, double grad_norm, double b1, double b2, double eps, int ts) {
     ensure_adam(s, t);
+    s->grids_stale(t);
     // gradient_norm as passed by the caller: Descend_Adam only uses 1/max(gn,1e-12)
     double sq = grad_norm * grad_norm;
     HIPCHK(hipMemcpyAsync(s->red + 4, &sq, sizeof(double), hipMemcpyHostToDevice, s->stream));
@@ -656,6 +691,7 @@ void backup_layer(dmpm_sim* s, int t, void* buf) {
 }
 void restore_layer(dmpm_sim* s, int t, const void* buf) {
     const long long e = 9 * s->ld;
+    s->grids_stale(t);
     copy_elems(s->pr, s->ctl[t], (const char*)buf + 0 * e * s->esz, e, s->stream);
     copy_elems(s->pr, s->am[t], (const char*)buf + 1 * e * s->esz, e, s->stream);
     copy_elems(s->pr, s->av[t], (const char*)buf + 2 * e * s->esz, e, s->stream);
@@ -874,7 +910,7 @@ int dmpm_set_sim_params(dmpm_sim* s, double dt, double drag, double smoothing, c
         if (f_ext)
             for (int i = 0; i < 3; ++i) s->cfg.f_ext[i] = f_ext[i];
         s->ph = phys_of(s->cfg);
-        s->seg_base = -1;
+        s->grids_stale(-1);
     });
 }
 
@@ -1196,7 +1232,7 @@ int dmpm_set_control(dmpm_sim* s, int layer, const void* dFc, int dtype, int on_
         const void* d = s->stage(dFc, (size_t)s->n * 9 * dsize(dtype), on_device);
         import_planes(s->pr, d, dtype == DMPM_DTYPE_F64, s->ctl[layer], s->perm, s->n, 9, s->ld, s->stream);
         s->unstage();
-        s->seg_base = -1;
+        s->grids_stale(layer);
     });
 }
 

@@ -14,7 +14,8 @@ Differences a caller can observe:
     the device on each call (they are live views, as in the reference, but
     carry no host copy).
   * Extra keyword arguments (a superset): CompGraph(..., precision="f32",
-    physics=None, checkpoint_interval=1); PointCloud.from_arrays(...);
+    physics=None, checkpoint_interval=1, recompute_grids=False);
+    PointCloud.from_arrays(...);
     device=... on the *_torch getters; get_velocities/get_volumes.
 """
 from __future__ import annotations
@@ -236,12 +237,13 @@ class CompGraph:
     like the reference) or "f64"."""
 
     def __init__(self, pc: PointCloud, grid: Grid, target_grid: Grid, precision: str = "f32",
-                 physics: Optional[Physics] = None, checkpoint_interval: int = 1):
+                 physics: Optional[Physics] = None, checkpoint_interval: int = 1, recompute_grids: bool = False):
         if pc is None or grid is None or target_grid is None:
             raise TypeError("CompGraph(pc, grid, target_grid): arguments must not be None")
         ph = physics or Physics.reference()
         self._sim = Simulator(grid.dims, grid.dx, tuple(float(a) for a in grid.min_point), physics=ph,
-                              precision=precision, checkpoint_interval=checkpoint_interval)
+                              precision=precision, checkpoint_interval=checkpoint_interval,
+                              recompute_grids=recompute_grids)
         self._sim.set_state(pc._state())
         self._sim.set_target_grid_mass(target_grid.m)
         self._grid = grid

@@ -49,6 +49,7 @@ class Config(C.Structure):
         ("block", C.c_int),
         ("unfused", C.c_int),
         ("deterministic", C.c_int),
+        ("recompute_grids", C.c_int),
     ]
 
 

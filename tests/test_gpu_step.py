@@ -16,10 +16,10 @@ DT = 1e-3
 TOL = {"f64": (1e-10, 1e-10), "f32": (1e-4, 1e-3)}
 
 
-def _sim(sc, name, prec, K=1):
+def _sim(sc, name, prec, K=1, recompute_grids=False):
     from dmpm import Physics, Simulator
     return Simulator(sc.dims, sc.dx, sc.grid_min, physics=Physics(**physics_kwargs(name, DT)), precision=prec,
-                     checkpoint_interval=K)
+                     checkpoint_interval=K, recompute_grids=recompute_grids)
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
@@ -66,35 +66,56 @@ def test_backward_step(name, prec):
     sim.close()
 
 
-@pytest.mark.parametrize("K", [1, 3])
-def test_rollout_checkpointing(K):
-    """A 7-step episode with controls at layers 0 and 2; the checkpointed
-    backward (K=3) must give the oracle's gradients like the stored one."""
+# (K, recompute_grids): every layer with its kept P2G grid; every layer with
+# the grid recomputed in the backward pass; checkpoints every 3 layers; a
+# short last segment (layers 5-6 of 0-7); one segment holding the whole tape.
+TAPES = [(1, False), (1, True), (3, False), (5, False), (7, False)]
+
+
+@pytest.mark.parametrize("K,recompute", TAPES)
+def test_rollout_checkpointing(K, recompute):
+    """A 7-step episode with controls at layers 0 and 2: every tape layout
+    must give the oracle's states and gradients.  Then a control is set at
+    layer 3 and the forward pass re-run from layer 3 only: the layers and
+    gradients must follow the new control (kept grids and segments of the
+    old control must not be reused)."""
     sc, s = perturbed_scene(dfc=0.0)
     name, prec, T = "plain", "f64", 7
     rng = np.random.default_rng(3)
-    ctl = [0.01 * rng.normal(size=(sc.n, 3, 3)) if t in (0, 2) else None for t in range(T)]
+    ctl = [0.01 * rng.normal(size=(sc.n, 3, 3)) if t in (0, 2, 3) else None for t in range(T)]
     adj = random_adjoint(sc.n, seed=9)
     orc = Oracle(prec)
     prm = oracle_params(name, sc, DT)
-    layers = orc.rollout(prm, dict(s, dFc=None), T, [c if c is not None else np.zeros((sc.n, 3, 3)) for c in ctl])
-    g0, gctl = orc.rollout_grad(prm, layers, adj)
-    print(f"[rollout] K={K} T={T} n={sc.n}", flush=True)
-    sim = _sim(sc, name, prec, K=K)
+    z33 = np.zeros((sc.n, 3, 3))
+
+    def reference(ctl_layers):
+        cs = [ctl[t] if t in ctl_layers else z33 for t in range(T)]
+        layers = orc.rollout(prm, dict(s, dFc=None), T, cs)
+        g0, gctl = orc.rollout_grad(prm, layers, adj)
+        return layers, g0, gctl
+
+    print(f"[rollout] K={K} recompute_grids={recompute} T={T} n={sc.n}", flush=True)
+    sim = _sim(sc, name, prec, K=K, recompute_grids=recompute)
     sim.set_state(dict(s, dFc=None))
     sim.set_up(T + 1)
     for t in (0, 2):
         sim.set_control(t, ctl[t])
     sim.request_control_grad(2)
     sim.forward(0, keep_controls=True)
-    for k in ("x", "v", "F", "C"):
-        assert rel_err(sim.get(T, k, dtype=np.float64), layers[T][k]) < 1e-10, k
-        assert rel_err(sim.get(4, k, dtype=np.float64), layers[4][k]) < 1e-10, k
     sim.set_last_grad(x=adj["x"], v=adj["v"], F=adj["F"], C_=adj["C"])
-    sim.backward(0)
-    for k in ("x", "v", "F", "C"):
-        e = rel_err(sim.get_grad(0, k, dtype=np.float64), g0[k])
-        assert e < 1e-9, f"d{k}: {e:.3e}"
-    assert rel_err(sim.get_control_grad(0, dtype=np.float64), gctl[0]) < 1e-9
-    assert rel_err(sim.get_control_grad(2, dtype=np.float64), gctl[2]) < 1e-9
+    for case, live in (("controls 0, 2", (0, 2)), ("control 3 added, forward from 3", (0, 2, 3))):
+        if 3 in live:
+            sim.set_control(3, ctl[3])
+            sim.request_control_grad(3)
+            sim.forward(3, keep_controls=True)
+        layers, g0, gctl = reference(live)
+        for k in ("x", "v", "F", "C"):
+            assert rel_err(sim.get(T, k, dtype=np.float64), layers[T][k]) < 1e-10, (case, k)
+            assert rel_err(sim.get(4, k, dtype=np.float64), layers[4][k]) < 1e-10, (case, k)
+        sim.backward(0)
+        for k in ("x", "v", "F", "C"):
+            e = rel_err(sim.get_grad(0, k, dtype=np.float64), g0[k])
+            assert e < 1e-9, f"{case}: d{k}: {e:.3e}"
+        for t in live:
+            assert rel_err(sim.get_control_grad(t, dtype=np.float64), gctl[t]) < 1e-9, (case, t)
     sim.close()
