@@ -99,6 +99,15 @@ typedef struct {
                                for the backward pass, as CompGraph keeps each layer's Grid
                                (CompGraph.h:13-30), at 4 values per node per layer; 1 recomputes
                                each layer's P2G in the backward pass instead */
+    int adjoint_recompute;  /* 0: the adjoint of step t reads G2P's v' and C' from layer t+1 when
+                               that layer is the forward step of layer t as it stands (the
+                               reference reads layer n+1's C, BackPropagation.cpp:27-31) and the
+                               trust region is off or the adjoint is the reference's; 1: always
+                               recompute them by a G2P from layer t's grid */
+    int split_nodes;        /* thread blocks of the particle kernels take up to `block` consecutive
+                               particles (Hilbert order of the uploaded positions), cut where the
+                               box of their stencils would exceed this many grid nodes, so that
+                               each block's LDS window stays small (0: 512; negative: no cuts) */
 } dmpm_config;
 
 /* Fill `cfg` with the reference's physics (cubic, fixed-corotated, zeta

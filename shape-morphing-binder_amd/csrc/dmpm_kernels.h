@@ -48,6 +48,16 @@ struct Launch {
     void* gctl;        // dL/d dFc of layer t (output, 9 planes) or null
     const void* ctl1;  // fused step: dFc of layer t+1 or null
     void* grid1;       // fused step: grid t+1 (4 planes, zeroed by the caller)
+    // adjoint: state of layer t+1 when it is known to be the forward step of
+    // `st` under the current control and parameters (its v and C are then
+    // read instead of recomputed by a G2P from `grid`), else null
+    const void* nxt;
+    // particle blocks of the window kernels: block b holds particles
+    // [bst[b], bst[b+1]) (nblk blocks of at most `block` particles, cut where
+    // a block's stencil box would grow too large); null: consecutive blocks of
+    // `block` particles
+    const int* bst;
+    int nblk;
     unsigned long long* wstat;  // window counters: [0] blocks in LDS, [1] blocks falling back (or null)
     int lds_bytes;     // LDS budget per block for the window
     int block;         // threads per block (multiple of 64)
@@ -78,8 +88,9 @@ inline bool op_is_kernel(int kind) { return kind >= OP_P2G; }
 // kernel op `kind` for launch L, written to host memory dst.
 size_t kp_bytes(Prec pr);
 void build_kp(Prec pr, int kind, const Launch& L, void* dst);
-// Thread blocks one instance of n particles takes.
-int instance_blocks(int n, int block);
+// Thread blocks one instance takes (its block table's count, or n / block
+// rounded up without one).
+int instance_blocks(const Launch& L);
 // One launch of kernel op `kind` for `count` instances: tab (device) holds
 // their parameter blocks, bstart (device, count + 1 entries) the prefix sums
 // of their block counts, total = bstart[count].  The instances share the

@@ -52,6 +52,9 @@ struct KP {
     const T* ctl1;  // control of layer t+1 (fused step)
     T* gm1;         // grid t+1 (fused step): m, then 3 p planes
     T* gp1;
+    const T* nxt;   // adjoint: layer t+1 state (v, C read instead of a G2P) or null
+    const int* bst; // particle block table (nblk + 1 starts) or null
+    int nblk;
     unsigned long long* wstat;
     int cap;
 };
@@ -92,6 +95,9 @@ static KP<T> make_kp(const Launch& L, int narrays) {
     k.ctl1 = (const T*)L.ctl1;
     k.gm1 = (T*)L.grid1;
     k.gp1 = L.grid1 ? (T*)L.grid1 + L.gld : nullptr;
+    k.nxt = (const T*)L.nxt;
+    k.bst = L.bst;
+    k.nblk = L.nblk;
     k.wstat = L.wstat;
     k.cap = narrays > 0 ? (int)(L.lds_bytes / narrays) : 0;  // narrays = LDS bytes per window node
     return k;
@@ -130,6 +136,30 @@ static inline unsigned grid_blocks(int n, int block) {
     unsigned nb = (unsigned)((n + block - 1) / block);
     if (nb == 0) nb = 1;
     return (nb + 7u) & ~7u;
+}
+
+// Launch grid of a window kernel: the block table's blocks, or n / block.
+static inline unsigned launch_blocks(const Launch& L) {
+    if (!L.bst) return grid_blocks(L.n, L.block);
+    const unsigned nb = (unsigned)std::max(L.nblk, 1);
+    return (nb + 7u) & ~7u;
+}
+
+// Particles [i0, i1) of logical block `lblock` (block-uniform).  Logical
+// blocks past the end (the grid is padded to a multiple of 8) are empty.
+template <typename T>
+DMPM_DI void block_range(const KP<T>& a, int lblock, int& i0, int& i1) {
+    if (a.bst) {
+        if (lblock < a.nblk) {
+            i0 = a.bst[lblock];
+            i1 = a.bst[lblock + 1];
+        } else {
+            i0 = i1 = a.n;
+        }
+    } else {
+        i0 = lblock * (int)blockDim.x;
+        i1 = min(a.n, i0 + (int)blockDim.x);
+    }
 }
 
 struct Win {
@@ -265,6 +295,27 @@ struct Trust {
     T denom, J0;
 };
 
+// No trust-region scaling (C' used as it is).
+template <typename T>
+DMPM_DI Trust<T> trust_none() {
+    Trust<T> t;
+    t.s = T(1);
+    t.state = 0;
+    t.denom = T(0);
+    t.J0 = T(1);
+    return t;
+}
+
+// The adjoint of step t needs G2P's v' and C' (before the trust region).
+// Layer t+1 (a.nxt) stores v' and the C' after the trust region; that is all
+// the reference's adjoint reads (BackPropagation.cpp:27-31 use layer n+1's C),
+// and all the exact adjoint reads when the trust region is off.  Otherwise
+// (or without a.nxt) they are recomputed by a G2P from grid t.
+template <typename T, bool REF>
+DMPM_DI bool reads_next(const KP<T>& a) {
+    return a.nxt != nullptr && (REF || !a.trust);
+}
+
 template <typename T>
 DMPM_DI Trust<T> trust_region(const KP<T>& a, const m3<T>& Ft, const m3<T>& Craw) {
     Trust<T> t;
@@ -394,6 +445,51 @@ DMPM_DI T seg_sum(T v, const SegRun<T>& r) {
     return v;
 }
 
+// seg_sum of NV values at once.  In FP32 each Hillis-Steele step is a single
+// v_fmac_f32 with the DPP row shift on its first source (the compiler emits a
+// separate v_mov_b32_dpp for every step instead): lane i adds m_d * c of lane
+// i + d to its own c, and lanes whose source lies past their 16-lane row read
+// 0 (bound_ctrl).  DPP reads of a VGPR need 2 wait states after the VALU
+// write of that VGPR, and 5 after a VALU write of EXEC: the block opens with
+// s_nop 4 (instructions before it may have written the values or EXEC), and
+// the values are interleaved so that each is read again 3 (NV = 4) or 2
+// (NV = 3) instructions after it was written.
+template <typename T, int NV>
+DMPM_DI void seg_sums(T* c, const SegRun<T>& r) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) c[q] = seg_sum(c[q], r);
+}
+
+#define DMPM_FMAC_DPP(dst, m, shift) \
+    "v_fmac_f32_dpp " dst ", " dst ", " m " row_shl:" shift " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+
+template <>
+DMPM_DI void seg_sums<float, 4>(float* c, const SegRun<float>& r) {
+    asm volatile("s_nop 4\n\t"
+                 DMPM_FMAC_DPP("%0", "%4", "1") DMPM_FMAC_DPP("%1", "%4", "1")
+                 DMPM_FMAC_DPP("%2", "%4", "1") DMPM_FMAC_DPP("%3", "%4", "1")
+                 DMPM_FMAC_DPP("%0", "%5", "2") DMPM_FMAC_DPP("%1", "%5", "2")
+                 DMPM_FMAC_DPP("%2", "%5", "2") DMPM_FMAC_DPP("%3", "%5", "2")
+                 DMPM_FMAC_DPP("%0", "%6", "4") DMPM_FMAC_DPP("%1", "%6", "4")
+                 DMPM_FMAC_DPP("%2", "%6", "4") DMPM_FMAC_DPP("%3", "%6", "4")
+                 DMPM_FMAC_DPP("%0", "%7", "8") DMPM_FMAC_DPP("%1", "%7", "8")
+                 DMPM_FMAC_DPP("%2", "%7", "8") DMPM_FMAC_DPP("%3", "%7", "8")
+                 : "+v"(c[0]), "+v"(c[1]), "+v"(c[2]), "+v"(c[3])
+                 : "v"(r.m1), "v"(r.m2), "v"(r.m4), "v"(r.m8));
+}
+
+template <>
+DMPM_DI void seg_sums<float, 3>(float* c, const SegRun<float>& r) {
+    asm volatile("s_nop 4\n\t"
+                 DMPM_FMAC_DPP("%0", "%3", "1") DMPM_FMAC_DPP("%1", "%3", "1") DMPM_FMAC_DPP("%2", "%3", "1")
+                 DMPM_FMAC_DPP("%0", "%4", "2") DMPM_FMAC_DPP("%1", "%4", "2") DMPM_FMAC_DPP("%2", "%4", "2")
+                 DMPM_FMAC_DPP("%0", "%5", "4") DMPM_FMAC_DPP("%1", "%5", "4") DMPM_FMAC_DPP("%2", "%5", "4")
+                 DMPM_FMAC_DPP("%0", "%6", "8") DMPM_FMAC_DPP("%1", "%6", "8") DMPM_FMAC_DPP("%2", "%6", "8")
+                 : "+v"(c[0]), "+v"(c[1]), "+v"(c[2])
+                 : "v"(r.m1), "v"(r.m2), "v"(r.m4), "v"(r.m8));
+}
+#undef DMPM_FMAC_DPP
+
 // Window-relative index of stencil base node b (block_window guarantees that
 // every stencil node of a valid lane lies inside the window).
 DMPM_DI int window_base(const Win& w, const int* b) {
@@ -425,8 +521,7 @@ DMPM_DI void scatter_nodes(const KP<T>& a, const Win& win, Acc* sm, const Sten<T
                 for (int ic = 0; ic < KS; ++ic) {
                     T c[NV];
                     val(ia, ib, ic, c);
-#pragma unroll
-                    for (int q = 0; q < NV; ++q) c[q] = seg_sum(c[q], run);
+                    seg_sums<T, NV>(c, run);
                     if (issue) {
 #pragma unroll
                         for (int q = 0; q < NV; ++q) atomicAdd(&sm[q * a.cap + row + ic], (Acc)c[q]);
@@ -443,8 +538,7 @@ DMPM_DI void scatter_nodes(const KP<T>& a, const Win& win, Acc* sm, const Sten<T
                 for (int ic = 0; ic < KS; ++ic) {
                     T c[NV];
                     val(ia, ib, ic, c);
-#pragma unroll
-                    for (int q = 0; q < NV; ++q) c[q] = seg_sum(c[q], run);
+                    seg_sums<T, NV>(c, run);
                     const int gi = S.b[0] + ia, gj = S.b[1] + ib, gk = S.b[2] + ic;
                     if (!issue || !in_grid(a, gi, gj, gk)) continue;
                     const long long g = gindex(a, gi, gj, gk);
@@ -511,8 +605,10 @@ DMPM_DI void p2g_block(const KP<T>& a, int lblock, double* smem_d) {
     constexpr int KS = Spline<ORDER>::K;
     constexpr int NA = MASS_ONLY ? 1 : 4;
     Acc* sm = reinterpret_cast<Acc*>(smem_d);
-    const int i = lblock * blockDim.x + threadIdx.x;
-    const bool valid = i < a.n;
+    int i0, i1;
+    block_range(a, lblock, i0, i1);
+    const int i = i0 + (int)threadIdx.x;
+    const bool valid = i < i1;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
     if (valid) x = load_v3(a.st, a.ld, PX, i);
@@ -674,8 +770,10 @@ template <typename T, int ORDER>
 DMPM_DI void g2p_block(const KP<T>& a, int lblock, double* smem_d) {
     constexpr int KS = Spline<ORDER>::K;
     T* sv = reinterpret_cast<T*>(smem_d);
-    const int i = lblock * blockDim.x + threadIdx.x;
-    const bool valid = i < a.n;
+    int i0, i1;
+    block_range(a, lblock, i0, i1);
+    const int i = i0 + (int)threadIdx.x;
+    const bool valid = i < i1;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
     if (valid) x = load_v3(a.st, a.ld, PX, i);
@@ -716,8 +814,10 @@ DMPM_DI void step_block(const KP<T>& a, int lblock, double* smem_d) {
     constexpr int KS = Spline<ORDER>::K;
     T* sm = reinterpret_cast<T*>(smem_d);     // staged velocities
     Acc* sacc = reinterpret_cast<Acc*>(smem_d);  // then the P2G accumulators
-    const int i = lblock * blockDim.x + threadIdx.x;
-    const bool valid = i < a.n;
+    int i0, i1;
+    block_range(a, lblock, i0, i1);
+    const int i = i0 + (int)threadIdx.x;
+    const bool valid = i < i1;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
     if (valid) x = load_v3(a.st, a.ld, PX, i);
@@ -842,16 +942,19 @@ DMPM_DI void adj_scatter_block(const KP<T>& a, int lblock, double* smem_d) {
     constexpr int KS = Spline<ORDER>::K;
     T* sv = reinterpret_cast<T*>(smem_d);  // 3 velocity arrays, then 3 accumulators
     Acc* sg = reinterpret_cast<Acc*>(smem_d + (3 * a.cap * sizeof(T) + 7) / 8);
-    const int i = lblock * blockDim.x + threadIdx.x;
-    const bool valid = i < a.n;
+    int i0, i1;
+    block_range(a, lblock, i0, i1);
+    const int i = i0 + (int)threadIdx.x;
+    const bool valid = i < i1;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
     if (valid) x = load_v3(a.st, a.ld, PX, i);
     S.init(a, x);  // lanes without a particle get a finite stencil at x = 0
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
     count_window(a, win);
+    const bool nxt = reads_next<T, REF>(a);
     if (win.lds) {
-        stage_velocity(a, win, sv);
+        if (!nxt) stage_velocity(a, win, sv);
         for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
             sg[t] = Acc(0);
             sg[a.cap + t] = Acc(0);
@@ -862,12 +965,18 @@ DMPM_DI void adj_scatter_block(const KP<T>& a, int lblock, double* smem_d) {
     v3<T> gvtot = vzero<T>();
     m3<T> Gc = mzero<T>();
     if (valid) {
-        v3<T> vn;
-        m3<T> Craw;
-        g2p_gather<T, ORDER>(a, win, sv, S, vn, Craw);
         m3<T> Ft = load_m3(a.st, a.ld, PF, i);
         if (a.ctl) Ft = madd(Ft, load_m3(a.ctl, a.ld, 0, i));
-        const Trust<T> tr = trust_region(a, Ft, Craw);
+        m3<T> Craw;
+        Trust<T> tr;
+        if (nxt) {
+            Craw = load_m3(a.nxt, a.ld, PC, i);
+            tr = trust_none<T>();
+        } else {
+            v3<T> vn;
+            g2p_gather<T, ORDER>(a, win, sv, S, vn, Craw);
+            tr = trust_region(a, Ft, Craw);
+        }
         const Phase1<T, REF> p1 = phase1<T, REF>(a, i, Ft, Craw, tr);
         gvtot = p1.gvtot;
         Gc = mscale(Spline<ORDER>::dinv(a.dx), p1.gCraw);
@@ -944,8 +1053,10 @@ template <typename T, int ORDER, int MODEL, bool REF>
 DMPM_DI void adj_gather_block(const KP<T>& a, int lblock, double* smem_d) {
     constexpr int KS = Spline<ORDER>::K;
     T* sv = reinterpret_cast<T*>(smem_d);  // v (3 arrays), h (3), gm (1)
-    const int i = lblock * blockDim.x + threadIdx.x;
-    const bool valid = i < a.n;
+    int i0, i1;
+    block_range(a, lblock, i0, i1);
+    const int i = i0 + (int)threadIdx.x;
+    const bool valid = i < i1;
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
     if (valid) x = load_v3(a.st, a.ld, PX, i);
@@ -967,12 +1078,19 @@ DMPM_DI void adj_gather_block(const KP<T>& a, int lblock, double* smem_d) {
         __syncthreads();
     }
     if (!valid) return;
-    v3<T> vn;
-    m3<T> Craw;
-    g2p_gather<T, ORDER>(a, win, sv, S, vn, Craw);
     const m3<T> F = load_m3(a.st, a.ld, PF, i);
     const m3<T> Ft = a.ctl ? madd(F, load_m3(a.ctl, a.ld, 0, i)) : F;
-    const Trust<T> tr = trust_region(a, Ft, Craw);
+    v3<T> vn;
+    m3<T> Craw;
+    Trust<T> tr;
+    if (reads_next<T, REF>(a)) {
+        vn = load_v3(a.nxt, a.ld, PV, i);
+        Craw = load_m3(a.nxt, a.ld, PC, i);
+        tr = trust_none<T>();
+    } else {
+        g2p_gather<T, ORDER>(a, win, sv, S, vn, Craw);
+        tr = trust_region(a, Ft, Craw);
+    }
     const Phase1<T, REF> p1 = phase1<T, REF>(a, i, Ft, Craw, tr);
 
     const T m = a.stat[SM * a.ld + i], vol = a.stat[SVOL * a.ld + i];
@@ -1587,7 +1705,7 @@ template <typename T>
 static void launch_p2g(const Launch& L, bool mass_only, hipStream_t s) {
     const int na = mass_only ? 1 : 4;
     KP<T> k = make_kp<T>(L, na * (int)sizeof(Acc));
-    const unsigned nb = grid_blocks(L.n, L.block);
+    const unsigned nb = launch_blocks(L);
     const size_t sh = (size_t)k.cap * na * sizeof(Acc);
 #define P2G_CASE(O, M, MO) hipLaunchKernelGGL((k_p2g<T, O, M, MO>), dim3(nb), dim3(L.block), sh, s, k)
     const int o = L.ph.order, md = L.ph.model;
@@ -1604,7 +1722,7 @@ static void launch_p2g(const Launch& L, bool mass_only, hipStream_t s) {
 template <typename T>
 static void launch_g2p(const Launch& L, hipStream_t s) {
     KP<T> k = make_kp<T>(L, 3 * (int)sizeof(T));
-    const unsigned nb = grid_blocks(L.n, L.block);
+    const unsigned nb = launch_blocks(L);
     const size_t sh = (size_t)k.cap * 3 * sizeof(T);
     if (L.ph.order == 3) hipLaunchKernelGGL((k_g2p<T, 3>), dim3(nb), dim3(L.block), sh, s, k);
     else hipLaunchKernelGGL((k_g2p<T, 2>), dim3(nb), dim3(L.block), sh, s, k);
@@ -1614,7 +1732,7 @@ template <typename T>
 static void launch_step(const Launch& L, hipStream_t s) {
     // staging needs 3 T per node, the P2G accumulators 4 Acc per node
     KP<T> k = make_kp<T>(L, 4 * (int)sizeof(Acc));
-    const unsigned nb = grid_blocks(L.n, L.block);
+    const unsigned nb = launch_blocks(L);
     const size_t sh = (size_t)k.cap * 4 * sizeof(Acc);
     const int o = L.ph.order, md = L.ph.model;
 #define ST_CASE(O, M) hipLaunchKernelGGL((k_step<T, O, M>), dim3(nb), dim3(L.block), sh, s, k)
@@ -1626,7 +1744,7 @@ static void launch_step(const Launch& L, hipStream_t s) {
 template <typename T>
 static void launch_adj_scatter(const Launch& L, hipStream_t s) {
     KP<T> k = make_kp<T>(L, 3 * (int)sizeof(T) + 3 * (int)sizeof(Acc) + 8);
-    const unsigned nb = grid_blocks(L.n, L.block);
+    const unsigned nb = launch_blocks(L);
     const size_t sh = (size_t)k.cap * (3 * sizeof(T) + 3 * sizeof(Acc)) + 8;
     const bool ref = L.ph.adjoint == 1;
 #define AS_CASE(O, R) hipLaunchKernelGGL((k_adj_scatter<T, O, R>), dim3(nb), dim3(L.block), sh, s, k)
@@ -1638,7 +1756,7 @@ static void launch_adj_scatter(const Launch& L, hipStream_t s) {
 template <typename T>
 static void launch_adj_gather(const Launch& L, hipStream_t s) {
     KP<T> k = make_kp<T>(L, 7 * (int)sizeof(T));
-    const unsigned nb = grid_blocks(L.n, L.block);
+    const unsigned nb = launch_blocks(L);
     const size_t sh = (size_t)k.cap * 7 * sizeof(T);
     const bool ref = L.ph.adjoint == 1;
     const int o = L.ph.order, md = L.ph.model;
@@ -1686,7 +1804,10 @@ void build_kp(Prec pr, int kind, const Launch& L, void* dst) {
     }
 }
 
-int instance_blocks(int n, int block) { return n > 0 ? (n + block - 1) / block : 0; }
+int instance_blocks(const Launch& L) {
+    if (L.bst) return L.nblk;
+    return L.n > 0 ? (L.n + L.block - 1) / L.block : 0;
+}
 
 template <typename T>
 static void launch_batch_t(int kind, const Phys& ph, int lds_bytes, int block, const void* tab_, const int* bstart,

@@ -84,6 +84,8 @@ struct dmpm_sim {
     std::vector<std::pair<void*, size_t>> allocs;
 
     int* perm = nullptr;   // internal index -> caller index
+    int* bst = nullptr;    // particle block table of the window kernels (nblk + 1 starts)
+    int nblk = 0;
     void* stat = nullptr;  // m, vol, lam, mu
     std::vector<void*> layer;  // stored layer states (nullptr = not stored)
     std::vector<char> valid;   // layer computed (stored ones only)
@@ -100,13 +102,24 @@ struct dmpm_sim {
     // gvalid[t] whether it matches layer t's state and control.
     std::vector<void*> lgrid;
     std::vector<char> gvalid;
+    // fresh[t]: layer t's state (stored, or in the current segment) is the
+    // forward step of layer t-1 under the current control of layer t-1 and
+    // the current parameters.  The adjoint of step t-1 then reads v' and C'
+    // from it instead of recomputing them by a G2P.
+    std::vector<char> fresh;
     bool keep_grids() const { return K <= 1 && !cfg.recompute_grids; }
     // A control (or a parameter the P2G reads) changed: layer t's grid (all
-    // grids for t < 0) and any recomputed segment no longer hold.
+    // grids for t < 0), any recomputed segment and the freshness of layer
+    // t+1 (of every layer for t < 0) no longer hold.
     void grids_stale(int t) {
         seg_base = -1;
-        if (t < 0) std::fill(gvalid.begin(), gvalid.end(), 0);
-        else if (t < (int)gvalid.size()) gvalid[t] = 0;
+        if (t < 0) {
+            std::fill(gvalid.begin(), gvalid.end(), 0);
+            std::fill(fresh.begin(), fresh.end(), 0);
+        } else {
+            if (t < (int)gvalid.size()) gvalid[t] = 0;
+            if (t + 1 < (int)fresh.size()) fresh[t + 1] = 0;
+        }
     }
     void* scratch[2] = {nullptr, nullptr};
     void* grid = nullptr;  // m, p (4 planes)
@@ -195,6 +208,8 @@ struct dmpm_sim {
         Lc.ld = ld;
         Lc.gld = gld;
         Lc.wstat = wstat;
+        Lc.bst = bst;
+        Lc.nblk = nblk;
         Lc.lds_bytes = cfg.lds_bytes > 0 ? cfg.lds_bytes : 48 * 1024;
         Lc.block = cfg.block > 0 ? cfg.block : 256;
         return Lc;
@@ -287,6 +302,7 @@ struct dmpm_sim {
         layer.resize(L, nullptr);
         valid.assign(L, 0);
         valid[0] = 1;
+        fresh.assign(L, 0);
         for (int t = 1; t < L; ++t)
             if (stored(t)) layer[t] = dalloc(state_bytes());
         if (K > 1 && L > 1) {
@@ -316,7 +332,10 @@ struct dmpm_sim {
         seg_base = -1;
         seg[0] = layer[c];
         p2g_into(seg[0], c, seg_grid[0]);
-        for (int j = c; j < top; ++j) fused_step(seg[j - c], j, seg_grid[j - c], seg[j - c + 1], seg_grid[j - c + 1]);
+        for (int j = c; j < top; ++j) {
+            fused_step(seg[j - c], j, seg_grid[j - c], seg[j - c + 1], seg_grid[j - c + 1]);
+            fresh[j + 1] = 1;
+        }
         seg_base = c;
     }
 
@@ -421,6 +440,7 @@ struct dmpm_sim {
                     g2p_only(cur, t, ga, out);
                 }
                 if (stored(t + 1)) valid[t + 1] = 1;
+                fresh[t + 1] = 1;
                 cur = out;
             }
             if (!capturing && !rec) HIPCHK(hipEventRecord(ev[1], stream));
@@ -448,10 +468,17 @@ struct dmpm_sim {
         for (int t = L - 2; t >= stop; --t) {
             const void* st;
             void* g;
+            // layer t+1 when it is the forward step of layer t as it stands
+            // now (recomputed segment layers always are)
+            const void* nxt = nullptr;
             if (K > 1) {
                 ensure_segment(t);
                 st = t == seg_base ? layer[t] : seg[t - seg_base];
                 g = seg_grid[t - seg_base];
+                if (fresh[t + 1]) {
+                    if (!stored(t + 1)) nxt = seg[t + 1 - seg_base];
+                    else if (valid[t + 1]) nxt = layer[t + 1];
+                }
             } else if (keep_grids()) {
                 st = state(t);
                 g = lgrid[t];
@@ -464,12 +491,14 @@ struct dmpm_sim {
                 g = grid;
                 p2g_into(st, t, g);
             }
+            if (K <= 1 && fresh[t + 1] && valid[t + 1]) nxt = layer[t + 1];
             Launch Lc = launch();
             Lc.st = st;
             Lc.stat = stat;
             Lc.ctl = ctl[t];
             Lc.grid = g;
             Lc.ggv = ggv;
+            Lc.nxt = cfg.adjoint_recompute ? nullptr : nxt;
             Lc.adj1 = carry[carry_cur];
             Lc.adj0 = carry[carry_cur ^ 1];
             void* gout = nullptr;
@@ -831,6 +860,8 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
         s->seg_grid.clear();
         s->dfree(s->stat);
         s->dfree((void*&)s->perm);
+        s->dfree((void*&)s->bst);
+        s->nblk = 0;
         for (auto& p : s->carry) s->dfree(p);
         s->dfree(s->adj_last);
         s->dfree(s->gctl_stop);
@@ -843,6 +874,7 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
         s->L = 1;
         s->layer.assign(1, nullptr);
         s->valid.assign(1, 1);
+        s->fresh.assign(1, 0);
         s->ctl.assign(1, nullptr);
         s->am.assign(1, nullptr);
         s->av.assign(1, nullptr);
@@ -851,6 +883,50 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
         s->want_gctl.assign(1, 0);
         s->perm = (int*)s->dalloc(n * sizeof(int));
         HIPCHK(hipMemcpyAsync(s->perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        // Thread blocks of the window kernels: runs of up to `block`
+        // consecutive particles (in Hilbert order), cut where the box of the
+        // run's stencils would exceed split_nodes nodes.  The curve is
+        // continuous, but where it leaves the body and re-enters it elsewhere
+        // consecutive particles lie far apart; a block spanning such a jump
+        // would have a window too large for LDS and fall back to global
+        // atomics (and run several times longer than the others).
+        const long long split = s->cfg.split_nodes == 0 ? 512 : s->cfg.split_nodes;
+        if (split > 0) {
+            const int B = s->cfg.block > 0 ? s->cfg.block : 256;
+            const long long KS = s->cfg.order == 2 ? 3 : 4;
+            std::vector<int> cell((size_t)n * 3);
+            for (int j = 0; j < n; ++j)
+                for (int a = 0; a < 3; ++a) {
+                    const double f =
+                        (host_get(xh.data(), dtype, 3 * (size_t)perm[j] + a) - s->cfg.grid_min[a]) / s->cfg.dx;
+                    cell[3 * (size_t)j + a] = (int)std::max(-1e9, std::min(1e9, std::floor(f - shift)));
+                }
+            std::vector<int> starts{0};
+            for (int i = 0; i < n;) {
+                int lo[3], hi[3];
+                for (int a = 0; a < 3; ++a) lo[a] = hi[a] = cell[3 * (size_t)i + a];
+                int j = i + 1;
+                for (; j < n && j - i < B; ++j) {
+                    int l2[3], h2[3];
+                    long long vol = 1;
+                    for (int a = 0; a < 3; ++a) {
+                        l2[a] = std::min(lo[a], cell[3 * (size_t)j + a]);
+                        h2[a] = std::max(hi[a], cell[3 * (size_t)j + a]);
+                        vol *= (long long)h2[a] - l2[a] + KS;
+                    }
+                    if (vol > split) break;
+                    for (int a = 0; a < 3; ++a) {
+                        lo[a] = l2[a];
+                        hi[a] = h2[a];
+                    }
+                }
+                starts.push_back(j);
+                i = j;
+            }
+            s->nblk = (int)starts.size() - 1;
+            s->bst = (int*)s->dalloc(starts.size() * sizeof(int));
+            HIPCHK(hipMemcpy(s->bst, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
         s->layer[0] = s->dalloc(s->state_bytes());
         s->stat = s->dalloc((size_t)NSTAT * s->ld * s->esz);
         for (auto& p : s->carry) p = s->dalloc(s->state_bytes());
@@ -1715,7 +1791,7 @@ int dmpm_batch_capture(dmpm_sim** sims, int count, int start_layer, int stop_lay
                 for (int i = 0; i < count; ++i) {
                     const Launch& Li = ops[i][k].L;
                     build_kp(pr, kind, Li, host.data() + items[k].off + kpb * i);
-                    bs[i + 1] = bs[i] + instance_blocks(Li.n, Li.block);
+                    bs[i + 1] = bs[i] + instance_blocks(Li);
                 }
                 items[k].total = bs[count];
             } else {

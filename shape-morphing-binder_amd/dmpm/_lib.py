@@ -50,6 +50,8 @@ class Config(C.Structure):
         ("unfused", C.c_int),
         ("deterministic", C.c_int),
         ("recompute_grids", C.c_int),
+        ("adjoint_recompute", C.c_int),
+        ("split_nodes", C.c_int),
     ]
 
 

@@ -47,7 +47,8 @@ class Physics:
 
 def make_config(dims, dx, grid_min=(0.0, 0.0, 0.0), physics: Optional[Physics] = None, precision: str = "f32",
                 checkpoint_interval: int = 1, lds_bytes: int = 0, block: int = 0, unfused: bool = False,
-                deterministic: bool = False, recompute_grids: bool = False) -> L.Config:
+                deterministic: bool = False, recompute_grids: bool = False,
+                adjoint_recompute: bool = False, split_nodes: int = 0) -> L.Config:
     ph = physics or Physics()
     c = L.Config()
     gm = (C.c_double * 3)(*[float(a) for a in grid_min])
@@ -63,6 +64,8 @@ def make_config(dims, dx, grid_min=(0.0, 0.0, 0.0), physics: Optional[Physics] =
     c.unfused = 1 if unfused else 0
     c.deterministic = 1 if deterministic else 0
     c.recompute_grids = 1 if recompute_grids else 0
+    c.adjoint_recompute = 1 if adjoint_recompute else 0
+    c.split_nodes = int(split_nodes)
     return c
 
 
@@ -124,14 +127,15 @@ class Simulator:
 
     def __init__(self, dims, dx, grid_min=(0.0, 0.0, 0.0), physics: Optional[Physics] = None,
                  precision: str = "f32", checkpoint_interval: int = 1, lds_bytes: int = 0, block: int = 0,
-                 unfused: bool = False, deterministic: bool = False, recompute_grids: bool = False):
+                 unfused: bool = False, deterministic: bool = False, recompute_grids: bool = False,
+                 adjoint_recompute: bool = False, split_nodes: int = 0):
         self.dims = tuple(int(d) for d in dims)
         self.dx = float(dx)
         self.grid_min = tuple(float(a) for a in grid_min)
         self.physics = physics or Physics()
         self.precision = "f64" if precision in ("f64", "float64", "fp64") else "f32"
         self.cfg = make_config(dims, dx, grid_min, self.physics, self.precision, checkpoint_interval, lds_bytes,
-                               block, unfused, deterministic, recompute_grids)
+                               block, unfused, deterministic, recompute_grids, adjoint_recompute, split_nodes)
         h = C.c_void_p()
         L.check(L.lib().dmpm_create(C.byref(self.cfg), C.byref(h)))
         self._h = h

@@ -119,3 +119,79 @@ def test_rollout_checkpointing(K, recompute):
         for t in live:
             assert rel_err(sim.get_control_grad(t, dtype=np.float64), gctl[t]) < 1e-9, (case, t)
     sim.close()
+
+
+@pytest.mark.parametrize("K", [1, 3])
+@pytest.mark.parametrize("name", ["plain", "reference", "reference_exact", "quadratic_fcr"])
+def test_adjoint_reads_next_layer(name, K):
+    """The adjoint of step t reads G2P's v' and C' from layer t+1 (the
+    reference reads layer n+1's C) instead of recomputing them, while layer
+    t+1 is the forward step of layer t as it stands.  Gradients must match
+    the always-recompute form, also after a control changes without a new
+    forward pass (layer t+1 is then stale and must not be read)."""
+    from dmpm import Physics, Simulator
+    sc, s = perturbed_scene(dfc=0.0)
+    T = 6
+    rng = np.random.default_rng(21)
+    adj = random_adjoint(sc.n, seed=4)
+    ctl2 = 0.01 * rng.normal(size=(sc.n, 3, 3))
+    print(f"[adjoint_reads_next_layer] physics={name} K={K} T={T} n={sc.n}", flush=True)
+    grads = []
+    for recompute in (False, True):
+        sim = Simulator(sc.dims, sc.dx, sc.grid_min, physics=Physics(**physics_kwargs(name, DT)), precision="f64",
+                        checkpoint_interval=K, adjoint_recompute=recompute)
+        sim.set_state(dict(s, dFc=None))
+        sim.set_up(T + 1)
+        sim.request_control_grad(2)
+        sim.forward(0, keep_controls=True)
+        sim.set_last_grad(x=adj["x"], v=adj["v"], F=adj["F"], C_=adj["C"])
+        sim.backward(0)
+        g = {k: sim.get_grad(0, k, dtype=np.float64) for k in ("x", "v", "F", "C")}
+        g["ctl"] = sim.get_control_grad(2, dtype=np.float64)
+        # a control at layer 2 changes; no forward pass: layer 3 is stale
+        sim.set_control(2, ctl2)
+        sim.backward(0)
+        g2 = {k: sim.get_grad(0, k, dtype=np.float64) for k in ("x", "v", "F", "C")}
+        g2["ctl"] = sim.get_control_grad(2, dtype=np.float64)
+        grads.append((g, g2))
+        sim.close()
+    for case in (0, 1):
+        for k in ("x", "v", "F", "C", "ctl"):
+            e = rel_err(grads[0][case][k], grads[1][case][k])
+            assert e < 1e-11, f"case {case}: d{k}: {e:.3e}"
+    # the stale case differs from the fresh one (the control did act)
+    assert rel_err(grads[0][1]["v"], grads[0][0]["v"]) > 1e-8
+
+
+@pytest.mark.parametrize("name", ["plain", "reference"])
+def test_block_table_cuts(name):
+    """Thread blocks cut where their stencil box would exceed split_nodes:
+    with a tiny limit most blocks are cut short (more, partly empty blocks);
+    states and gradients of a 4-step episode must still match the oracle
+    within the FP64 bound, and the same as without any cuts."""
+    from dmpm import Physics, Simulator
+    sc, s = perturbed_scene(dfc=0.0)
+    T = 4
+    adj = random_adjoint(sc.n, seed=6)
+    orc = Oracle("f64")
+    prm = oracle_params(name, sc, DT)
+    layers = orc.rollout(prm, dict(s, dFc=None), T)
+    g0, _ = orc.rollout_grad(prm, layers, adj)
+    windows = {}
+    for split in (40, -1):
+        print(f"[block_table_cuts] physics={name} split_nodes={split} n={sc.n}", flush=True)
+        sim = Simulator(sc.dims, sc.dx, sc.grid_min, physics=Physics(**physics_kwargs(name, DT)), precision="f64",
+                        split_nodes=split)
+        sim.set_state(dict(s, dFc=None))
+        sim.set_up(T + 1)
+        sim.window_stats(reset=True)
+        sim.forward(0)
+        sim.set_last_grad(x=adj["x"], v=adj["v"], F=adj["F"], C_=adj["C"])
+        sim.backward(0)
+        windows[split] = sum(sim.window_stats())
+        for k in ("x", "v", "F", "C"):
+            assert rel_err(sim.get(T, k, dtype=np.float64), layers[T][k]) < 1e-10, (split, k)
+            e = rel_err(sim.get_grad(0, k, dtype=np.float64), g0[k])
+            assert e < 1e-10, f"split {split}: d{k}: {e:.3e}"
+        sim.close()
+    assert windows[40] > 2 * windows[-1], windows
