@@ -162,27 +162,44 @@ DMPM_DI void block_range(const KP<T>& a, int lblock, int& i0, int& i1) {
     }
 }
 
+// A block's window: the box [lo, lo + e) of grid nodes its particles'
+// stencils touch, laid out in LDS with node (i, j, k) of the box at
+// i*s1 + j*s2 + k.  The strides are padded to odd numbers of nodes, so the
+// same stencil offset of particles in neighbouring cells (what one LDS
+// instruction of a wave addresses) spreads over the banks instead of
+// landing on one every few rows; vol counts the padded slots.
 struct Win {
-    int lo0, lo1, lo2, e1, e2, vol;
+    int lo0, lo1, lo2, e1, e2, s1, s2, vol;
     bool lds;
     bool any;
 };
 
-// Bounding box of the block's stencils (base nodes b, width KS).
+// Wave-wide min (MAX = false) or max of v, through DPP: quad swaps, row
+// mirrors, then row broadcasts 15 and 31 fold the four 16-lane rows into
+// lane 63 (no LDS traffic, unlike a shuffle).
+template <bool MAX>
+DMPM_DI int wave_extreme(int v) {
+    auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));  // row_bcast:15 into rows 1, 3
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));  // row_bcast:31 into rows 2, 3
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Bounding box of the block's stencils (base nodes b, width KS).  Every
+// thread of the block must call this (all lanes of each wave active).
 template <int KS>
 DMPM_DI Win block_window(bool valid, int b0, int b1, int b2, int cap) {
     __shared__ int s_red[6][16];
-    int v0 = valid ? b0 : INT_MAX, v1 = valid ? b1 : INT_MAX, v2 = valid ? b2 : INT_MAX;
-    int w0 = valid ? b0 : INT_MIN, w1 = valid ? b1 : INT_MIN, w2 = valid ? b2 : INT_MIN;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        v0 = min(v0, __shfl_xor(v0, off));
-        v1 = min(v1, __shfl_xor(v1, off));
-        v2 = min(v2, __shfl_xor(v2, off));
-        w0 = max(w0, __shfl_xor(w0, off));
-        w1 = max(w1, __shfl_xor(w1, off));
-        w2 = max(w2, __shfl_xor(w2, off));
-    }
+    const int v0 = wave_extreme<false>(valid ? b0 : INT_MAX);
+    const int v1 = wave_extreme<false>(valid ? b1 : INT_MAX);
+    const int v2 = wave_extreme<false>(valid ? b2 : INT_MAX);
+    const int w0 = wave_extreme<true>(valid ? b0 : INT_MIN);
+    const int w1 = wave_extreme<true>(valid ? b1 : INT_MIN);
+    const int w2 = wave_extreme<true>(valid ? b2 : INT_MIN);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) {
         s_red[0][wave] = v0; s_red[1][wave] = v1; s_red[2][wave] = v2;
@@ -190,26 +207,41 @@ DMPM_DI Win block_window(bool valid, int b0, int b1, int b2, int cap) {
     }
     __syncthreads();
     const int nw = blockDim.x >> 6;
-    v0 = s_red[0][0]; v1 = s_red[1][0]; v2 = s_red[2][0];
-    w0 = s_red[3][0]; w1 = s_red[4][0]; w2 = s_red[5][0];
+    int l0 = s_red[0][0], l1 = s_red[1][0], l2 = s_red[2][0];
+    int h0 = s_red[3][0], h1 = s_red[4][0], h2 = s_red[5][0];
     for (int q = 1; q < nw; ++q) {
-        v0 = min(v0, s_red[0][q]); v1 = min(v1, s_red[1][q]); v2 = min(v2, s_red[2][q]);
-        w0 = max(w0, s_red[3][q]); w1 = max(w1, s_red[4][q]); w2 = max(w2, s_red[5][q]);
+        l0 = min(l0, s_red[0][q]); l1 = min(l1, s_red[1][q]); l2 = min(l2, s_red[2][q]);
+        h0 = max(h0, s_red[3][q]); h1 = max(h1, s_red[4][q]); h2 = max(h2, s_red[5][q]);
     }
     Win w;
-    w.any = v0 != INT_MAX;
-    w.lo0 = v0; w.lo1 = v1; w.lo2 = v2;
+    w.any = l0 != INT_MAX;
+    w.lo0 = l0; w.lo1 = l1; w.lo2 = l2;
     if (!w.any) {
-        w.e1 = w.e2 = 0; w.vol = 0; w.lds = true;
+        w.e1 = w.e2 = w.s1 = w.s2 = 0; w.vol = 0; w.lds = true;
         return w;
     }
-    long long e0 = (long long)w0 - v0 + KS, e1 = (long long)w1 - v1 + KS, e2 = (long long)w2 - v2 + KS;
-    long long vol = e0 * e1 * e2;
+    const long long e0 = (long long)h0 - l0 + KS, e1 = (long long)h1 - l1 + KS, e2 = (long long)h2 - l2 + KS;
+    // (extents are checked one by one first: base cells of far-away or
+    // non-finite positions must not overflow the product)
+    const long long s2 = e2 | 1, s1 = (e1 * s2) | 1;
+    const long long vol = e0 <= cap && e1 <= cap && e2 <= cap ? e0 * s1 : LLONG_MAX;
+    w.lds = vol <= (long long)cap;
     w.e1 = (int)e1;
     w.e2 = (int)e2;
-    w.lds = vol <= (long long)cap;
+    w.s1 = w.lds ? (int)s1 : 0;
+    w.s2 = w.lds ? (int)s2 : 0;
     w.vol = w.lds ? (int)vol : 0;
     return w;
+}
+
+// Grid node of window slot t; false for the padding slots of the strides.
+DMPM_DI bool window_node(const Win& w, int t, int& gi, int& gj, int& gk) {
+    const int i = t / w.s1, r = t - i * w.s1;
+    const int j = r / w.s2, k = r - j * w.s2;
+    gi = w.lo0 + i;
+    gj = w.lo1 + j;
+    gk = w.lo2 + k;
+    return j < w.e1 && k < w.e2;
 }
 
 // Count a scatter window as held in LDS or fallen back to global atomics.
@@ -237,12 +269,14 @@ DMPM_DI long long gindex(const KP<T>& a, int i, int j, int k) {
 // Grid velocity update (ForwardSimulation.cpp:82-87) plus sticky walls.
 template <typename T>
 DMPM_DI void node_velocity(const KP<T>& a, long long g, int i, int j, int k, T& v0, T& v1, T& v2) {
+    // all four loads issued before the branch (one memory round trip)
     const T m = a.gm[g];
+    const T p0 = a.gp[g], p1 = a.gp[a.gld + g], p2 = a.gp[2 * a.gld + g];
     if (m > T(1e-12) && !is_wall(a, i, j, k)) {
         const T inv = T(1) / m;
-        v0 = a.gp[g] * inv + a.dt * a.f0;
-        v1 = a.gp[a.gld + g] * inv + a.dt * a.f1;
-        v2 = a.gp[2 * a.gld + g] * inv + a.dt * a.f2;
+        v0 = p0 * inv + a.dt * a.f0;
+        v1 = p1 * inv + a.dt * a.f1;
+        v2 = p2 * inv + a.dt * a.f2;
     } else {
         v0 = v1 = v2 = T(0);
     }
@@ -344,10 +378,8 @@ DMPM_DI Trust<T> trust_region(const KP<T>& a, const m3<T>& Ft, const m3<T>& Craw
 // Affine momentum matrix A_p = -D^{-1} dt vol (P Ft^T + zeta tr(C) I) + m C
 // and the drag-scaled momentum m (1 - dt drag) v of one particle.
 template <typename T, int ORDER, int MODEL>
-DMPM_DI void p2g_terms(const KP<T>& a, int i, const v3<T>& v, const m3<T>& Ft, const m3<T>& C, T m, m3<T>& A,
-                       v3<T>& mv) {
-    const T vol = a.stat[SVOL * a.ld + i];
-    const T lam = a.stat[SLAM * a.ld + i], mu = a.stat[SMU * a.ld + i];
+DMPM_DI void p2g_terms(const KP<T>& a, T vol, T lam, T mu, const v3<T>& v, const m3<T>& Ft, const m3<T>& C, T m,
+                       m3<T>& A, v3<T>& mv) {
     const m3<T> P = Model<MODEL>::pk1(Ft, lam, mu);
     m3<T> str = mmult(P, Ft);
     const T tau = a.zeta * mtrace(C);
@@ -357,6 +389,13 @@ DMPM_DI void p2g_terms(const KP<T>& a, int i, const v3<T>& v, const m3<T>& Ft, c
     for (int q = 0; q < 9; ++q) A.a[q] = k0 * str.a[q] + m * C.a[q];
     const T dragf = T(1) - a.dt * a.drag;
     mv = {{m * v.a[0] * dragf, m * v.a[1] * dragf, m * v.a[2] * dragf}};
+}
+
+template <typename T, int ORDER, int MODEL>
+DMPM_DI void p2g_terms(const KP<T>& a, int i, const v3<T>& v, const m3<T>& Ft, const m3<T>& C, T m, m3<T>& A,
+                       v3<T>& mv) {
+    p2g_terms<T, ORDER, MODEL>(a, a.stat[SVOL * a.ld + i], a.stat[SLAM * a.ld + i], a.stat[SMU * a.ld + i], v, Ft,
+                               C, m, A, mv);
 }
 
 // Type of the LDS scatter accumulators.  ds_add_f64 sustains about twice
@@ -493,7 +532,7 @@ DMPM_DI void seg_sums<float, 3>(float* c, const SegRun<float>& r) {
 // Window-relative index of stencil base node b (block_window guarantees that
 // every stencil node of a valid lane lies inside the window).
 DMPM_DI int window_base(const Win& w, const int* b) {
-    return ((b[0] - w.lo0) * w.e1 + (b[1] - w.lo1)) * w.e2 + (b[2] - w.lo2);
+    return (b[0] - w.lo0) * w.s1 + (b[1] - w.lo1) * w.s2 + (b[2] - w.lo2);
 }
 
 // Scatter NV values per stencil node: into the block's LDS window (NV Acc
@@ -510,7 +549,7 @@ DMPM_DI void scatter_nodes(const KP<T>& a, const Win& win, Acc* sm, const Sten<T
     const SegRun<T> run = seg_run<T>(valid, S.b);
     const bool issue = valid && run.head;
     if (win.lds) {
-        const int e2 = win.e2, e12 = win.e1 * win.e2;
+        const int e2 = win.s2, e12 = win.s1;
         const int lb = valid ? window_base(win, S.b) : 0;
 #pragma unroll
         for (int ia = 0; ia < KS; ++ia) {
@@ -579,10 +618,9 @@ DMPM_DI void window_zero(const KP<T>& a, const Win& win, U* sm) {
 // Flush the window to the grid: one atomic per touched node value.
 template <typename T, bool MASS_ONLY>
 DMPM_DI void p2g_flush(const KP<T>& a, const Win& win, const Acc* sm, T* gm, T* gp) {
-    const int e12 = win.e1 * win.e2;
     for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
-        const int gi = win.lo0 + t / e12, gj = win.lo1 + (t / win.e2) % win.e1, gk = win.lo2 + t % win.e2;
-        if (!in_grid(a, gi, gj, gk)) continue;
+        int gi, gj, gk;
+        if (!window_node(win, t, gi, gj, gk) || !in_grid(a, gi, gj, gk)) continue;
         const T vm = (T)sm[t];
         const long long g = gindex(a, gi, gj, gk);
         if (MASS_ONLY) {
@@ -653,11 +691,12 @@ __global__ void __launch_bounds__(256) k_p2g_batch(const KP<T>* tab, const int* 
 // Stage the window's grid velocities in LDS (3 arrays of cap).
 template <typename T>
 DMPM_DI void stage_velocity(const KP<T>& a, const Win& win, T* sv) {
-    const int e12 = win.e1 * win.e2;
+#pragma unroll 2
     for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
-        const int gi = win.lo0 + t / e12, gj = win.lo1 + (t / win.e2) % win.e1, gk = win.lo2 + t % win.e2;
+        int gi, gj, gk;
+        const bool real = window_node(win, t, gi, gj, gk);
         T v0 = T(0), v1 = T(0), v2 = T(0);
-        if (in_grid(a, gi, gj, gk)) node_velocity(a, gindex(a, gi, gj, gk), gi, gj, gk, v0, v1, v2);
+        if (real && in_grid(a, gi, gj, gk)) node_velocity(a, gindex(a, gi, gj, gk), gi, gj, gk, v0, v1, v2);
         sv[t] = v0;
         sv[a.cap + t] = v1;
         sv[2 * a.cap + t] = v2;
@@ -689,7 +728,7 @@ DMPM_DI void g2p_gather(const KP<T>& a, const Win& win, const T* sv, const Sten<
         // S = sum_y w1 s, U = sum_y w1 d1 s, T = sum_y w1 t, and finally
         // v' = sum_x w0 S, C' columns sum_x w0 d0 S, sum_x w0 U, sum_x w0 T
         // (about 2/3 of the per-node arithmetic).
-        const int e2 = win.e2, e12 = win.e1 * win.e2;
+        const int e2 = win.s2, e12 = win.s1;
         const int lb = window_base(win, S.b);
         T w2d[KS];
 #pragma unroll
@@ -747,10 +786,8 @@ DMPM_DI void g2p_gather(const KP<T>& a, const Win& win, const T* sv, const Sten<
 // P_op_2 (ForwardSimulation.cpp:131-157): trust region on C', F update with
 // smoothing, x update.  Writes layer t+1 and returns its x, v, F, C.
 template <typename T>
-DMPM_DI void particle_update(const KP<T>& a, int i, const v3<T>& x, const v3<T>& vn, const m3<T>& Craw,
-                             v3<T>& xn, m3<T>& Fn, m3<T>& Cn) {
-    const m3<T> F = load_m3(a.st, a.ld, PF, i);
-    const m3<T> Ft = a.ctl ? madd(F, load_m3(a.ctl, a.ld, 0, i)) : F;
+DMPM_DI void particle_update(const KP<T>& a, int i, const v3<T>& x, const m3<T>& F, const m3<T>& Ft,
+                             const v3<T>& vn, const m3<T>& Craw, v3<T>& xn, m3<T>& Fn, m3<T>& Cn) {
     const Trust<T> tr = trust_region(a, Ft, Craw);
     Cn = tr.state ? mscale(tr.s, Craw) : Craw;
     m3<T> A1 = mscale(a.dt, Cn);
@@ -763,6 +800,14 @@ DMPM_DI void particle_update(const KP<T>& a, int i, const v3<T>& x, const v3<T>&
     store_v3(a.out, a.ld, PV, i, vn);
     store_m3(a.out, a.ld, PF, i, Fn);
     store_m3(a.out, a.ld, PC, i, Cn);
+}
+
+template <typename T>
+DMPM_DI void particle_update(const KP<T>& a, int i, const v3<T>& x, const v3<T>& vn, const m3<T>& Craw,
+                             v3<T>& xn, m3<T>& Fn, m3<T>& Cn) {
+    const m3<T> F = load_m3(a.st, a.ld, PF, i);
+    const m3<T> Ft = a.ctl ? madd(F, load_m3(a.ctl, a.ld, 0, i)) : F;
+    particle_update(a, i, x, F, Ft, vn, Craw, xn, Fn, Cn);
 }
 
 // G2P + P_op_2 (ForwardSimulation.cpp:89-157): writes layer t+1.
@@ -818,9 +863,23 @@ DMPM_DI void step_block(const KP<T>& a, int lblock, double* smem_d) {
     block_range(a, lblock, i0, i1);
     const int i = i0 + (int)threadIdx.x;
     const bool valid = i < i1;
+    // Every per-particle input is loaded up front, so that the memory latency
+    // overlaps the window reduction and the staging of grid velocities
+    // (loads are not moved across the barriers by the compiler).
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
-    if (valid) x = load_v3(a.st, a.ld, PX, i);
+    m3<T> F = mzero<T>(), Ft = mzero<T>(), ctl1 = mzero<T>();
+    T m = T(0), vol = T(0), lam = T(0), mu = T(0);
+    if (valid) {
+        x = load_v3(a.st, a.ld, PX, i);
+        F = load_m3(a.st, a.ld, PF, i);
+        Ft = a.ctl ? madd(F, load_m3(a.ctl, a.ld, 0, i)) : F;
+        if (a.ctl1) ctl1 = load_m3(a.ctl1, a.ld, 0, i);
+        m = a.stat[SM * a.ld + i];
+        vol = a.stat[SVOL * a.ld + i];
+        lam = a.stat[SLAM * a.ld + i];
+        mu = a.stat[SMU * a.ld + i];
+    }
     S.init(a, x);  // lanes without a particle get a finite stencil at x = 0
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
     if (win.lds) {
@@ -831,7 +890,7 @@ DMPM_DI void step_block(const KP<T>& a, int lblock, double* smem_d) {
     m3<T> Craw, Fn = mzero<T>(), Cn = mzero<T>();
     if (valid) {
         g2p_gather<T, ORDER>(a, win, sm, S, vn, Craw);
-        particle_update(a, i, x, vn, Craw, xn, Fn, Cn);
+        particle_update(a, i, x, F, Ft, vn, Craw, xn, Fn, Cn);
     }
     // every wave is done with the staged velocities and the first window
     __syncthreads();
@@ -840,13 +899,11 @@ DMPM_DI void step_block(const KP<T>& a, int lblock, double* smem_d) {
     const Win win1 = block_window<KS>(valid, S1.b[0], S1.b[1], S1.b[2], a.cap);
     count_window(a, win1);
     if (win1.lds) window_zero<T, 4>(a, win1, sacc);
-    T m = T(0);
     m3<T> A = mzero<T>();
     v3<T> mv = vzero<T>();
     if (valid) {
-        m = a.stat[SM * a.ld + i];
-        const m3<T> Ft1 = a.ctl1 ? madd(Fn, load_m3(a.ctl1, a.ld, 0, i)) : Fn;
-        p2g_terms<T, ORDER, MODEL>(a, i, vn, Ft1, Cn, m, A, mv);
+        const m3<T> Ft1 = a.ctl1 ? madd(Fn, ctl1) : Fn;
+        p2g_terms<T, ORDER, MODEL>(a, vol, lam, mu, vn, Ft1, Cn, m, A, mv);
     }
     __syncthreads();
     p2g_scatter<T, ORDER, false>(a, win1, sacc, S1, valid, m, A, mv, a.gm1, a.gp1);
@@ -882,11 +939,31 @@ struct Phase1 {
     m3<T> gFdir;  // dL/dF through the smoothing blend (F only, not dFc)
 };
 
+// Adjoint carry of layer t+1 for one particle: dL/dx', dL/dv', dL/dF', dL/dC'.
+template <typename T>
+struct Carry {
+    v3<T> gx, gv;
+    m3<T> gF, gC;
+};
+
+template <typename T>
+DMPM_DI Carry<T> load_carry(const KP<T>& a, int i) {
+    Carry<T> c;
+    c.gx = load_v3(a.adj1, a.ld, PX, i);
+    c.gv = load_v3(a.adj1, a.ld, PV, i);
+    c.gF = load_m3(a.adj1, a.ld, PF, i);
+    c.gC = load_m3(a.adj1, a.ld, PC, i);
+    return c;
+}
+
 template <typename T, bool REF>
-DMPM_DI Phase1<T, REF> phase1(const KP<T>& a, int i, const m3<T>& Ft, const m3<T>& Craw, const Trust<T>& tr) {
+DMPM_DI Phase1<T, REF> phase1(const KP<T>& a, const Carry<T>& cy, const m3<T>& Ft, const m3<T>& Craw,
+                              const Trust<T>& tr) {
     Phase1<T, REF> r;
-    const v3<T> gx1 = load_v3(a.adj1, a.ld, PX, i), gv1 = load_v3(a.adj1, a.ld, PV, i);
-    const m3<T> gF1 = load_m3(a.adj1, a.ld, PF, i), gC1 = load_m3(a.adj1, a.ld, PC, i);
+    const v3<T>& gx1 = cy.gx;
+    const v3<T>& gv1 = cy.gv;
+    const m3<T>& gF1 = cy.gF;
+    const m3<T>& gC1 = cy.gC;
     const m3<T> Cn = tr.state ? mscale(tr.s, Craw) : Craw;
     m3<T> A1 = mscale(a.dt, Cn);
     A1.a[0] += T(1); A1.a[4] += T(1); A1.a[8] += T(1);
@@ -935,6 +1012,11 @@ DMPM_DI Phase1<T, REF> phase1(const KP<T>& a, int i, const m3<T>& Ft, const m3<T
     return r;
 }
 
+template <typename T, bool REF>
+DMPM_DI Phase1<T, REF> phase1(const KP<T>& a, int i, const m3<T>& Ft, const m3<T>& Craw, const Trust<T>& tr) {
+    return phase1<T, REF>(a, load_carry(a, i), Ft, Craw, tr);
+}
+
 // Adjoint phases 1 + 2: dL/dv_i += w gvtot + D^{-1} w gCraw (x_i - x_p)
 // (BackPropagation.cpp:36-55), accumulated in an LDS window.
 template <typename T, int ORDER, bool REF>
@@ -946,13 +1028,24 @@ DMPM_DI void adj_scatter_block(const KP<T>& a, int lblock, double* smem_d) {
     block_range(a, lblock, i0, i1);
     const int i = i0 + (int)threadIdx.x;
     const bool valid = i < i1;
+    const bool nxt = reads_next<T, REF>(a);
+    // Every per-particle input is loaded up front, so that the memory latency
+    // overlaps the window reduction and the clearing of the accumulators
+    // (loads are not moved across the barriers by the compiler).
     Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
-    if (valid) x = load_v3(a.st, a.ld, PX, i);
+    m3<T> Ft = mzero<T>(), Craw = mzero<T>();
+    Carry<T> cy;
+    if (valid) {
+        x = load_v3(a.st, a.ld, PX, i);
+        Ft = load_m3(a.st, a.ld, PF, i);
+        if (a.ctl) Ft = madd(Ft, load_m3(a.ctl, a.ld, 0, i));
+        if (nxt) Craw = load_m3(a.nxt, a.ld, PC, i);
+        cy = load_carry(a, i);
+    }
     S.init(a, x);  // lanes without a particle get a finite stencil at x = 0
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
     count_window(a, win);
-    const bool nxt = reads_next<T, REF>(a);
     if (win.lds) {
         if (!nxt) stage_velocity(a, win, sv);
         for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
@@ -965,19 +1058,15 @@ DMPM_DI void adj_scatter_block(const KP<T>& a, int lblock, double* smem_d) {
     v3<T> gvtot = vzero<T>();
     m3<T> Gc = mzero<T>();
     if (valid) {
-        m3<T> Ft = load_m3(a.st, a.ld, PF, i);
-        if (a.ctl) Ft = madd(Ft, load_m3(a.ctl, a.ld, 0, i));
-        m3<T> Craw;
         Trust<T> tr;
         if (nxt) {
-            Craw = load_m3(a.nxt, a.ld, PC, i);
             tr = trust_none<T>();
         } else {
             v3<T> vn;
             g2p_gather<T, ORDER>(a, win, sv, S, vn, Craw);
             tr = trust_region(a, Ft, Craw);
         }
-        const Phase1<T, REF> p1 = phase1<T, REF>(a, i, Ft, Craw, tr);
+        const Phase1<T, REF> p1 = phase1<T, REF>(a, cy, Ft, Craw, tr);
         gvtot = p1.gvtot;
         Gc = mscale(Spline<ORDER>::dinv(a.dx), p1.gCraw);
     }
@@ -993,10 +1082,9 @@ DMPM_DI void adj_scatter_block(const KP<T>& a, int lblock, double* smem_d) {
     });
     if (win.lds && win.any) {
         __syncthreads();
-        const int e12 = win.e1 * win.e2;
         for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
-            const int gi = win.lo0 + t / e12, gj = win.lo1 + (t / win.e2) % win.e1, gk = win.lo2 + t % win.e2;
-            if (!in_grid(a, gi, gj, gk)) continue;
+            int gi, gj, gk;
+            if (!window_node(win, t, gi, gj, gk) || !in_grid(a, gi, gj, gk)) continue;
             const T g0 = (T)sg[t], g1 = (T)sg[a.cap + t], g2 = (T)sg[2 * a.cap + t];
             if (g0 != T(0) || g1 != T(0) || g2 != T(0)) {
                 const long long g = gindex(a, gi, gj, gk);
@@ -1027,14 +1115,16 @@ __global__ void __launch_bounds__(256) k_adj_scatter_batch(const KP<T>* tab, con
 // gm = dL/dm_i.  Exact: dm uses v_i - dt f_ext; REF uses v_i (as written).
 template <typename T, bool REF>
 DMPM_DI void node_adjoint(const KP<T>& a, long long g, int i, int j, int k, T* v, T* h, T& gmass) {
+    // all seven loads issued before the branch (one memory round trip)
     const T m = a.gm[g];
+    const T q0 = a.gp[g], q1 = a.gp[a.gld + g], q2 = a.gp[2 * a.gld + g];
+    const T g0 = a.ggv[g], g1 = a.ggv[a.gld + g], g2 = a.ggv[2 * a.gld + g];
     if (m > T(1e-12) && !is_wall(a, i, j, k)) {
         const T inv = T(1) / m;
-        const T p0 = a.gp[g] * inv, p1 = a.gp[a.gld + g] * inv, p2 = a.gp[2 * a.gld + g] * inv;
+        const T p0 = q0 * inv, p1 = q1 * inv, p2 = q2 * inv;
         v[0] = p0 + a.dt * a.f0;
         v[1] = p1 + a.dt * a.f1;
         v[2] = p2 + a.dt * a.f2;
-        const T g0 = a.ggv[g], g1 = a.ggv[a.gld + g], g2 = a.ggv[2 * a.gld + g];
         h[0] = g0 * inv;
         h[1] = g1 * inv;
         h[2] = g2 * inv;
@@ -1057,17 +1147,33 @@ DMPM_DI void adj_gather_block(const KP<T>& a, int lblock, double* smem_d) {
     block_range(a, lblock, i0, i1);
     const int i = i0 + (int)threadIdx.x;
     const bool valid = i < i1;
+    const bool nxt = reads_next<T, REF>(a);
+    // Per-particle inputs are loaded up front, so that the memory latency
+    // overlaps the window reduction and the staging of the node values
+    // (loads are not moved across the barriers by the compiler).
     Sten<T, ORDER> S;
-    v3<T> x = vzero<T>();
-    if (valid) x = load_v3(a.st, a.ld, PX, i);
+    v3<T> x = vzero<T>(), vn = vzero<T>();
+    m3<T> F = mzero<T>(), Ft = mzero<T>(), Craw = mzero<T>();
+    Carry<T> cy;
+    if (valid) {
+        x = load_v3(a.st, a.ld, PX, i);
+        F = load_m3(a.st, a.ld, PF, i);
+        Ft = a.ctl ? madd(F, load_m3(a.ctl, a.ld, 0, i)) : F;
+        if (nxt) {
+            vn = load_v3(a.nxt, a.ld, PV, i);
+            Craw = load_m3(a.nxt, a.ld, PC, i);
+        }
+        cy = load_carry(a, i);
+    }
     S.init(a, x);  // lanes without a particle get a finite stencil at x = 0
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
     if (win.lds) {
-        const int e12 = win.e1 * win.e2;
+#pragma unroll 2
         for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
-            const int gi = win.lo0 + t / e12, gj = win.lo1 + (t / win.e2) % win.e1, gk = win.lo2 + t % win.e2;
+            int gi, gj, gk;
+            const bool real = window_node(win, t, gi, gj, gk);
             T v[3] = {T(0), T(0), T(0)}, h[3] = {T(0), T(0), T(0)}, gmass = T(0);
-            if (in_grid(a, gi, gj, gk)) node_adjoint<T, REF>(a, gindex(a, gi, gj, gk), gi, gj, gk, v, h, gmass);
+            if (real && in_grid(a, gi, gj, gk)) node_adjoint<T, REF>(a, gindex(a, gi, gj, gk), gi, gj, gk, v, h, gmass);
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 sv[q * a.cap + t] = v[q];
@@ -1078,20 +1184,14 @@ DMPM_DI void adj_gather_block(const KP<T>& a, int lblock, double* smem_d) {
         __syncthreads();
     }
     if (!valid) return;
-    const m3<T> F = load_m3(a.st, a.ld, PF, i);
-    const m3<T> Ft = a.ctl ? madd(F, load_m3(a.ctl, a.ld, 0, i)) : F;
-    v3<T> vn;
-    m3<T> Craw;
     Trust<T> tr;
-    if (reads_next<T, REF>(a)) {
-        vn = load_v3(a.nxt, a.ld, PV, i);
-        Craw = load_m3(a.nxt, a.ld, PC, i);
+    if (nxt) {
         tr = trust_none<T>();
     } else {
         g2p_gather<T, ORDER>(a, win, sv, S, vn, Craw);
         tr = trust_region(a, Ft, Craw);
     }
-    const Phase1<T, REF> p1 = phase1<T, REF>(a, i, Ft, Craw, tr);
+    const Phase1<T, REF> p1 = phase1<T, REF>(a, cy, Ft, Craw, tr);
 
     const T m = a.stat[SM * a.ld + i], vol = a.stat[SVOL * a.ld + i];
     const T lam = a.stat[SLAM * a.ld + i], mu = a.stat[SMU * a.ld + i];
@@ -1112,6 +1212,7 @@ DMPM_DI void adj_gather_block(const KP<T>& a, int lblock, double* smem_d) {
     const T mom = REF ? m : m * (T(1) - a.dt * a.drag);
     const m3<T> Gc = mscale(dinv, p1.gCraw);
     const v3<T> mvp = {{mom * vp.a[0], mom * vp.a[1], mom * vp.a[2]}};
+    const v3<T> gvtot = p1.gvtot;
 
     // accumulators
     m3<T> gA = mzero<T>();   // sum w h dgp^T
@@ -1133,7 +1234,7 @@ DMPM_DI void adj_gather_block(const KP<T>& a, int lblock, double* smem_d) {
         const T Cd1 = Gc.a[3] * d0 + Gc.a[4] * d1 + Gc.a[5] * d2;
         const T Cd2 = Gc.a[6] * d0 + Gc.a[7] * d1 + Gc.a[8] * d2;
         const T s = m * gmass + (mvp.a[0] + Gd0) * h[0] + (mvp.a[1] + Gd1) * h[1] + (mvp.a[2] + Gd2) * h[2] +
-                    v[0] * (p1.gvtot.a[0] + Cd0) + v[1] * (p1.gvtot.a[1] + Cd1) + v[2] * (p1.gvtot.a[2] + Cd2);
+                    v[0] * (gvtot.a[0] + Cd0) + v[1] * (gvtot.a[1] + Cd1) + v[2] * (gvtot.a[2] + Cd2);
         dxa.a[0] += s * gw0;
         dxa.a[1] += s * gw1;
         dxa.a[2] += s * gw2;
@@ -1145,7 +1246,7 @@ DMPM_DI void adj_gather_block(const KP<T>& a, int lblock, double* smem_d) {
     };
     if (win.lds) {
         // window nodes outside the grid were staged as zeros
-        const int e2 = win.e2, e12 = win.e1 * win.e2;
+        const int e2 = win.s2, e12 = win.s1;
         const int lb = window_base(win, S.b);
 #pragma unroll
         for (int ia = 0; ia < KS; ++ia)
@@ -1178,18 +1279,41 @@ DMPM_DI void adj_gather_block(const KP<T>& a, int lblock, double* smem_d) {
                     add(ia, ib, ic, v, h, gmass);
                 }
     }
+    // What only the code below needs (the carry, F + dFc, v' and C', the
+    // phase-1 terms of F and P) is loaded or computed again here instead of
+    // being held in registers through the node loop: the kernel then fits
+    // three waves per SIMD.  The index goes through an empty asm statement so
+    // that the compiler reloads instead of keeping the earlier values.
+    int j = i;
+    asm volatile("" : "+v"(j));
+    const Carry<T> cy2 = load_carry(a, j);
+    m3<T> Ft2 = load_m3(a.st, a.ld, PF, j);
+    if (a.ctl) Ft2 = madd(Ft2, load_m3(a.ctl, a.ld, 0, j));
+    v3<T> vn2;
+    m3<T> Craw2;
+    Trust<T> tr2;
+    if (nxt) {
+        vn2 = load_v3(a.nxt, a.ld, PV, j);
+        Craw2 = load_m3(a.nxt, a.ld, PC, j);
+        tr2 = trust_none<T>();
+    } else {
+        g2p_gather<T, ORDER>(a, win, sv, S, vn2, Craw2);
+        tr2 = trust_region(a, Ft2, Craw2);
+    }
+    const Phase1<T, REF> p1b = phase1<T, REF>(a, cy2, Ft2, Craw2, tr2);
+    const m3<T> P2 = Model<MODEL>::pk1(Ft2, lam, mu);
     // dL/dx_p: sum s_i grad w - Gt^T sum w h - D^{-1} gCraw^T v' + dL/dx'
-    const v3<T> gx1 = load_v3(a.adj1, a.ld, PX, i);
+    const v3<T> gx1 = cy2.gx;
     const v3<T> Gsh = mtv(Gt, sh);
-    const v3<T> Cv = mtv(Gc, vn);
+    const v3<T> Cv = mtv(Gc, vn2);
     v3<T> gx;
 #pragma unroll
     for (int q = 0; q < 3; ++q) gx.a[q] = dxa.a[q] - Gsh.a[q] - Cv.a[q] + gx1.a[q];
     const T dragf = m * (T(1) - a.dt * a.drag);
     const v3<T> gv = {{dragf * sh.a[0], dragf * sh.a[1], dragf * sh.a[2]}};
     // A_p = -k0 (P Ft^T + zeta tr C I) + m C
-    const m3<T> dP = mscale(-k0, mmul(gA, Ft));
-    m3<T> gFt = madd(p1.gFt, mscale(-k0, mtmul(gA, P)));
+    const m3<T> dP = mscale(-k0, mmul(gA, Ft2));
+    m3<T> gFt = madd(p1b.gFt, mscale(-k0, mtmul(gA, P2)));
     m3<T> gC = mscale(m, gA);
     if (REF) {
         // bulk-viscosity adjoint on the diagonal entries only (:117-123), and
@@ -1198,8 +1322,8 @@ DMPM_DI void adj_gather_block(const KP<T>& a, int lblock, double* smem_d) {
         gC.a[0] += factor * gA.a[0];
         gC.a[4] += factor * gA.a[4];
         gC.a[8] += factor * gA.a[8];
-        const m3<T> gF1 = load_m3(a.adj1, a.ld, PF, i);
-        const m3<T> Cn = tr.state ? mscale(tr.s, Craw) : Craw;
+        const m3<T> gF1 = cy2.gF;
+        const m3<T> Cn = tr2.state ? mscale(tr2.s, Craw2) : Craw2;
         m3<T> A1 = mscale(a.dt, Cn);
         A1.a[0] += T(1); A1.a[4] += T(1); A1.a[8] += T(1);
         gFt = madd(gFt, mtmul(A1, gF1));
@@ -1207,10 +1331,10 @@ DMPM_DI void adj_gather_block(const KP<T>& a, int lblock, double* smem_d) {
         const T tb = -k0 * a.zeta * mtrace(gA);
         gC.a[0] += tb; gC.a[4] += tb; gC.a[8] += tb;
     }
-    gFt = madd(gFt, Model<MODEL>::hvp(Ft, lam, mu, dP));
+    gFt = madd(gFt, Model<MODEL>::hvp(Ft2, lam, mu, dP));
     store_v3(a.adj0, a.ld, PX, i, gx);
     store_v3(a.adj0, a.ld, PV, i, gv);
-    const m3<T> gF = madd(gFt, p1.gFdir);
+    const m3<T> gF = madd(gFt, p1b.gFdir);
     store_m3(a.adj0, a.ld, PF, i, gF);
     store_m3(a.adj0, a.ld, PC, i, gC);
     if (a.gctl) store_m3(a.gctl, a.ld, 0, i, REF ? gF : gFt);

@@ -277,32 +277,34 @@ DMPM_DI void polar3(const m3<T>& F, m3<T>& R, m3<T>& S) {
 template <int MODEL> struct Model;
 
 template <> struct Model<1> {
-    // P = mu F + (lam log J - mu) F^{-T}, J clamped >= 1e-6 inside the log
+    // P = mu F + (lam log J - mu) F^{-T}, J clamped >= 1e-6 inside the log.
+    // det F, log J and F^{-T} are evaluated in FP64 (cancellation in det F
+    // under strong compression); F^{-T} and the scalar factor are then
+    // rounded to T and the products formed in T.
     template <typename T>
     static DMPM_DI m3<T> pk1(const m3<T>& F, T lam, T mu) {
-        m3<double> Fd = mcast<double>(F);
-        double J = mdet(Fd);
-        m3<double> Fit = mscale(1.0 / J, mcof(Fd));
-        double logJ = log(fmax(J, 1e-6));
-        double k = (double)lam * logJ - (double)mu;
+        const m3<double> Fd = mcast<double>(F);
+        const double J = mdet(Fd);
+        const m3<T> Fit = mcast<T>(mscale(1.0 / J, mcof(Fd)));
+        const T k = (T)((double)lam * log(fmax(J, 1e-6)) - (double)mu);
         m3<T> P;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) P.a[i] = (T)((double)mu * Fd.a[i] + k * Fit.a[i]);
+        for (int i = 0; i < 9; ++i) P.a[i] = mu * F.a[i] + k * Fit.a[i];
         return P;
     }
     // dP = mu dF + lam (F^{-T}:dF) F^{-T} + (mu - lam log J) F^{-T} dF^T F^{-T}
+    // (FP64 as in pk1 for det F, log J and F^{-T}).
     template <typename T>
     static DMPM_DI m3<T> hvp(const m3<T>& F, T lam, T mu, const m3<T>& dF) {
-        m3<double> Fd = mcast<double>(F), dFd = mcast<double>(dF);
-        double J = mdet(Fd);
-        m3<double> Fit = mscale(1.0 / J, mcof(Fd));
-        double logJ = log(fmax(J, 1e-6));
-        double a = (double)lam * mddot(Fit, dFd);
-        double b = (double)mu - (double)lam * logJ;
-        m3<double> X = mmul(mmult(Fit, dFd), Fit);  // F^{-T} dF^T F^{-T}
+        const m3<double> Fd = mcast<double>(F);
+        const double J = mdet(Fd);
+        const m3<T> Fit = mcast<T>(mscale(1.0 / J, mcof(Fd)));
+        const T b = (T)((double)mu - (double)lam * log(fmax(J, 1e-6)));
+        const T a = lam * mddot(Fit, dF);
+        const m3<T> X = mmul(mmult(Fit, dF), Fit);  // F^{-T} dF^T F^{-T}
         m3<T> R;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) R.a[i] = (T)((double)mu * dFd.a[i] + a * Fit.a[i] + b * X.a[i]);
+        for (int i = 0; i < 9; ++i) R.a[i] = mu * dF.a[i] + a * Fit.a[i] + b * X.a[i];
         return R;
     }
 };

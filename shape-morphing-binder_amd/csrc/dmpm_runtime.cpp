@@ -881,8 +881,6 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
         s->avmax.assign(1, nullptr);
         s->gctl.assign(1, nullptr);
         s->want_gctl.assign(1, 0);
-        s->perm = (int*)s->dalloc(n * sizeof(int));
-        HIPCHK(hipMemcpyAsync(s->perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, s->stream));
         // Thread blocks of the window kernels: runs of up to `block`
         // consecutive particles (in Hilbert order), cut where the box of the
         // run's stencils would exceed split_nodes nodes.  The curve is
@@ -923,10 +921,55 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
                 starts.push_back(j);
                 i = j;
             }
+            // Inside each block, order the runs of particles that share a
+            // base cell so that few of them cross a 16-lane row: the scatter
+            // kernels sum a run across lanes inside a row, and a run cut by a
+            // row boundary leaves two lanes adding to the same LDS address in
+            // one instruction, which the LDS serialises.  Best fit: fill each
+            // row with the longest remaining run that fits its free lanes;
+            // when none fits, the longest remaining run starts there.
+            std::vector<int> tmp;
+            for (size_t b = 0; b + 1 < starts.size(); ++b) {
+                const int s0 = starts[b], s1 = starts[b + 1];
+                std::vector<std::pair<int, int>> runs;  // (length, start)
+                for (int r = s0; r < s1;) {
+                    int e = r + 1;
+                    while (e < s1 && cell[3 * (size_t)e] == cell[3 * (size_t)r] &&
+                           cell[3 * (size_t)e + 1] == cell[3 * (size_t)r + 1] &&
+                           cell[3 * (size_t)e + 2] == cell[3 * (size_t)r + 2])
+                        ++e;
+                    runs.emplace_back(e - r, r);
+                    r = e;
+                }
+                std::stable_sort(runs.begin(), runs.end(),
+                                 [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first > y.first; });
+                std::vector<char> used(runs.size(), 0);
+                tmp.clear();
+                for (size_t placed = 0; placed < runs.size(); ++placed) {
+                    const int room = 16 - (int)(tmp.size() % 16);
+                    size_t pick = runs.size();
+                    for (size_t q = 0; q < runs.size(); ++q)
+                        if (!used[q] && runs[q].first <= room) {
+                            pick = q;
+                            break;
+                        }
+                    if (pick == runs.size())
+                        for (size_t q = 0; q < runs.size(); ++q)
+                            if (!used[q]) {
+                                pick = q;
+                                break;
+                            }
+                    used[pick] = 1;
+                    for (int k = 0; k < runs[pick].first; ++k) tmp.push_back(perm[runs[pick].second + k]);
+                }
+                std::copy(tmp.begin(), tmp.end(), perm.begin() + s0);
+            }
             s->nblk = (int)starts.size() - 1;
             s->bst = (int*)s->dalloc(starts.size() * sizeof(int));
             HIPCHK(hipMemcpy(s->bst, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice));
         }
+        s->perm = (int*)s->dalloc(n * sizeof(int));
+        HIPCHK(hipMemcpy(s->perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice));
         s->layer[0] = s->dalloc(s->state_bytes());
         s->stat = s->dalloc((size_t)NSTAT * s->ld * s->esz);
         for (auto& p : s->carry) p = s->dalloc(s->state_bytes());
