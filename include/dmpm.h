@@ -322,6 +322,7 @@ int dmpm_last_timing(const dmpm_sim* sim, double* fwd_ms, double* bwd_ms);
 #define DMPM_KERNEL_ADJ_GATHER 3
 #define DMPM_KERNEL_GRID_CLEAR 4
 #define DMPM_KERNEL_STEP 5
+#define DMPM_KERNEL_ADJ_FUSED 6  /* adjoint gather of step t + adjoint scatter of step t-1 in one launch */
 int dmpm_kernel_timer(dmpm_sim* sim, int capacity);
 int dmpm_kernel_stats(dmpm_sim* sim, int kind, double* total_ms, int* count);
 /* Scatter windows since the last reset: blocks whose stencil box was held

@@ -82,12 +82,17 @@ enum OpKind {
     OP_STEP,
     OP_ADJ_SCATTER,
     OP_ADJ_GATHER,
+    OP_ADJ_FUSED,    // Launch of step t (gather) + Launch of step t-1 (scatter)
 };
 inline bool op_is_kernel(int kind) { return kind >= OP_P2G; }
-// Size of one kernel parameter block for precision pr, and the block of
-// kernel op `kind` for launch L, written to host memory dst.
+// Parameter blocks one instance of kernel op `kind` takes (2 for the fused
+// adjoint, 1 otherwise).
+inline int kp_count(int kind) { return kind == OP_ADJ_FUSED ? 2 : 1; }
+// Size of one kernel parameter block for precision pr, and the block(s) of
+// kernel op `kind` for launch L (and L2, the second launch of a fused op),
+// written to host memory dst.
 size_t kp_bytes(Prec pr);
-void build_kp(Prec pr, int kind, const Launch& L, void* dst);
+void build_kp(Prec pr, int kind, const Launch& L, const Launch* L2, void* dst);
 // Thread blocks one instance takes (its block table's count, or n / block
 // rounded up without one).
 int instance_blocks(const Launch& L);
@@ -125,6 +130,12 @@ void step(Prec pr, const Launch& L, hipStream_t s);
 void adj_scatter(Prec pr, const Launch& L, hipStream_t s);
 // Adjoint phases 3+4 (+ the stress VJP): writes adj0 and gctl.
 void adj_gather(Prec pr, const Launch& L, hipStream_t s);
+// adj_gather of step t (L) and the adj_scatter of step t-1 (L2, into L2.ggv,
+// a different grid from L.ggv) in one launch; the carry of layer t stays in
+// registers between the two and C' of step t-1 is C of layer t.  Only for
+// steps whose adjoint reads C' from the next layer (no trust region, or the
+// reference adjoint) and when layer t is the forward step of layer t-1.
+void adj_fused(Prec pr, const Launch& L, const Launch& L2, hipStream_t s);
 
 // Deterministic two-pass mode: scratch size for n particles on a grid of
 // `dims` nodes (element size esize), P2G and adjoint scatter that write every

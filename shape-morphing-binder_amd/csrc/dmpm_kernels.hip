@@ -1017,11 +1017,14 @@ DMPM_DI Phase1<T, REF> phase1(const KP<T>& a, int i, const m3<T>& Ft, const m3<T
     return phase1<T, REF>(a, load_carry(a, i), Ft, Craw, tr);
 }
 
+template <typename T, int ORDER, bool REF>
+DMPM_DI void adj_scatter_core(const KP<T>& a, bool valid, const v3<T>& x, const m3<T>& Ft, m3<T> Craw,
+                              const Carry<T>& cy, bool nxt, T* sv, Acc* sg);
+
 // Adjoint phases 1 + 2: dL/dv_i += w gvtot + D^{-1} w gCraw (x_i - x_p)
 // (BackPropagation.cpp:36-55), accumulated in an LDS window.
 template <typename T, int ORDER, bool REF>
 DMPM_DI void adj_scatter_block(const KP<T>& a, int lblock, double* smem_d) {
-    constexpr int KS = Spline<ORDER>::K;
     T* sv = reinterpret_cast<T*>(smem_d);  // 3 velocity arrays, then 3 accumulators
     Acc* sg = reinterpret_cast<Acc*>(smem_d + (3 * a.cap * sizeof(T) + 7) / 8);
     int i0, i1;
@@ -1032,7 +1035,6 @@ DMPM_DI void adj_scatter_block(const KP<T>& a, int lblock, double* smem_d) {
     // Every per-particle input is loaded up front, so that the memory latency
     // overlaps the window reduction and the clearing of the accumulators
     // (loads are not moved across the barriers by the compiler).
-    Sten<T, ORDER> S;
     v3<T> x = vzero<T>();
     m3<T> Ft = mzero<T>(), Craw = mzero<T>();
     Carry<T> cy;
@@ -1043,6 +1045,18 @@ DMPM_DI void adj_scatter_block(const KP<T>& a, int lblock, double* smem_d) {
         if (nxt) Craw = load_m3(a.nxt, a.ld, PC, i);
         cy = load_carry(a, i);
     }
+    adj_scatter_core<T, ORDER, REF>(a, valid, x, Ft, Craw, cy, nxt, sv, sg);
+}
+
+// Phases 1 + 2 of one step for the block's particles, given their inputs:
+// x, F + dFc of layer t, C' (of layer t+1 when nxt, else recomputed by a G2P
+// from the grid staged in sv) and the carry of layer t+1.  Every thread of
+// the block must call this.
+template <typename T, int ORDER, bool REF>
+DMPM_DI void adj_scatter_core(const KP<T>& a, bool valid, const v3<T>& x, const m3<T>& Ft, m3<T> Craw,
+                              const Carry<T>& cy, bool nxt, T* sv, Acc* sg) {
+    constexpr int KS = Spline<ORDER>::K;
+    Sten<T, ORDER> S;
     S.init(a, x);  // lanes without a particle get a finite stencil at x = 0
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
     count_window(a, win);
@@ -1139,8 +1153,13 @@ DMPM_DI void node_adjoint(const KP<T>& a, long long g, int i, int j, int k, T* v
 
 // Adjoint phases 3 + 4 and the stress VJP (BackPropagation.cpp:59-143).
 // The window holds per node v_i (3), h_i = dL/dp_i (3), dL/dm_i (1).
-template <typename T, int ORDER, int MODEL, bool REF>
-DMPM_DI void adj_gather_block(const KP<T>& a, int lblock, double* smem_d) {
+// FUSED: then phases 1 + 2 of the previous step (t-1, parameters *b) for
+// the same particles, with the carry of layer t just computed kept in
+// registers and C' of step t-1 = C of layer t (the caller fuses only when
+// layer t is the forward step of layer t-1 and the adjoint reads C' from
+// it); b->ggv must be a different grid from a.ggv.
+template <typename T, int ORDER, int MODEL, bool REF, bool FUSED>
+DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double* smem_d) {
     constexpr int KS = Spline<ORDER>::K;
     T* sv = reinterpret_cast<T*>(smem_d);  // v (3 arrays), h (3), gm (1)
     int i0, i1;
@@ -1183,167 +1202,196 @@ DMPM_DI void adj_gather_block(const KP<T>& a, int lblock, double* smem_d) {
         }
         __syncthreads();
     }
-    if (!valid) return;
-    Trust<T> tr;
-    if (nxt) {
-        tr = trust_none<T>();
-    } else {
-        g2p_gather<T, ORDER>(a, win, sv, S, vn, Craw);
-        tr = trust_region(a, Ft, Craw);
-    }
-    const Phase1<T, REF> p1 = phase1<T, REF>(a, cy, Ft, Craw, tr);
+    Carry<T> cyt;  // carry of layer t (the outputs)
+    cyt.gx = cyt.gv = vzero<T>();
+    cyt.gF = cyt.gC = mzero<T>();
+    if (valid) {
+        Trust<T> tr;
+        if (nxt) {
+            tr = trust_none<T>();
+        } else {
+            g2p_gather<T, ORDER>(a, win, sv, S, vn, Craw);
+            tr = trust_region(a, Ft, Craw);
+        }
+        const Phase1<T, REF> p1 = phase1<T, REF>(a, cy, Ft, Craw, tr);
 
-    const T m = a.stat[SM * a.ld + i], vol = a.stat[SVOL * a.ld + i];
-    const T lam = a.stat[SLAM * a.ld + i], mu = a.stat[SMU * a.ld + i];
-    const v3<T> vp = load_v3(a.st, a.ld, PV, i);
-    const m3<T> C = load_m3(a.st, a.ld, PC, i);
-    const m3<T> P = Model<MODEL>::pk1(Ft, lam, mu);
-    const T dinv = Spline<ORDER>::dinv(a.dx);
-    const T k0 = dinv * a.dt * vol;
-    // Gt = A_p = -k0 (P Ft^T + zeta tr(C) I) + m C
-    m3<T> Gt;
-    {
-        const m3<T> PFt = mmult(P, Ft);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) Gt.a[q] = -k0 * PFt.a[q] + m * C.a[q];
-        const T bulk = -k0 * a.zeta * mtrace(C);
-        Gt.a[0] += bulk; Gt.a[4] += bulk; Gt.a[8] += bulk;
-    }
-    const T mom = REF ? m : m * (T(1) - a.dt * a.drag);
-    const m3<T> Gc = mscale(dinv, p1.gCraw);
-    const v3<T> mvp = {{mom * vp.a[0], mom * vp.a[1], mom * vp.a[2]}};
-    const v3<T> gvtot = p1.gvtot;
+        const T m = a.stat[SM * a.ld + i], vol = a.stat[SVOL * a.ld + i];
+        const T lam = a.stat[SLAM * a.ld + i], mu = a.stat[SMU * a.ld + i];
+        const v3<T> vp = load_v3(a.st, a.ld, PV, i);
+        const m3<T> C = load_m3(a.st, a.ld, PC, i);
+        const m3<T> P = Model<MODEL>::pk1(Ft, lam, mu);
+        const T dinv = Spline<ORDER>::dinv(a.dx);
+        const T k0 = dinv * a.dt * vol;
+        // Gt = A_p = -k0 (P Ft^T + zeta tr(C) I) + m C
+        m3<T> Gt;
+        {
+            const m3<T> PFt = mmult(P, Ft);
+    #pragma unroll
+            for (int q = 0; q < 9; ++q) Gt.a[q] = -k0 * PFt.a[q] + m * C.a[q];
+            const T bulk = -k0 * a.zeta * mtrace(C);
+            Gt.a[0] += bulk; Gt.a[4] += bulk; Gt.a[8] += bulk;
+        }
+        const T mom = REF ? m : m * (T(1) - a.dt * a.drag);
+        const m3<T> Gc = mscale(dinv, p1.gCraw);
+        const v3<T> mvp = {{mom * vp.a[0], mom * vp.a[1], mom * vp.a[2]}};
+        const v3<T> gvtot = p1.gvtot;
 
-    // accumulators
-    m3<T> gA = mzero<T>();   // sum w h dgp^T
-    v3<T> sh = vzero<T>();   // sum w h
-    v3<T> dxa = vzero<T>();  // sum s_i grad w
-    const T invdx = T(-1) / a.dx;
-    auto add = [&](int ia, int ib, int ic, const T* v, const T* h, T gmass) {
-        const T d0 = S.u[0][ia] * a.dx, d1 = S.u[1][ib] * a.dx, d2 = S.u[2][ic] * a.dx;
-        const T wt = S.w[0][ia] * S.w[1][ib] * S.w[2][ic];
-        // grad_{x_p} w = -(1/dx) (N' N N, N N' N, N N N')
-        const T gw0 = invdx * S.dw[0][ia] * S.w[1][ib] * S.w[2][ic];
-        const T gw1 = invdx * S.w[0][ia] * S.dw[1][ib] * S.w[2][ic];
-        const T gw2 = invdx * S.w[0][ia] * S.w[1][ib] * S.dw[2][ic];
-        // q = mom v_p + Gt dgp ; (Gc dgp)
-        const T Gd0 = Gt.a[0] * d0 + Gt.a[1] * d1 + Gt.a[2] * d2;
-        const T Gd1 = Gt.a[3] * d0 + Gt.a[4] * d1 + Gt.a[5] * d2;
-        const T Gd2 = Gt.a[6] * d0 + Gt.a[7] * d1 + Gt.a[8] * d2;
-        const T Cd0 = Gc.a[0] * d0 + Gc.a[1] * d1 + Gc.a[2] * d2;
-        const T Cd1 = Gc.a[3] * d0 + Gc.a[4] * d1 + Gc.a[5] * d2;
-        const T Cd2 = Gc.a[6] * d0 + Gc.a[7] * d1 + Gc.a[8] * d2;
-        const T s = m * gmass + (mvp.a[0] + Gd0) * h[0] + (mvp.a[1] + Gd1) * h[1] + (mvp.a[2] + Gd2) * h[2] +
-                    v[0] * (gvtot.a[0] + Cd0) + v[1] * (gvtot.a[1] + Cd1) + v[2] * (gvtot.a[2] + Cd2);
-        dxa.a[0] += s * gw0;
-        dxa.a[1] += s * gw1;
-        dxa.a[2] += s * gw2;
-        const T wh0 = wt * h[0], wh1 = wt * h[1], wh2 = wt * h[2];
-        sh.a[0] += wh0; sh.a[1] += wh1; sh.a[2] += wh2;
-        gA.a[0] += wh0 * d0; gA.a[1] += wh0 * d1; gA.a[2] += wh0 * d2;
-        gA.a[3] += wh1 * d0; gA.a[4] += wh1 * d1; gA.a[5] += wh1 * d2;
-        gA.a[6] += wh2 * d0; gA.a[7] += wh2 * d1; gA.a[8] += wh2 * d2;
-    };
-    if (win.lds) {
-        // window nodes outside the grid were staged as zeros
-        const int e2 = win.s2, e12 = win.s1;
-        const int lb = window_base(win, S.b);
-#pragma unroll
-        for (int ia = 0; ia < KS; ++ia)
-#pragma unroll
-            for (int ib = 0; ib < KS; ++ib) {
-                const int row = lb + ia * e12 + ib * e2;
-#pragma unroll
-                for (int ic = 0; ic < KS; ++ic) {
-                    const int li = row + ic;
-                    T v[3], h[3];
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        v[q] = sv[q * a.cap + li];
-                        h[q] = sv[(3 + q) * a.cap + li];
+        // accumulators
+        m3<T> gA = mzero<T>();   // sum w h dgp^T
+        v3<T> sh = vzero<T>();   // sum w h
+        v3<T> dxa = vzero<T>();  // sum s_i grad w
+        const T invdx = T(-1) / a.dx;
+        auto add = [&](int ia, int ib, int ic, const T* v, const T* h, T gmass) {
+            const T d0 = S.u[0][ia] * a.dx, d1 = S.u[1][ib] * a.dx, d2 = S.u[2][ic] * a.dx;
+            const T wt = S.w[0][ia] * S.w[1][ib] * S.w[2][ic];
+            // grad_{x_p} w = -(1/dx) (N' N N, N N' N, N N N')
+            const T gw0 = invdx * S.dw[0][ia] * S.w[1][ib] * S.w[2][ic];
+            const T gw1 = invdx * S.w[0][ia] * S.dw[1][ib] * S.w[2][ic];
+            const T gw2 = invdx * S.w[0][ia] * S.w[1][ib] * S.dw[2][ic];
+            // q = mom v_p + Gt dgp ; (Gc dgp)
+            const T Gd0 = Gt.a[0] * d0 + Gt.a[1] * d1 + Gt.a[2] * d2;
+            const T Gd1 = Gt.a[3] * d0 + Gt.a[4] * d1 + Gt.a[5] * d2;
+            const T Gd2 = Gt.a[6] * d0 + Gt.a[7] * d1 + Gt.a[8] * d2;
+            const T Cd0 = Gc.a[0] * d0 + Gc.a[1] * d1 + Gc.a[2] * d2;
+            const T Cd1 = Gc.a[3] * d0 + Gc.a[4] * d1 + Gc.a[5] * d2;
+            const T Cd2 = Gc.a[6] * d0 + Gc.a[7] * d1 + Gc.a[8] * d2;
+            const T s = m * gmass + (mvp.a[0] + Gd0) * h[0] + (mvp.a[1] + Gd1) * h[1] + (mvp.a[2] + Gd2) * h[2] +
+                        v[0] * (gvtot.a[0] + Cd0) + v[1] * (gvtot.a[1] + Cd1) + v[2] * (gvtot.a[2] + Cd2);
+            dxa.a[0] += s * gw0;
+            dxa.a[1] += s * gw1;
+            dxa.a[2] += s * gw2;
+            const T wh0 = wt * h[0], wh1 = wt * h[1], wh2 = wt * h[2];
+            sh.a[0] += wh0; sh.a[1] += wh1; sh.a[2] += wh2;
+            gA.a[0] += wh0 * d0; gA.a[1] += wh0 * d1; gA.a[2] += wh0 * d2;
+            gA.a[3] += wh1 * d0; gA.a[4] += wh1 * d1; gA.a[5] += wh1 * d2;
+            gA.a[6] += wh2 * d0; gA.a[7] += wh2 * d1; gA.a[8] += wh2 * d2;
+        };
+        if (win.lds) {
+            // window nodes outside the grid were staged as zeros
+            const int e2 = win.s2, e12 = win.s1;
+            const int lb = window_base(win, S.b);
+    #pragma unroll
+            for (int ia = 0; ia < KS; ++ia)
+    #pragma unroll
+                for (int ib = 0; ib < KS; ++ib) {
+                    const int row = lb + ia * e12 + ib * e2;
+    #pragma unroll
+                    for (int ic = 0; ic < KS; ++ic) {
+                        const int li = row + ic;
+                        T v[3], h[3];
+    #pragma unroll
+                        for (int q = 0; q < 3; ++q) {
+                            v[q] = sv[q * a.cap + li];
+                            h[q] = sv[(3 + q) * a.cap + li];
+                        }
+                        add(ia, ib, ic, v, h, sv[6 * a.cap + li]);
                     }
-                    add(ia, ib, ic, v, h, sv[6 * a.cap + li]);
                 }
-            }
-    } else {
-#pragma unroll
-        for (int ia = 0; ia < KS; ++ia)
-#pragma unroll
-            for (int ib = 0; ib < KS; ++ib)
-#pragma unroll
-                for (int ic = 0; ic < KS; ++ic) {
-                    const int gi = S.b[0] + ia, gj = S.b[1] + ib, gk = S.b[2] + ic;
-                    if (!in_grid(a, gi, gj, gk)) continue;
-                    T v[3], h[3], gmass;
-                    node_adjoint<T, REF>(a, gindex(a, gi, gj, gk), gi, gj, gk, v, h, gmass);
-                    add(ia, ib, ic, v, h, gmass);
-                }
+        } else {
+    #pragma unroll
+            for (int ia = 0; ia < KS; ++ia)
+    #pragma unroll
+                for (int ib = 0; ib < KS; ++ib)
+    #pragma unroll
+                    for (int ic = 0; ic < KS; ++ic) {
+                        const int gi = S.b[0] + ia, gj = S.b[1] + ib, gk = S.b[2] + ic;
+                        if (!in_grid(a, gi, gj, gk)) continue;
+                        T v[3], h[3], gmass;
+                        node_adjoint<T, REF>(a, gindex(a, gi, gj, gk), gi, gj, gk, v, h, gmass);
+                        add(ia, ib, ic, v, h, gmass);
+                    }
+        }
+        // What only the code below needs (the carry, F + dFc, v' and C', the
+        // phase-1 terms of F and P) is loaded or computed again here instead of
+        // being held in registers through the node loop: the kernel then fits
+        // three waves per SIMD.  The index goes through an empty asm statement so
+        // that the compiler reloads instead of keeping the earlier values.
+        int j = i;
+        asm volatile("" : "+v"(j));
+        const Carry<T> cy2 = load_carry(a, j);
+        m3<T> Ft2 = load_m3(a.st, a.ld, PF, j);
+        if (a.ctl) Ft2 = madd(Ft2, load_m3(a.ctl, a.ld, 0, j));
+        v3<T> vn2;
+        m3<T> Craw2;
+        Trust<T> tr2;
+        if (nxt) {
+            vn2 = load_v3(a.nxt, a.ld, PV, j);
+            Craw2 = load_m3(a.nxt, a.ld, PC, j);
+            tr2 = trust_none<T>();
+        } else {
+            g2p_gather<T, ORDER>(a, win, sv, S, vn2, Craw2);
+            tr2 = trust_region(a, Ft2, Craw2);
+        }
+        const Phase1<T, REF> p1b = phase1<T, REF>(a, cy2, Ft2, Craw2, tr2);
+        const m3<T> P2 = Model<MODEL>::pk1(Ft2, lam, mu);
+        // dL/dx_p: sum s_i grad w - Gt^T sum w h - D^{-1} gCraw^T v' + dL/dx'
+        const v3<T> gx1 = cy2.gx;
+        const v3<T> Gsh = mtv(Gt, sh);
+        const v3<T> Cv = mtv(Gc, vn2);
+        v3<T> gx;
+    #pragma unroll
+        for (int q = 0; q < 3; ++q) gx.a[q] = dxa.a[q] - Gsh.a[q] - Cv.a[q] + gx1.a[q];
+        const T dragf = m * (T(1) - a.dt * a.drag);
+        const v3<T> gv = {{dragf * sh.a[0], dragf * sh.a[1], dragf * sh.a[2]}};
+        // A_p = -k0 (P Ft^T + zeta tr C I) + m C
+        const m3<T> dP = mscale(-k0, mmul(gA, Ft2));
+        m3<T> gFt = madd(p1b.gFt, mscale(-k0, mtmul(gA, P2)));
+        m3<T> gC = mscale(m, gA);
+        if (REF) {
+            // bulk-viscosity adjoint on the diagonal entries only (:117-123), and
+            // (I + dt C')^T dL/dF' added a second time (:140)
+            const T factor = -k0 * a.zeta;
+            gC.a[0] += factor * gA.a[0];
+            gC.a[4] += factor * gA.a[4];
+            gC.a[8] += factor * gA.a[8];
+            const m3<T> gF1 = cy2.gF;
+            const m3<T> Cn = tr2.state ? mscale(tr2.s, Craw2) : Craw2;
+            m3<T> A1 = mscale(a.dt, Cn);
+            A1.a[0] += T(1); A1.a[4] += T(1); A1.a[8] += T(1);
+            gFt = madd(gFt, mtmul(A1, gF1));
+        } else {
+            const T tb = -k0 * a.zeta * mtrace(gA);
+            gC.a[0] += tb; gC.a[4] += tb; gC.a[8] += tb;
+        }
+        gFt = madd(gFt, Model<MODEL>::hvp(Ft2, lam, mu, dP));
+        store_v3(a.adj0, a.ld, PX, i, gx);
+        store_v3(a.adj0, a.ld, PV, i, gv);
+        const m3<T> gF = madd(gFt, p1b.gFdir);
+        store_m3(a.adj0, a.ld, PF, i, gF);
+        store_m3(a.adj0, a.ld, PC, i, gC);
+        if (FUSED) {
+            cyt.gx = gx;
+            cyt.gv = gv;
+            cyt.gF = gF;
+            cyt.gC = gC;
+        }
+        if (a.gctl) store_m3(a.gctl, a.ld, 0, i, REF ? gF : gFt);
     }
-    // What only the code below needs (the carry, F + dFc, v' and C', the
-    // phase-1 terms of F and P) is loaded or computed again here instead of
-    // being held in registers through the node loop: the kernel then fits
-    // three waves per SIMD.  The index goes through an empty asm statement so
-    // that the compiler reloads instead of keeping the earlier values.
-    int j = i;
-    asm volatile("" : "+v"(j));
-    const Carry<T> cy2 = load_carry(a, j);
-    m3<T> Ft2 = load_m3(a.st, a.ld, PF, j);
-    if (a.ctl) Ft2 = madd(Ft2, load_m3(a.ctl, a.ld, 0, j));
-    v3<T> vn2;
-    m3<T> Craw2;
-    Trust<T> tr2;
-    if (nxt) {
-        vn2 = load_v3(a.nxt, a.ld, PV, j);
-        Craw2 = load_m3(a.nxt, a.ld, PC, j);
-        tr2 = trust_none<T>();
-    } else {
-        g2p_gather<T, ORDER>(a, win, sv, S, vn2, Craw2);
-        tr2 = trust_region(a, Ft2, Craw2);
+    if constexpr (FUSED) {
+        v3<T> x1 = vzero<T>();
+        m3<T> Ft1 = mzero<T>(), C1 = mzero<T>();
+        if (valid) {
+            x1 = load_v3(b->st, b->ld, PX, i);
+            Ft1 = load_m3(b->st, b->ld, PF, i);
+            if (b->ctl) Ft1 = madd(Ft1, load_m3(b->ctl, b->ld, 0, i));
+            C1 = load_m3(a.st, a.ld, PC, i);
+        }
+        __syncthreads();  // the node values staged above are read no more
+        adj_scatter_core<T, ORDER, REF>(*b, valid, x1, Ft1, C1, cyt, true, nullptr, reinterpret_cast<Acc*>(smem_d));
     }
-    const Phase1<T, REF> p1b = phase1<T, REF>(a, cy2, Ft2, Craw2, tr2);
-    const m3<T> P2 = Model<MODEL>::pk1(Ft2, lam, mu);
-    // dL/dx_p: sum s_i grad w - Gt^T sum w h - D^{-1} gCraw^T v' + dL/dx'
-    const v3<T> gx1 = cy2.gx;
-    const v3<T> Gsh = mtv(Gt, sh);
-    const v3<T> Cv = mtv(Gc, vn2);
-    v3<T> gx;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) gx.a[q] = dxa.a[q] - Gsh.a[q] - Cv.a[q] + gx1.a[q];
-    const T dragf = m * (T(1) - a.dt * a.drag);
-    const v3<T> gv = {{dragf * sh.a[0], dragf * sh.a[1], dragf * sh.a[2]}};
-    // A_p = -k0 (P Ft^T + zeta tr C I) + m C
-    const m3<T> dP = mscale(-k0, mmul(gA, Ft2));
-    m3<T> gFt = madd(p1b.gFt, mscale(-k0, mtmul(gA, P2)));
-    m3<T> gC = mscale(m, gA);
-    if (REF) {
-        // bulk-viscosity adjoint on the diagonal entries only (:117-123), and
-        // (I + dt C')^T dL/dF' added a second time (:140)
-        const T factor = -k0 * a.zeta;
-        gC.a[0] += factor * gA.a[0];
-        gC.a[4] += factor * gA.a[4];
-        gC.a[8] += factor * gA.a[8];
-        const m3<T> gF1 = cy2.gF;
-        const m3<T> Cn = tr2.state ? mscale(tr2.s, Craw2) : Craw2;
-        m3<T> A1 = mscale(a.dt, Cn);
-        A1.a[0] += T(1); A1.a[4] += T(1); A1.a[8] += T(1);
-        gFt = madd(gFt, mtmul(A1, gF1));
-    } else {
-        const T tb = -k0 * a.zeta * mtrace(gA);
-        gC.a[0] += tb; gC.a[4] += tb; gC.a[8] += tb;
-    }
-    gFt = madd(gFt, Model<MODEL>::hvp(Ft2, lam, mu, dP));
-    store_v3(a.adj0, a.ld, PX, i, gx);
-    store_v3(a.adj0, a.ld, PV, i, gv);
-    const m3<T> gF = madd(gFt, p1b.gFdir);
-    store_m3(a.adj0, a.ld, PF, i, gF);
-    store_m3(a.adj0, a.ld, PC, i, gC);
-    if (a.gctl) store_m3(a.gctl, a.ld, 0, i, REF ? gF : gFt);
 }
 
 template <typename T, int ORDER, int MODEL, bool REF>
 __global__ void __launch_bounds__(256) k_adj_gather(KP<T> a) {
     extern __shared__ double smem_d[];
-    adj_gather_block<T, ORDER, MODEL, REF>(a, logical_block(), smem_d);
+    adj_gather_block<T, ORDER, MODEL, REF, false>(a, nullptr, logical_block(), smem_d);
+}
+
+// Adjoint of step t (gather) and of step t-1 (scatter) in one launch.
+template <typename T, int ORDER, int MODEL, bool REF>
+__global__ void __launch_bounds__(256) k_adj_fused(KP<T> a, KP<T> b) {
+    extern __shared__ double smem_d[];
+    adj_gather_block<T, ORDER, MODEL, REF, true>(a, &b, logical_block(), smem_d);
 }
 
 template <typename T, int ORDER, int MODEL, bool REF>
@@ -1352,7 +1400,19 @@ __global__ void __launch_bounds__(256) k_adj_gather_batch(const KP<T>* tab, cons
     int inst, lblock;
     if (!batch_block(bstart, count, inst, lblock)) return;
     const KP<T> a = tab[inst];
-    adj_gather_block<T, ORDER, MODEL, REF>(a, lblock, smem_d);
+    adj_gather_block<T, ORDER, MODEL, REF, false>(a, nullptr, lblock, smem_d);
+}
+
+// Batched k_adj_fused: tab holds two parameter blocks per instance (step t,
+// then step t-1).
+template <typename T, int ORDER, int MODEL, bool REF>
+__global__ void __launch_bounds__(256) k_adj_fused_batch(const KP<T>* tab, const int* bstart, int count) {
+    extern __shared__ double smem_d[];
+    int inst, lblock;
+    if (!batch_block(bstart, count, inst, lblock)) return;
+    const KP<T> a = tab[2 * inst];
+    const KP<T> b = tab[2 * inst + 1];
+    adj_gather_block<T, ORDER, MODEL, REF, true>(a, &b, lblock, smem_d);
 }
 
 // ---------------------------------------------------------------------------
@@ -1895,6 +1955,25 @@ static void launch_adj_gather(const Launch& L, hipStream_t s) {
 #undef AG_CASE
 }
 
+template <typename T>
+static void launch_adj_fused(const Launch& L, const Launch& L2, hipStream_t s) {
+    KP<T> k = make_kp<T>(L, 7 * (int)sizeof(T));
+    KP<T> k2 = make_kp<T>(L2, 3 * (int)sizeof(Acc));
+    const unsigned nb = launch_blocks(L);
+    const size_t sh = std::max((size_t)k.cap * 7 * sizeof(T), (size_t)k2.cap * 3 * sizeof(Acc));
+    const bool ref = L.ph.adjoint == 1;
+    const int o = L.ph.order, md = L.ph.model;
+#define AF_CASE(O, M, R) hipLaunchKernelGGL((k_adj_fused<T, O, M, R>), dim3(nb), dim3(L.block), sh, s, k, k2)
+    if (o == 3) {
+        if (md == 0) { if (ref) AF_CASE(3, 0, true); else AF_CASE(3, 0, false); }
+        else { if (ref) AF_CASE(3, 1, true); else AF_CASE(3, 1, false); }
+    } else {
+        if (md == 0) { if (ref) AF_CASE(2, 0, true); else AF_CASE(2, 0, false); }
+        else { if (ref) AF_CASE(2, 1, true); else AF_CASE(2, 1, false); }
+    }
+#undef AF_CASE
+}
+
 // ---- batched launches --------------------------------------------------------
 // Bytes of LDS per window node of each kernel op (the window capacity is the
 // LDS budget divided by this), and the dynamic LDS a block of it takes.
@@ -1907,6 +1986,7 @@ static int node_bytes(int kind) {
         case OP_STEP: return 4 * (int)sizeof(Acc);
         case OP_ADJ_SCATTER: return 3 * (int)sizeof(T) + 3 * (int)sizeof(Acc) + 8;
         case OP_ADJ_GATHER: return 7 * (int)sizeof(T);
+        case OP_ADJ_FUSED: return 7 * (int)sizeof(T);  // the gather's window; the scatter's uses 3 Acc
         default: return 0;
     }
 }
@@ -1918,14 +1998,19 @@ static size_t block_lds(int kind, int cap) {
 
 size_t kp_bytes(Prec pr) { return pr == PREC_F32 ? sizeof(KP<float>) : sizeof(KP<double>); }
 
-void build_kp(Prec pr, int kind, const Launch& L, void* dst) {
-    if (pr == PREC_F32) {
-        const KP<float> k = make_kp<float>(L, node_bytes<float>(kind));
-        std::memcpy(dst, &k, sizeof(k));
-    } else {
-        const KP<double> k = make_kp<double>(L, node_bytes<double>(kind));
-        std::memcpy(dst, &k, sizeof(k));
+template <typename T>
+static void build_kp_t(int kind, const Launch& L, const Launch* L2, void* dst) {
+    const KP<T> k = make_kp<T>(L, node_bytes<T>(kind));
+    std::memcpy(dst, &k, sizeof(k));
+    if (kind == OP_ADJ_FUSED) {
+        const KP<T> k2 = make_kp<T>(*L2, 3 * (int)sizeof(Acc));
+        std::memcpy((char*)dst + sizeof(k), &k2, sizeof(k2));
     }
+}
+
+void build_kp(Prec pr, int kind, const Launch& L, const Launch* L2, void* dst) {
+    if (pr == PREC_F32) build_kp_t<float>(kind, L, L2, dst);
+    else build_kp_t<double>(kind, L, L2, dst);
 }
 
 int instance_blocks(const Launch& L) {
@@ -1939,7 +2024,8 @@ static void launch_batch_t(int kind, const Phys& ph, int lds_bytes, int block, c
     const KP<T>* tab = (const KP<T>*)tab_;
     const unsigned nb = (unsigned)std::max(8, (total + 7) & ~7);
     const int cap = lds_bytes / node_bytes<T>(kind);
-    const size_t sh = block_lds<T>(kind, cap);
+    size_t sh = block_lds<T>(kind, cap);
+    if (kind == OP_ADJ_FUSED) sh = std::max(sh, (size_t)(lds_bytes / (3 * (int)sizeof(Acc))) * 3 * sizeof(Acc));
     const int o = ph.order, md = ph.model;
     const bool ref = ph.adjoint == 1;
 #define BATCH(KERNEL) hipLaunchKernelGGL(KERNEL, dim3(nb), dim3(block), sh, s, tab, bstart, count)
@@ -1969,6 +2055,15 @@ static void launch_batch_t(int kind, const Phys& ph, int lds_bytes, int block, c
             } else {
                 if (md == 0) { if (ref) BATCH((k_adj_gather_batch<T, 2, 0, true>)); else BATCH((k_adj_gather_batch<T, 2, 0, false>)); }
                 else { if (ref) BATCH((k_adj_gather_batch<T, 2, 1, true>)); else BATCH((k_adj_gather_batch<T, 2, 1, false>)); }
+            }
+            break;
+        case OP_ADJ_FUSED:
+            if (o == 3) {
+                if (md == 0) { if (ref) BATCH((k_adj_fused_batch<T, 3, 0, true>)); else BATCH((k_adj_fused_batch<T, 3, 0, false>)); }
+                else { if (ref) BATCH((k_adj_fused_batch<T, 3, 1, true>)); else BATCH((k_adj_fused_batch<T, 3, 1, false>)); }
+            } else {
+                if (md == 0) { if (ref) BATCH((k_adj_fused_batch<T, 2, 0, true>)); else BATCH((k_adj_fused_batch<T, 2, 0, false>)); }
+                else { if (ref) BATCH((k_adj_fused_batch<T, 2, 1, true>)); else BATCH((k_adj_fused_batch<T, 2, 1, false>)); }
             }
             break;
         default: break;
@@ -2047,6 +2142,10 @@ void adj_scatter(Prec pr, const Launch& L, hipStream_t s) {
 void adj_gather(Prec pr, const Launch& L, hipStream_t s) {
     if (pr == PREC_F32) launch_adj_gather<float>(L, s);
     else launch_adj_gather<double>(L, s);
+}
+void adj_fused(Prec pr, const Launch& L, const Launch& L2, hipStream_t s) {
+    if (pr == PREC_F32) launch_adj_fused<float>(L, L2, s);
+    else launch_adj_fused<double>(L, L2, s);
 }
 
 void winding_numbers(const double* tri, int nf, const double* P, int np, double* partial, int chunks, double* w,

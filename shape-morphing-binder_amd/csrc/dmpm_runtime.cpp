@@ -134,6 +134,7 @@ struct dmpm_sim {
         return det;
     }
     void* ggv = nullptr;   // 3 planes
+    void* ggv2 = nullptr;  // 3 planes: dL/dv of the previous step, written by the fused adjoint
     void* carry[2] = {nullptr, nullptr};
     int carry_cur = 0;
     int carry_layer = -1;   // layer whose adjoint the current carry holds
@@ -153,7 +154,7 @@ struct dmpm_sim {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double fwd_ms = 0, bwd_ms = 0;
     // per-kernel event timing (dmpm_kernel_timer): pairs of events per launch
-    static constexpr int NKIND = 6;  // p2g, g2p, adj_scatter, adj_gather, grid clear, fused step
+    static constexpr int NKIND = 7;  // p2g, g2p, adj_scatter, adj_gather, grid clear, fused step, fused adjoint
     bool ktimer = false;
     bool capturing = false;  // inside dmpm_graph_capture: no timing events
     std::vector<hipEvent_t> kpool;
@@ -244,19 +245,20 @@ struct dmpm_sim {
         void* dst;
         const void* src;
         long long n;
+        Launch L2;  // second launch of a fused adjoint op
     };
     std::vector<Op>* rec = nullptr;
 
     void op_zero(void* p, long long elems) {
         if (rec) {
-            rec->push_back(Op{OP_ZERO, Launch{}, p, nullptr, elems});
+            rec->push_back(Op{OP_ZERO, Launch{}, p, nullptr, elems, Launch{}});
             return;
         }
         timed(4, [&] { zero(p, elems); });
     }
     void op_copy(void* dst, const void* src, long long elems) {
         if (rec) {
-            rec->push_back(Op{OP_COPY, Launch{}, dst, src, elems});
+            rec->push_back(Op{OP_COPY, Launch{}, dst, src, elems, Launch{}});
             return;
         }
         copy_elems(pr, dst, src, elems, stream);
@@ -264,14 +266,14 @@ struct dmpm_sim {
     // dst += src over `elems` elements of the sim's precision
     void op_accum(void* dst, const void* src, long long elems) {
         if (rec) {
-            rec->push_back(Op{OP_ACCUM, Launch{}, dst, src, elems});
+            rec->push_back(Op{OP_ACCUM, Launch{}, dst, src, elems, Launch{}});
             return;
         }
         accumulate_carry(pr, src, pr == PREC_F64, dst, 0, 1, nullptr, (int)elems, elems, stream);
     }
     void op_kernel(int kind, const Launch& Lc) {
         if (rec) {
-            rec->push_back(Op{kind, Lc, nullptr, nullptr, 0});
+            rec->push_back(Op{kind, Lc, nullptr, nullptr, 0, Launch{}});
             return;
         }
         switch (kind) {
@@ -283,6 +285,15 @@ struct dmpm_sim {
             case OP_ADJ_GATHER: timed(3, [&] { adj_gather(pr, Lc, stream); }); break;
             default: throw Error(DMPM_ERR_ARG, "unknown kernel op");
         }
+    }
+    // The adjoint gather of step t (Lc) fused with the adjoint scatter of
+    // step t-1 (Lp).
+    void op_adj_fused(const Launch& Lc, const Launch& Lp) {
+        if (rec) {
+            rec->push_back(Op{OP_ADJ_FUSED, Lc, nullptr, nullptr, 0, Lp});
+            return;
+        }
+        timed(6, [&] { adj_fused(pr, Lc, Lp, stream); });
     }
 
     bool stored(int t) const { return K <= 1 || t % K == 0 || t == L - 1; }
@@ -348,6 +359,17 @@ struct dmpm_sim {
         }
         ensure_segment(t);
         return seg[t - seg_base];
+    }
+
+    // State of layer t if it is in device memory now (stored and computed,
+    // or in the current segment) without a recompute, else null.
+    const void* resident_state(int t) const {
+        if (t < 0 || t >= L) return nullptr;
+        if (K > 1) {
+            if (seg_base < 0 || t < seg_base || t >= seg_base + K || t > L - 2) return nullptr;
+            return t == seg_base ? layer[t] : seg[t - seg_base];
+        }
+        return valid[t] ? layer[t] : nullptr;
     }
 
     // Zero `g` and P2G layer t's state `in` into it.
@@ -465,6 +487,12 @@ struct dmpm_sim {
         op_copy(carry[carry_cur], adj_last, NSTATE * ld);
         carry_layer = L - 1;
         if (!capturing && !rec) HIPCHK(hipEventRecord(ev[2], stream));
+        // dL/dv grid of the step being processed, and the other one; a fused
+        // launch (gather of step t + scatter of step t-1) leaves step t-1's
+        // dL/dv in the other grid, so the next step skips its scatter
+        void* gcur = ggv;
+        void* gnext = ggv2;
+        bool scattered = false;
         for (int t = L - 2; t >= stop; --t) {
             const void* st;
             void* g;
@@ -497,7 +525,7 @@ struct dmpm_sim {
             Lc.stat = stat;
             Lc.ctl = ctl[t];
             Lc.grid = g;
-            Lc.ggv = ggv;
+            Lc.ggv = gcur;
             Lc.nxt = cfg.adjoint_recompute ? nullptr : nxt;
             Lc.adj1 = carry[carry_cur];
             Lc.adj0 = carry[carry_cur ^ 1];
@@ -511,11 +539,32 @@ struct dmpm_sim {
             if (cfg.deterministic) {
                 if (rec) throw Error(DMPM_ERR_STATE, "deterministic mode cannot be batched");
                 timed(2, [&] { det_adj_scatter(pr, Lc, det_buf(), stream); });
-            } else {
-                op_zero(ggv, 3 * gld);
+            } else if (!scattered) {
+                op_zero(gcur, 3 * gld);
                 op_kernel(OP_ADJ_SCATTER, Lc);
             }
-            op_kernel(OP_ADJ_GATHER, Lc);
+            // Fuse this step's gather with the scatter of step t-1 when layer
+            // t-1 is resident (stored, or in the current segment) and layer t
+            // is its forward step as it stands, and the adjoint reads C' from
+            // the next layer.
+            const void* prev = t - 1 >= stop ? resident_state(t - 1) : nullptr;
+            const bool fuse = prev && !cfg.deterministic && !cfg.adjoint_recompute && fresh[t] &&
+                              (cfg.adjoint == DMPM_ADJOINT_REFERENCE || !cfg.trust_region);
+            if (fuse) {
+                Launch Lp = launch();
+                Lp.st = prev;
+                Lp.stat = stat;
+                Lp.ctl = ctl[t - 1];
+                Lp.ggv = gnext;
+                Lp.nxt = st;
+                op_zero(gnext, 3 * gld);
+                op_adj_fused(Lc, Lp);
+                std::swap(gcur, gnext);
+                scattered = true;
+            } else {
+                op_kernel(OP_ADJ_GATHER, Lc);
+                scattered = false;
+            }
             if (t == stop && want_gctl[t]) {
                 if (!gctl[t]) gctl[t] = dalloc(ctl_bytes());
                 op_copy(gctl[t], gctl_stop, 9 * ld);
@@ -782,6 +831,7 @@ int dmpm_create(const dmpm_config* cfg, dmpm_sim** out) {
         s->grid = s->dalloc(4 * s->gld * s->esz);
         s->grid2 = s->dalloc(4 * s->gld * s->esz);
         s->ggv = s->dalloc(3 * s->gld * s->esz);
+        s->ggv2 = s->dalloc(3 * s->gld * s->esz);
         s->target_m = s->dalloc(s->gld * s->esz);
         s->dldm = s->dalloc(s->gld * s->esz);
         s->zero(s->target_m, s->gld);
@@ -1816,7 +1866,7 @@ int dmpm_batch_capture(dmpm_sim** sims, int count, int start_layer, int stop_lay
         for (size_t k = 0; k < nops; ++k) {
             items[k].off = bytes;
             if (op_is_kernel(ops[0][k].kind)) {
-                bytes += kpb * count;
+                bytes += kpb * kp_count(ops[0][k].kind) * count;
                 bytes = (bytes + 15) & ~size_t(15);
                 items[k].off2 = bytes;
                 bytes += ((count + 1) * sizeof(int) + 15) & ~size_t(15);
@@ -1833,7 +1883,7 @@ int dmpm_batch_capture(dmpm_sim** sims, int count, int start_layer, int stop_lay
                 bs[0] = 0;
                 for (int i = 0; i < count; ++i) {
                     const Launch& Li = ops[i][k].L;
-                    build_kp(pr, kind, Li, host.data() + items[k].off + kpb * i);
+                    build_kp(pr, kind, Li, &ops[i][k].L2, host.data() + items[k].off + kpb * kp_count(kind) * i);
                     bs[i + 1] = bs[i] + instance_blocks(Li);
                 }
                 items[k].total = bs[count];

@@ -21,8 +21,9 @@ ADJOINT_EXACT, ADJOINT_REFERENCE = 0, 1
 (FIELD_X, FIELD_V, FIELD_F, FIELD_C, FIELD_DFC, FIELD_FTOTAL, FIELD_M, FIELD_VOL, FIELD_LAM,
  FIELD_MU, FIELD_P, FIELD_ADAM_M, FIELD_ADAM_V, FIELD_ADAM_VMAX) = range(14)
 
-KERNEL_P2G, KERNEL_G2P, KERNEL_ADJ_SCATTER, KERNEL_ADJ_GATHER, KERNEL_GRID_CLEAR, KERNEL_STEP = range(6)
-KERNEL_NAMES = ["p2g", "g2p", "adj_scatter", "adj_gather", "grid_clear", "step"]
+(KERNEL_P2G, KERNEL_G2P, KERNEL_ADJ_SCATTER, KERNEL_ADJ_GATHER, KERNEL_GRID_CLEAR, KERNEL_STEP,
+ KERNEL_ADJ_FUSED) = range(7)
+KERNEL_NAMES = ["p2g", "g2p", "adj_scatter", "adj_gather", "grid_clear", "step", "adj_fused"]
 
 ERR_NAMES = {-1: "ERR_ARG", -2: "ERR_INDEX", -3: "ERR_HIP", -4: "ERR_STATE", -5: "ERR_NOMEM"}
 
