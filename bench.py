@@ -40,27 +40,32 @@ METRIC = "fwd+bwd particle-steps/s at 1M pts,128³ grid; % HBM peak; grad err vs
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per launch, FP32 state (4-byte elements): per particle and
-# per active grid node (a node any particle's stencil touches).
+# per active grid node (a node any particle's stencil touches).  The adjoint
+# kernels read G2P's v' and C' of step t from layer t+1 (no recompute).
 #   p2g         reads x, v, F, C, m, vol, lam, mu (28 floats); writes m, p per node
 #   g2p         reads x, F (12 floats) and m, p per node; writes x, v, F, C (24 floats)
-#   adj_scatter reads x, F (12) and the adjoint carry (24), m, p per node; writes dL/dv per node
-#   adj_gather  reads x, v, F, C, m, vol, lam, mu (28) and the carry (24), m, p, dL/dv per node;
-#               writes the carry (24)
+#   adj_scatter reads x, F (12), the adjoint carry (24) and C of layer t+1 (9); writes dL/dv per node
+#   adj_gather  reads x, v, F, C, m, vol, lam, mu (28), v and C of layer t+1 (12) and the carry (24),
+#               m, p, dL/dv per node; writes the carry (24)
+#   adj_fused   adj_gather of step t, then adj_scatter of step t-1: adds x, F of layer t-1 (12 floats)
+#               (C of layer t and the carry of layer t are already in registers) and dL/dv of
+#               step t-1 per node
 #   step        the fused forward step: reads x, F, m, vol, lam, mu (16 floats) and m, p of grid t per node;
 #               writes x, v, F, C (24 floats) and m, p of grid t+1 per node
 BYTES = {
     "step": (40 * 4, 32),
     "p2g": (28 * 4, 16),
     "g2p": (36 * 4, 16),
-    "adj_scatter": (36 * 4, 28),
-    "adj_gather": (76 * 4, 28),
+    "adj_scatter": (45 * 4, 12),
+    "adj_gather": (88 * 4, 28),
+    "adj_fused": (100 * 4, 40),
 }
 
 
 def profiled_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary of the default workload (None if absent)."""
-    path = os.path.join(ROOT, "profiles", "k_step_1M_128cubed_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", "kernels_1M_128cubed_hbm_traffic.md")
     try:
         for line in open(path):
             f = line.split()
@@ -492,7 +497,7 @@ def main():
             "physics": "plain: quadratic B-spline, Neo-Hookean, sticky walls (3 cells)",
             "precision": "FP32 state/arithmetic, FP64 det/log J/F^-T", "E": 5e4, "nu": 0.45, "dt": 1e-4,
             "active_nodes": gact, "peak_device_bytes": mem_peak,
-            "lds_bytes": lds_bytes or 48 * 1024, "block": block or 256,
+            "lds_bytes": lds_bytes or 32 * 1024, "block": block or 256,
         },
         "windows": {"lds_blocks": win_lds, "fallback_blocks": win_fb},
         "autotune": tuned,

@@ -211,7 +211,10 @@ struct dmpm_sim {
         Lc.wstat = wstat;
         Lc.bst = bst;
         Lc.nblk = nblk;
-        Lc.lds_bytes = cfg.lds_bytes > 0 ? cfg.lds_bytes : 48 * 1024;
+        // default LDS budget: the windows of 256-particle blocks cut at 512
+        // stencil nodes fit 32 KiB with FP32 state (the autotuner's pick on
+        // the 1M-particle workload); FP64 staging needs 48 KiB
+        Lc.lds_bytes = cfg.lds_bytes > 0 ? cfg.lds_bytes : (pr == PREC_F32 ? 32 : 48) * 1024;
         Lc.block = cfg.block > 0 ? cfg.block : 256;
         return Lc;
     }
