@@ -1267,14 +1267,35 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
             gA.a[6] += wh2 * d0; gA.a[7] += wh2 * d1; gA.a[8] += wh2 * d2;
         };
         if (win.lds) {
-            // window nodes outside the grid were staged as zeros
+            // Window nodes outside the grid were staged as zeros.  The same
+            // sums as add(), grouped by axis: the node weight w0 w1 w2, its
+            // gradient (-1/dx)(w0' w1 w2, w0 w1' w2, w0 w1 w2') and the
+            // offsets d are separable, so each (x, y) column first sums over
+            // z with the z factors -- s w2, s w2', h w2, h w2 d2 -- and the
+            // x, y factors are applied once per column.  The column part of
+            // Gt d and Gc d is also formed once per column.
             const int e2 = win.s2, e12 = win.s1;
             const int lb = window_base(win, S.b);
+            T d2[KS], w2d[KS];
     #pragma unroll
-            for (int ia = 0; ia < KS; ++ia)
+            for (int ic = 0; ic < KS; ++ic) {
+                d2[ic] = S.u[2][ic] * a.dx;
+                w2d[ic] = S.w[2][ic] * d2[ic];
+            }
+    #pragma unroll
+            for (int ia = 0; ia < KS; ++ia) {
+                const T d0 = S.u[0][ia] * a.dx;
     #pragma unroll
                 for (int ib = 0; ib < KS; ++ib) {
+                    const T d1 = S.u[1][ib] * a.dx;
                     const int row = lb + ia * e12 + ib * e2;
+                    T g01[3], c01[3];
+    #pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        g01[q] = mvp.a[q] + Gt.a[3 * q] * d0 + Gt.a[3 * q + 1] * d1;
+                        c01[q] = gvtot.a[q] + Gc.a[3 * q] * d0 + Gc.a[3 * q + 1] * d1;
+                    }
+                    T zs = T(0), zsd = T(0), zh[3] = {T(0), T(0), T(0)}, zhd[3] = {T(0), T(0), T(0)};
     #pragma unroll
                     for (int ic = 0; ic < KS; ++ic) {
                         const int li = row + ic;
@@ -1284,9 +1305,32 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
                             v[q] = sv[q * a.cap + li];
                             h[q] = sv[(3 + q) * a.cap + li];
                         }
-                        add(ia, ib, ic, v, h, sv[6 * a.cap + li]);
+                        T sn = m * sv[6 * a.cap + li];
+    #pragma unroll
+                        for (int q = 0; q < 3; ++q)
+                            sn += (g01[q] + Gt.a[3 * q + 2] * d2[ic]) * h[q] + (c01[q] + Gc.a[3 * q + 2] * d2[ic]) * v[q];
+                        zs += S.w[2][ic] * sn;
+                        zsd += S.dw[2][ic] * sn;
+    #pragma unroll
+                        for (int q = 0; q < 3; ++q) {
+                            zh[q] += S.w[2][ic] * h[q];
+                            zhd[q] += w2d[ic] * h[q];
+                        }
+                    }
+                    const T w0 = S.w[0][ia], w1 = S.w[1][ib], w01 = w0 * w1;
+                    dxa.a[0] += invdx * S.dw[0][ia] * w1 * zs;
+                    dxa.a[1] += invdx * w0 * S.dw[1][ib] * zs;
+                    dxa.a[2] += invdx * w01 * zsd;
+                    const T w01d0 = w01 * d0, w01d1 = w01 * d1;
+    #pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        sh.a[q] += w01 * zh[q];
+                        gA.a[3 * q] += w01d0 * zh[q];
+                        gA.a[3 * q + 1] += w01d1 * zh[q];
+                        gA.a[3 * q + 2] += w01 * zhd[q];
                     }
                 }
+            }
         } else {
     #pragma unroll
             for (int ia = 0; ia < KS; ++ia)
