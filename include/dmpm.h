@@ -182,6 +182,15 @@ int dmpm_set_last_layer_grad(dmpm_sim* sim, const void* dLdx, const void* dLdv, 
 int dmpm_set_render_gradients(dmpm_sim* sim, int n, const void* dLdF, const void* dLdx, int dtype, int on_device);
 int dmpm_clear_render_gradients(dmpm_sim* sim);
 int dmpm_render_gradient_info(const dmpm_sim* sim, int* has, int* num_points, double* norm_F, double* norm_x);
+/* Reorder the particle storage by the Hilbert order of layer `layer`'s
+ * positions (thread blocks and cell runs rebuilt as at upload).  Particles
+ * drift away from the order they were uploaded in, which widens the blocks'
+ * LDS windows and breaks the runs the scatters merge; sorting by a layer in
+ * the middle of an episode halves the drift.  Every per-particle array (all
+ * layers, controls, Adam state, gradients) is permuted; results do not
+ * change beyond summation order, and getters keep the caller's order.  No
+ * counterpart in the reference (its particles have no device order). */
+int dmpm_sort_particles(dmpm_sim* sim, int layer);
 /* promote_last_as_initial(carry_grid) (bind.cpp:282-291). */
 int dmpm_promote_last_as_initial(dmpm_sim* sim, int carry_grid);
 

@@ -162,6 +162,10 @@ void export_stress(Prec pr, int model, const void* st, const void* ctl, const vo
                    const int* perm, int n, long long ld, hipStream_t s);
 void export_F_total(Prec pr, const void* st, const void* ctl, void* dst, int dst_f64, const int* perm, int n,
                     long long ld, hipStream_t s);
+// Reorder particle planes: dst[c*ld + j] = src[c*ld + q[j]] (c < nc, j < n).
+void permute_planes(Prec pr, const void* src, void* dst, const int* q, int n, int nc, long long ld, hipStream_t s);
+// dst[j] = src[q[j]] for n ints.
+void permute_ints(const int* src, int* dst, const int* q, int n, hipStream_t s);
 // Copy between same-layout plane buffers (`count` elements).
 void copy_elems(Prec pr, void* dst, const void* src, long long count, hipStream_t s);
 

@@ -2201,6 +2201,27 @@ void winding_numbers(const double* tri, int nf, const double* P, int np, double*
     hipLaunchKernelGGL(k_winding_sum, dim3(nbx), dim3(256), 0, s, partial, chunks, np, w);
 }
 
+template <typename T>
+__global__ void k_permute(const T* src, T* dst, const int* q, int n, int nc, long long ld) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int i = q[j];
+    for (int c = 0; c < nc; ++c) dst[c * ld + j] = src[c * ld + i];
+}
+
+void permute_planes(Prec pr, const void* src, void* dst, const int* q, int n, int nc, long long ld, hipStream_t s) {
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (pr == PREC_F32)
+        hipLaunchKernelGGL(k_permute<float>, dim3(nb), dim3(256), 0, s, (const float*)src, (float*)dst, q, n, nc, ld);
+    else
+        hipLaunchKernelGGL(k_permute<double>, dim3(nb), dim3(256), 0, s, (const double*)src, (double*)dst, q, n, nc,
+                           ld);
+}
+
+void permute_ints(const int* src, int* dst, const int* q, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_permute<int>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, q, n, 1, 0LL);
+}
+
 void fill_zero(Prec pr, void* p, long long n, hipStream_t s) {
     if (n <= 0) return;
     const long long bytes = n * (pr == PREC_F32 ? 4 : 8);

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -869,6 +870,122 @@ int dmpm_synchronize(dmpm_sim* s) {
     });
 }
 
+namespace {
+// Order n particles for the particle kernels.  pos(i, a) is coordinate a of
+// particle i in the caller's numbering.  order[j] = the particle stored j-th;
+// starts = the thread blocks' first particles (nblk + 1 entries, the last n),
+// or empty when blocks are not cut (split_nodes < 0).
+void order_particles(const dmpm_config& cfg, int n, const std::function<double(int, int)>& pos,
+                     std::vector<int>& perm, std::vector<int>& starts) {
+    // Hilbert order of the stencil base cell of each particle (host): the
+    // base cell is floor(x/dx - 1/2) for the quadratic B-spline and
+    // floor(x/dx) - 1 for the cubic one, so particles sharing a stencil
+    // end up adjacent and the scatter kernels can merge their sums
+    // across lanes before touching LDS.
+    const double shift = cfg.order == 2 ? 0.5 : 0.0;
+    std::vector<uint64_t> key(n);
+    for (int i = 0; i < n; ++i) {
+        uint32_t c[3];
+        for (int a = 0; a < 3; ++a) {
+            double f = (pos(i, a) - cfg.grid_min[a]) / cfg.dx;
+            long long ci = (long long)std::floor(f - shift) + 2;
+            ci = std::max(0LL, std::min(ci, 1023LL));
+            c[a] = (uint32_t)ci;
+        }
+        key[i] = ((uint64_t)hilbert3(c[0], c[1], c[2]) << 32) | (uint32_t)i;
+    }
+    std::sort(key.begin(), key.end());
+    perm.resize(n);
+    for (int i = 0; i < n; ++i) perm[i] = (int)(key[i] & 0xffffffffu);
+    starts.clear();
+
+    // Thread blocks of the window kernels: runs of up to `block`
+    // consecutive particles (in Hilbert order), cut where the box of the
+    // run's stencils would exceed split_nodes nodes.  The curve is
+    // continuous, but where it leaves the body and re-enters it elsewhere
+    // consecutive particles lie far apart; a block spanning such a jump
+    // would have a window too large for LDS and fall back to global
+    // atomics (and run several times longer than the others).
+    const long long split = cfg.split_nodes == 0 ? 512 : cfg.split_nodes;
+    if (split > 0) {
+        const int B = cfg.block > 0 ? cfg.block : 256;
+        const long long KS = cfg.order == 2 ? 3 : 4;
+        std::vector<int> cell((size_t)n * 3);
+        for (int j = 0; j < n; ++j)
+            for (int a = 0; a < 3; ++a) {
+                const double f = (pos(perm[j], a) - cfg.grid_min[a]) / cfg.dx;
+                cell[3 * (size_t)j + a] = (int)std::max(-1e9, std::min(1e9, std::floor(f - shift)));
+            }
+        starts.assign(1, 0);
+        for (int i = 0; i < n;) {
+            int lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) lo[a] = hi[a] = cell[3 * (size_t)i + a];
+            int j = i + 1;
+            for (; j < n && j - i < B; ++j) {
+                int l2[3], h2[3];
+                long long vol = 1;
+                for (int a = 0; a < 3; ++a) {
+                    l2[a] = std::min(lo[a], cell[3 * (size_t)j + a]);
+                    h2[a] = std::max(hi[a], cell[3 * (size_t)j + a]);
+                    vol *= (long long)h2[a] - l2[a] + KS;
+                }
+                if (vol > split) break;
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = l2[a];
+                    hi[a] = h2[a];
+                }
+            }
+            starts.push_back(j);
+            i = j;
+        }
+        // Inside each block, order the runs of particles that share a
+        // base cell so that few of them cross a 16-lane row: the scatter
+        // kernels sum a run across lanes inside a row, and a run cut by a
+        // row boundary leaves two lanes adding to the same LDS address in
+        // one instruction, which the LDS serialises.  Best fit: fill each
+        // row with the longest remaining run that fits its free lanes;
+        // when none fits, the longest remaining run starts there.
+        std::vector<int> tmp;
+        for (size_t b = 0; b + 1 < starts.size(); ++b) {
+            const int s0 = starts[b], s1 = starts[b + 1];
+            std::vector<std::pair<int, int>> runs;  // (length, start)
+            for (int r = s0; r < s1;) {
+                int e = r + 1;
+                while (e < s1 && cell[3 * (size_t)e] == cell[3 * (size_t)r] &&
+                       cell[3 * (size_t)e + 1] == cell[3 * (size_t)r + 1] &&
+                       cell[3 * (size_t)e + 2] == cell[3 * (size_t)r + 2])
+                    ++e;
+                runs.emplace_back(e - r, r);
+                r = e;
+            }
+            std::stable_sort(runs.begin(), runs.end(),
+                             [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first > y.first; });
+            std::vector<char> used(runs.size(), 0);
+            tmp.clear();
+            for (size_t placed = 0; placed < runs.size(); ++placed) {
+                const int room = 16 - (int)(tmp.size() % 16);
+                size_t pick = runs.size();
+                for (size_t q = 0; q < runs.size(); ++q)
+                    if (!used[q] && runs[q].first <= room) {
+                        pick = q;
+                        break;
+                    }
+                if (pick == runs.size())
+                    for (size_t q = 0; q < runs.size(); ++q)
+                        if (!used[q]) {
+                            pick = q;
+                            break;
+                        }
+                used[pick] = 1;
+                for (int k = 0; k < runs[pick].first; ++k) tmp.push_back(perm[runs[pick].second + k]);
+            }
+            std::copy(tmp.begin(), tmp.end(), perm.begin() + s0);
+        }
+    }
+}
+
+}  // namespace
+
 int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const void* F, const void* C,
                        const void* dFc, const void* m, const void* vol, const void* lam, const void* mu, int dtype,
                        int on_device) {
@@ -877,29 +994,12 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
         if (n <= 0) throw Error(DMPM_ERR_ARG, "n must be positive");
         if (!x || !v || !F || !C || !m || !vol || !lam || !mu) throw Error(DMPM_ERR_ARG, "null particle array");
         const size_t es = dsize(dtype);
-        // Hilbert order of the stencil base cell of each particle (host): the
-        // base cell is floor(x/dx - 1/2) for the quadratic B-spline and
-        // floor(x/dx) - 1 for the cubic one, so particles sharing a stencil
-        // end up adjacent and the scatter kernels can merge their sums
-        // across lanes before touching LDS.
         std::vector<char> xh((size_t)n * 3 * es);
         if (on_device) HIPCHK(hipMemcpy(xh.data(), x, xh.size(), hipMemcpyDeviceToHost));
         else std::memcpy(xh.data(), x, xh.size());
-        const double shift = s->cfg.order == 2 ? 0.5 : 0.0;
-        std::vector<uint64_t> key(n);
-        for (int i = 0; i < n; ++i) {
-            uint32_t c[3];
-            for (int a = 0; a < 3; ++a) {
-                double f = (host_get(xh.data(), dtype, 3 * (size_t)i + a) - s->cfg.grid_min[a]) / s->cfg.dx;
-                long long ci = (long long)std::floor(f - shift) + 2;
-                ci = std::max(0LL, std::min(ci, 1023LL));
-                c[a] = (uint32_t)ci;
-            }
-            key[i] = ((uint64_t)hilbert3(c[0], c[1], c[2]) << 32) | (uint32_t)i;
-        }
-        std::sort(key.begin(), key.end());
-        std::vector<int> perm(n);
-        for (int i = 0; i < n; ++i) perm[i] = (int)(key[i] & 0xffffffffu);
+        std::vector<int> perm, starts;
+        order_particles(s->cfg, n, [&](int i, int a) { return host_get(xh.data(), dtype, 3 * (size_t)i + a); }, perm,
+                        starts);
 
         // (re)allocate particle storage
         for (auto& p : s->layer) s->dfree(p);
@@ -934,89 +1034,7 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
         s->avmax.assign(1, nullptr);
         s->gctl.assign(1, nullptr);
         s->want_gctl.assign(1, 0);
-        // Thread blocks of the window kernels: runs of up to `block`
-        // consecutive particles (in Hilbert order), cut where the box of the
-        // run's stencils would exceed split_nodes nodes.  The curve is
-        // continuous, but where it leaves the body and re-enters it elsewhere
-        // consecutive particles lie far apart; a block spanning such a jump
-        // would have a window too large for LDS and fall back to global
-        // atomics (and run several times longer than the others).
-        const long long split = s->cfg.split_nodes == 0 ? 512 : s->cfg.split_nodes;
-        if (split > 0) {
-            const int B = s->cfg.block > 0 ? s->cfg.block : 256;
-            const long long KS = s->cfg.order == 2 ? 3 : 4;
-            std::vector<int> cell((size_t)n * 3);
-            for (int j = 0; j < n; ++j)
-                for (int a = 0; a < 3; ++a) {
-                    const double f =
-                        (host_get(xh.data(), dtype, 3 * (size_t)perm[j] + a) - s->cfg.grid_min[a]) / s->cfg.dx;
-                    cell[3 * (size_t)j + a] = (int)std::max(-1e9, std::min(1e9, std::floor(f - shift)));
-                }
-            std::vector<int> starts{0};
-            for (int i = 0; i < n;) {
-                int lo[3], hi[3];
-                for (int a = 0; a < 3; ++a) lo[a] = hi[a] = cell[3 * (size_t)i + a];
-                int j = i + 1;
-                for (; j < n && j - i < B; ++j) {
-                    int l2[3], h2[3];
-                    long long vol = 1;
-                    for (int a = 0; a < 3; ++a) {
-                        l2[a] = std::min(lo[a], cell[3 * (size_t)j + a]);
-                        h2[a] = std::max(hi[a], cell[3 * (size_t)j + a]);
-                        vol *= (long long)h2[a] - l2[a] + KS;
-                    }
-                    if (vol > split) break;
-                    for (int a = 0; a < 3; ++a) {
-                        lo[a] = l2[a];
-                        hi[a] = h2[a];
-                    }
-                }
-                starts.push_back(j);
-                i = j;
-            }
-            // Inside each block, order the runs of particles that share a
-            // base cell so that few of them cross a 16-lane row: the scatter
-            // kernels sum a run across lanes inside a row, and a run cut by a
-            // row boundary leaves two lanes adding to the same LDS address in
-            // one instruction, which the LDS serialises.  Best fit: fill each
-            // row with the longest remaining run that fits its free lanes;
-            // when none fits, the longest remaining run starts there.
-            std::vector<int> tmp;
-            for (size_t b = 0; b + 1 < starts.size(); ++b) {
-                const int s0 = starts[b], s1 = starts[b + 1];
-                std::vector<std::pair<int, int>> runs;  // (length, start)
-                for (int r = s0; r < s1;) {
-                    int e = r + 1;
-                    while (e < s1 && cell[3 * (size_t)e] == cell[3 * (size_t)r] &&
-                           cell[3 * (size_t)e + 1] == cell[3 * (size_t)r + 1] &&
-                           cell[3 * (size_t)e + 2] == cell[3 * (size_t)r + 2])
-                        ++e;
-                    runs.emplace_back(e - r, r);
-                    r = e;
-                }
-                std::stable_sort(runs.begin(), runs.end(),
-                                 [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first > y.first; });
-                std::vector<char> used(runs.size(), 0);
-                tmp.clear();
-                for (size_t placed = 0; placed < runs.size(); ++placed) {
-                    const int room = 16 - (int)(tmp.size() % 16);
-                    size_t pick = runs.size();
-                    for (size_t q = 0; q < runs.size(); ++q)
-                        if (!used[q] && runs[q].first <= room) {
-                            pick = q;
-                            break;
-                        }
-                    if (pick == runs.size())
-                        for (size_t q = 0; q < runs.size(); ++q)
-                            if (!used[q]) {
-                                pick = q;
-                                break;
-                            }
-                    used[pick] = 1;
-                    for (int k = 0; k < runs[pick].first; ++k) tmp.push_back(perm[runs[pick].second + k]);
-                }
-                std::copy(tmp.begin(), tmp.end(), perm.begin() + s0);
-            }
+        if (!starts.empty()) {
             s->nblk = (int)starts.size() - 1;
             s->bst = (int*)s->dalloc(starts.size() * sizeof(int));
             HIPCHK(hipMemcpy(s->bst, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -1351,6 +1369,60 @@ int dmpm_render_gradient_info(const dmpm_sim* s, int* has, int* num_points, doub
         if (num_points) *num_points = s->rg_n;
         if (nF) *nF = s->has_rg ? s->rg_normF : 0.0;
         if (nx) *nx = s->has_rg ? s->rg_normx : 0.0;
+    });
+}
+
+int dmpm_sort_particles(dmpm_sim* s, int layer) {
+    return guard([&] {
+        check_sim(s);
+        if (layer < 0 || layer >= s->L) throw Error(DMPM_ERR_INDEX, "Layer index out of range.");
+        const void* st = s->state(layer);
+        std::vector<char> xh((size_t)3 * s->ld * s->esz);
+        HIPCHK(hipMemcpyAsync(xh.data(), st, xh.size(), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        const long long ld = s->ld;
+        const bool f64 = s->pr == PREC_F64;
+        std::vector<int> q, starts;
+        order_particles(s->cfg, s->n,
+                        [&](int i, int a) {
+                            return f64 ? ((const double*)xh.data())[a * ld + i] : (double)((const float*)xh.data())[a * ld + i];
+                        },
+                        q, starts);
+        int* dq = (int*)s->dalloc((size_t)s->n * sizeof(int));
+        HIPCHK(hipMemcpy(dq, q.data(), (size_t)s->n * sizeof(int), hipMemcpyHostToDevice));
+        void* tmp = s->dalloc(s->state_bytes());
+        // every per-particle array in the internal order: into tmp, then
+        // swapped in (whole states) or copied back
+        auto reorder = [&](void*& buf, int nc) {
+            if (!buf) return;
+            permute_planes(s->pr, buf, tmp, dq, s->n, nc, ld, s->stream);
+            if (nc == NSTATE) std::swap(buf, tmp);
+            else copy_elems(s->pr, buf, tmp, (long long)nc * ld, s->stream);
+        };
+        for (auto& p : s->layer) reorder(p, NSTATE);
+        for (auto& p : s->carry) reorder(p, NSTATE);
+        reorder(s->adj_last, NSTATE);
+        reorder(s->stat, NSTAT);
+        for (auto* v : {&s->ctl, &s->am, &s->av, &s->avmax, &s->gctl})
+            for (auto& p : *v) reorder(p, 9);
+        reorder(s->gctl_stop, 9);
+        reorder(s->rg, 12);
+        int* perm2 = (int*)s->dalloc((size_t)s->n * sizeof(int));
+        permute_ints(s->perm, perm2, dq, s->n, s->stream);
+        HIPCHK(hipStreamSynchronize(s->stream));
+        s->dfree((void*&)s->perm);
+        s->perm = perm2;
+        s->dfree(tmp);
+        s->dfree((void*&)dq);
+        s->dfree((void*&)s->bst);
+        s->nblk = 0;
+        if (!starts.empty()) {
+            s->nblk = (int)starts.size() - 1;
+            s->bst = (int*)s->dalloc(starts.size() * sizeof(int));
+            HIPCHK(hipMemcpy(s->bst, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        // recomputed segments were in the old order
+        s->seg_base = -1;
     });
 }
 

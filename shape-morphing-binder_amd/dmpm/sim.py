@@ -163,6 +163,11 @@ class Simulator:
     def synchronize(self):
         L.check(L.lib().dmpm_synchronize(self._h))
 
+    def sort_particles(self, layer: int):
+        """Reorder the device storage by the Hilbert order of `layer`'s
+        positions (results unchanged up to summation order)."""
+        L.check(L.lib().dmpm_sort_particles(self._h, int(layer)))
+
     # -- inputs ----------------------------------------------------------------
     def set_particles(self, x, v, F, C_, m, vol, lam, mu, dFc=None):
         n = int(np.shape(x)[0]) if not _is_torch(x) else int(x.shape[0])
