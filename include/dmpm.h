@@ -188,8 +188,10 @@ int dmpm_render_gradient_info(const dmpm_sim* sim, int* has, int* num_points, do
  * LDS windows and breaks the runs the scatters merge; sorting by a layer in
  * the middle of an episode halves the drift.  Every per-particle array (all
  * layers, controls, Adam state, gradients) is permuted; results do not
- * change beyond summation order, and getters keep the caller's order.  No
- * counterpart in the reference (its particles have no device order). */
+ * change beyond summation order, and getters keep the caller's order.
+ * Refused (DMPM_ERR_STATE) while a captured graph or batched episode of the
+ * sim is alive: those hold its device buffers.  No counterpart in the
+ * reference (its particles have no device order). */
 int dmpm_sort_particles(dmpm_sim* sim, int layer);
 /* promote_last_as_initial(carry_grid) (bind.cpp:282-291). */
 int dmpm_promote_last_as_initial(dmpm_sim* sim, int carry_grid);

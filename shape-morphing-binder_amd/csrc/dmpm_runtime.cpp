@@ -158,6 +158,10 @@ struct dmpm_sim {
     static constexpr int NKIND = 7;  // p2g, g2p, adj_scatter, adj_gather, grid clear, fused step, fused adjoint
     bool ktimer = false;
     bool capturing = false;  // inside dmpm_graph_capture: no timing events
+    // captured graphs and batched episodes alive that hold this sim's device
+    // pointers (shared with them, so that one destroyed after the sim still
+    // counts down safely)
+    std::shared_ptr<int> captures = std::make_shared<int>(0);
     std::vector<hipEvent_t> kpool;
     size_t knext = 0;
     std::vector<std::pair<int, size_t>> krec;  // (kind, index of start event)
@@ -1376,6 +1380,9 @@ int dmpm_sort_particles(dmpm_sim* s, int layer) {
     return guard([&] {
         check_sim(s);
         if (layer < 0 || layer >= s->L) throw Error(DMPM_ERR_INDEX, "Layer index out of range.");
+        if (*s->captures > 0)
+            throw Error(DMPM_ERR_STATE, "captured graphs or batched episodes hold this sim's buffers; destroy them "
+                                        "before sorting its particles");
         const void* st = s->state(layer);
         std::vector<char> xh((size_t)3 * s->ld * s->esz);
         HIPCHK(hipMemcpyAsync(xh.data(), st, xh.size(), hipMemcpyDeviceToHost, s->stream));
@@ -1779,12 +1786,27 @@ int dmpm_last_timing(const dmpm_sim* s, double* fwd_ms, double* bwd_ms) {
 // from layer `start` and its backward pass to layer `stop`, recorded on the
 // sim's own stream, forked from and joined back to an origin stream.  The
 // branches are independent, so the GPU runs the instances concurrently.
+// Holds one count on each captured sim while alive.
+struct CaptureHold {
+    std::vector<std::shared_ptr<int>> counts;
+    void hold(dmpm_sim** sims, int n) {
+        for (int i = 0; i < n; ++i) {
+            counts.push_back(sims[i]->captures);
+            ++*counts.back();
+        }
+    }
+    ~CaptureHold() {
+        for (auto& c : counts) --*c;
+    }
+};
+
 struct dmpm_graph {
     hipStream_t origin = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<hipEvent_t> events;
     int count = 0;
+    CaptureHold hold;
     ~dmpm_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -1845,6 +1867,7 @@ int dmpm_graph_capture(dmpm_sim** sims, int count, int start_layer, int stop_lay
         HIPCHK(end_err);
         g->graph = graph;
         HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        g->hold.hold(sims, count);
         *out = g.release();
     });
 }
@@ -1870,6 +1893,7 @@ int dmpm_graph_destroy(dmpm_graph* g) {
 // ---- batched episodes ---------------------------------------------------------
 struct dmpm_batch {
     int count = 0;
+    CaptureHold hold;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1999,6 +2023,7 @@ int dmpm_batch_capture(dmpm_sim** sims, int count, int start_layer, int stop_lay
         HIPCHK(launch_err);
         HIPCHK(end_err);
         HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->hold.hold(sims, count);
         *out = b.release();
     });
 }

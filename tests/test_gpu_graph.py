@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from dmpm.scenes import Body, make_scene
-from helpers import rel_err
+from helpers import perturbed_scene, physics_kwargs, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -57,3 +57,23 @@ def test_graph_replays_match_eager():
     g.close()
     for s in (a, b, c):
         s.close()
+
+
+def test_sort_refused_while_a_graph_holds_the_sim():
+    """dmpm_sort_particles moves the sim's buffers: it is refused while a
+    captured episode graph of the sim is alive, and allowed once it is
+    destroyed."""
+    from dmpm import EpisodeGraph, Physics, Simulator
+    from dmpm._lib import DmpmError
+    sc, s = perturbed_scene(n=800, dfc=0.0)
+    print(f"[sort_refused] n={sc.n}", flush=True)
+    sim = Simulator(sc.dims, sc.dx, sc.grid_min, physics=Physics(**physics_kwargs("plain", 1e-3)), precision="f32")
+    sim.set_state(dict(s, dFc=None))
+    sim.set_up(4)
+    sim.forward(0)
+    g = EpisodeGraph([sim], start=0, stop=0)
+    with pytest.raises(DmpmError):
+        sim.sort_particles(2)
+    g.close()
+    sim.sort_particles(2)
+    sim.close()
