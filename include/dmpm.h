@@ -90,7 +90,7 @@ typedef struct {
     int adjoint;            /* DMPM_ADJOINT_* */
     int precision;          /* DMPM_PREC_* */
     int checkpoint_interval;/* K: store every K-th layer and recompute the rest (1: store all) */
-    int lds_bytes;          /* LDS budget per block for the grid window (0: default 32 KiB with FP32
+    int lds_bytes;          /* LDS budget per block for the grid window (0: default 30 KiB with FP32
                                state, 48 KiB with FP64; max 64 KiB) */
     int block;              /* threads per block: 64, 128, 192 or 256 (0: default 256) */
     int unfused;            /* 1: separate G2P and P2G kernels instead of the fused forward step */

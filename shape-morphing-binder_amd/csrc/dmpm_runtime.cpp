@@ -217,9 +217,13 @@ struct dmpm_sim {
         Lc.bst = bst;
         Lc.nblk = nblk;
         // default LDS budget: the windows of 256-particle blocks cut at 512
-        // stencil nodes fit 32 KiB with FP32 state (the autotuner's pick on
-        // the 1M-particle workload); FP64 staging needs 48 KiB
-        Lc.lds_bytes = cfg.lds_bytes > 0 ? cfg.lds_bytes : (pr == PREC_F32 ? 32 : 48) * 1024;
+        // stencil nodes fit 30 KiB with FP32 state.  The fused step holds 94
+        // VGPRs (five waves per SIMD), so five blocks should share a CU's
+        // 160 KiB; with the kernels' few hundred bytes of static LDS, a
+        // 32 KiB (or 31 KiB) budget leaves room for four only (measured on
+        // the 1M-particle workload: 112 µs per step at 32 KiB, 105 µs at
+        // 30 KiB).  FP64 staging needs 48 KiB.
+        Lc.lds_bytes = cfg.lds_bytes > 0 ? cfg.lds_bytes : (pr == PREC_F32 ? 30 : 48) * 1024;
         Lc.block = cfg.block > 0 ? cfg.block : 256;
         return Lc;
     }
