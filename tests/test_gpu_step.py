@@ -239,3 +239,49 @@ def test_sort_particles_keeps_results(K):
         assert e < 1e-10, f"{k}: {e:.3e}"
     assert rel_err(sim.get(3, "x", dtype=np.float64), mid["x"]) < 1e-12
     sim.close()
+
+
+# (physics, precision, LDS budget, block size): 64-, 128- and 192-thread
+# blocks; LDS budgets so small that most windows (1 KiB: every window of more
+# than 32 FP64 nodes) or all of them (2 KiB with 64-node cubic stencils and
+# seven staged values) fall back to global atomics.
+SHAPES = [("plain", "f64", 0, 64), ("plain", "f64", 0, 192), ("plain", "f64", 1024, 256),
+          ("reference", "f64", 2048, 128), ("plain", "f32", 1024, 256), ("reference", "f32", 0, 64)]
+
+
+@pytest.mark.parametrize("name,prec,lds,block", SHAPES)
+def test_launch_shapes_match_oracle(name, prec, lds, block):
+    """A 3-step episode (the fused forward step and the fused adjoint) under
+    other block sizes and LDS budgets, including windows too large for the
+    budget, whose particles scatter and gather through global memory:
+    states and gradients must match the oracle's."""
+    from dmpm import Physics, Simulator
+    sc, s = perturbed_scene(dfc=0.0)
+    T = 3
+    adj = random_adjoint(sc.n, seed=13)
+    orc = Oracle(prec)
+    prm = oracle_params(name, sc, DT)
+    layers = orc.rollout(prm, dict(s, dFc=None), T, [np.zeros((sc.n, 3, 3))] * T)
+    g0, _ = orc.rollout_grad(prm, layers, adj)
+    print(f"[launch_shapes] physics={name} prec={prec} lds={lds} block={block} T={T} n={sc.n}", flush=True)
+    sim = Simulator(sc.dims, sc.dx, sc.grid_min, physics=Physics(**physics_kwargs(name, DT)), precision=prec,
+                    lds_bytes=lds, block=block)
+    sim.set_state(dict(s, dFc=None))
+    sim.set_up(T + 1)
+    sim.window_stats(reset=True)
+    sim.forward(0)
+    sim.set_last_grad(x=adj["x"], v=adj["v"], F=adj["F"], C_=adj["C"])
+    sim.backward(0)
+    in_lds, fell_back = sim.window_stats()
+    if lds:
+        assert fell_back > 0, "the small LDS budget should send windows to global atomics"
+    else:
+        assert in_lds > 0 and fell_back == 0, (in_lds, fell_back)
+    dt = np.float64 if prec == "f64" else np.float32
+    tol_s, tol_g = (1e-10, 1e-9) if prec == "f64" else (1e-3, 1e-2)
+    for k in ("x", "v", "F", "C"):
+        e = rel_err(sim.get(T, k, dtype=dt), layers[T][k])
+        assert e < tol_s, f"{k}: rel err {e:.3e}"
+        e = rel_err(sim.get_grad(0, k, dtype=dt), g0[k])
+        assert e < tol_g, f"d{k}: rel err {e:.3e}"
+    sim.close()
