@@ -59,6 +59,11 @@ struct Launch {
     const int* bst;
     int nblk;
     unsigned long long* wstat;  // window counters: [0] blocks in LDS, [1] blocks falling back (or null)
+    // fused step and fused adjoint: a buffer that a later launch on the
+    // stream accumulates into, cleared by this launch's blocks after their
+    // own work (16-byte aligned, zero_bytes a multiple of 16), or null
+    void* zero;
+    long long zero_bytes;
     int lds_bytes;     // LDS budget per block for the window
     int block;         // threads per block (multiple of 64)
 };

@@ -23,6 +23,7 @@
 #include <climits>
 #include <cstdint>
 #include <cstring>
+#include <stdexcept>
 
 #include <hipcub/hipcub.hpp>
 
@@ -56,8 +57,24 @@ struct KP {
     const int* bst; // particle block table (nblk + 1 starts) or null
     int nblk;
     unsigned long long* wstat;
+    uint4* zero;       // cleared by this launch for a later one (or null)
+    long long zero16;  // its 16-byte chunks
+    int zblk;          // logical blocks sharing the clear
     int cap;
 };
+
+static inline unsigned grid_blocks(int n, int block) {
+    unsigned nb = (unsigned)((n + block - 1) / block);
+    if (nb == 0) nb = 1;
+    return (nb + 7u) & ~7u;
+}
+
+// Launch grid of a window kernel: the block table's blocks, or n / block.
+static inline unsigned launch_blocks(const Launch& L) {
+    if (!L.bst) return grid_blocks(L.n, L.block);
+    const unsigned nb = (unsigned)std::max(L.nblk, 1);
+    return (nb + 7u) & ~7u;
+}
 
 template <typename T>
 static KP<T> make_kp(const Launch& L, int narrays) {
@@ -99,6 +116,9 @@ static KP<T> make_kp(const Launch& L, int narrays) {
     k.bst = L.bst;
     k.nblk = L.nblk;
     k.wstat = L.wstat;
+    k.zero = (uint4*)L.zero;
+    k.zero16 = L.zero ? L.zero_bytes / 16 : 0;
+    k.zblk = (int)launch_blocks(L);
     k.cap = narrays > 0 ? (int)(L.lds_bytes / narrays) : 0;  // narrays = LDS bytes per window node
     return k;
 }
@@ -132,17 +152,17 @@ DMPM_DI bool batch_block(const int* bstart, int count, int& inst, int& lblock) {
     return true;
 }
 
-static inline unsigned grid_blocks(int n, int block) {
-    unsigned nb = (unsigned)((n + block - 1) / block);
-    if (nb == 0) nb = 1;
-    return (nb + 7u) & ~7u;
-}
-
-// Launch grid of a window kernel: the block table's blocks, or n / block.
-static inline unsigned launch_blocks(const Launch& L) {
-    if (!L.bst) return grid_blocks(L.n, L.block);
-    const unsigned nb = (unsigned)std::max(L.nblk, 1);
-    return (nb + 7u) & ~7u;
+// A launch's blocks clear, after their own work, a buffer that a later
+// launch on the same stream accumulates into, so that the clear takes no
+// launch of its own and its stores overlap the launch's arithmetic: logical
+// block `lblock` of a.zblk clears its share of the 16-byte chunks.
+template <typename T>
+DMPM_DI void clear_share(const KP<T>& a, int lblock) {
+    if (!a.zero) return;
+    const long long per = (a.zero16 + a.zblk - 1) / a.zblk;
+    const long long c1 = min(a.zero16, (long long)(lblock + 1) * per);
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for (long long c = (long long)lblock * per + threadIdx.x; c < c1; c += blockDim.x) a.zero[c] = z;
 }
 
 // Particles [i0, i1) of logical block `lblock` (block-uniform).  Logical
@@ -916,7 +936,9 @@ DMPM_DI void step_block(const KP<T>& a, int lblock, double* smem_d) {
 template <typename T, int ORDER, int MODEL>
 __global__ void __launch_bounds__(256) k_step(KP<T> a) {
     extern __shared__ double smem_d[];
-    step_block<T, ORDER, MODEL>(a, logical_block(), smem_d);
+    const int lb = logical_block();
+    step_block<T, ORDER, MODEL>(a, lb, smem_d);
+    clear_share(a, lb);
 }
 
 template <typename T, int ORDER, int MODEL>
@@ -926,6 +948,7 @@ __global__ void __launch_bounds__(256) k_step_batch(const KP<T>* tab, const int*
     if (!batch_block(bstart, count, inst, lblock)) return;
     const KP<T> a = tab[inst];
     step_block<T, ORDER, MODEL>(a, lblock, smem_d);
+    clear_share(a, lblock);
 }
 
 // Phase 1 of the adjoint (BackPropagation.cpp:22-32).  REF reproduces the
@@ -1435,7 +1458,9 @@ __global__ void __launch_bounds__(256) k_adj_gather(KP<T> a) {
 template <typename T, int ORDER, int MODEL, bool REF>
 __global__ void __launch_bounds__(256) k_adj_fused(KP<T> a, KP<T> b) {
     extern __shared__ double smem_d[];
-    adj_gather_block<T, ORDER, MODEL, REF, true>(a, &b, logical_block(), smem_d);
+    const int lb = logical_block();
+    adj_gather_block<T, ORDER, MODEL, REF, true>(a, &b, lb, smem_d);
+    clear_share(a, lb);
 }
 
 template <typename T, int ORDER, int MODEL, bool REF>
@@ -1457,6 +1482,7 @@ __global__ void __launch_bounds__(256) k_adj_fused_batch(const KP<T>* tab, const
     const KP<T> a = tab[2 * inst];
     const KP<T> b = tab[2 * inst + 1];
     adj_gather_block<T, ORDER, MODEL, REF, true>(a, &b, lblock, smem_d);
+    clear_share(a, lblock);
 }
 
 // ---------------------------------------------------------------------------
@@ -2044,7 +2070,9 @@ size_t kp_bytes(Prec pr) { return pr == PREC_F32 ? sizeof(KP<float>) : sizeof(KP
 
 template <typename T>
 static void build_kp_t(int kind, const Launch& L, const Launch* L2, void* dst) {
-    const KP<T> k = make_kp<T>(L, node_bytes<T>(kind));
+    KP<T> k = make_kp<T>(L, node_bytes<T>(kind));
+    k.zblk = instance_blocks(L);  // the instance's blocks of a batched launch
+    if (k.zero && k.zblk <= 0) throw std::runtime_error("a clear over an instance without blocks");
     std::memcpy(dst, &k, sizeof(k));
     if (kind == OP_ADJ_FUSED) {
         const KP<T> k2 = make_kp<T>(*L2, 3 * (int)sizeof(Acc));

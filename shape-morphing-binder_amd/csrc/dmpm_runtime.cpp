@@ -136,6 +136,7 @@ struct dmpm_sim {
     }
     void* ggv = nullptr;   // 3 planes
     void* ggv2 = nullptr;  // 3 planes: dL/dv of the previous step, written by the fused adjoint
+    void* ggv3 = nullptr;  // 3 planes: the dL/dv grid a fused adjoint launch clears for the next one
     void* carry[2] = {nullptr, nullptr};
     int carry_cur = 0;
     int carry_layer = -1;   // layer whose adjoint the current carry holds
@@ -412,7 +413,11 @@ struct dmpm_sim {
     }
 
     // Layer t (`in`, grid `g`) -> layer t+1 (`out`) and its grid `g1`.
-    void fused_step(const void* in, int t, void* g, void* out, void* g1) {
+    // g1_clear: g1 was cleared by the previous launch.  ahead: the grid of
+    // the next step, cleared by this launch's blocks when the fused kernel
+    // runs (returns whether it did).
+    bool fused_step(const void* in, int t, void* g, void* out, void* g1, bool g1_clear = false,
+                    void* ahead = nullptr) {
         Launch Lc = launch();
         Lc.st = in;
         Lc.stat = stat;
@@ -424,10 +429,15 @@ struct dmpm_sim {
         if (cfg.unfused || cfg.deterministic) {
             op_kernel(OP_G2P, Lc);
             p2g_into(out, t + 1, g1);
-        } else {
-            op_zero(g1, 4 * gld);
-            op_kernel(OP_STEP, Lc);
+            return false;
         }
+        if (!g1_clear) op_zero(g1, 4 * gld);
+        if (ahead) {
+            Lc.zero = ahead;
+            Lc.zero_bytes = 4 * gld * (long long)esz;
+        }
+        op_kernel(OP_STEP, Lc);
+        return ahead != nullptr;
     }
 
     void forward(int start, bool zero_controls = true) {
@@ -452,6 +462,10 @@ struct dmpm_sim {
             p2g_into(cur, start, ga);
             if (kg) gvalid[start] = 1;
             int sc = 0;
+            // With every layer's grid kept, step t's launch clears the grid
+            // of layer t+2 (neither read nor written by it), which step t+1
+            // then accumulates into without a clear of its own.
+            bool gb_clear = false;
             for (int t = start; t < L - 1; ++t) {
                 const bool in_last = keep && t + 1 >= c_last;
                 void* out;
@@ -467,7 +481,8 @@ struct dmpm_sim {
                     void* gb = kg        ? lgrid[t + 1]
                                : in_last ? seg_grid[t + 1 - c_last]
                                          : (ga == grid ? grid2 : grid);
-                    fused_step(cur, t, ga, out, gb);
+                    void* ahead = kg && t + 2 < L - 1 ? lgrid[t + 2] : nullptr;
+                    gb_clear = fused_step(cur, t, ga, out, gb, gb_clear, ahead);
                     if (kg) gvalid[t + 1] = 1;
                     ga = gb;
                 } else {
@@ -501,9 +516,13 @@ struct dmpm_sim {
         if (!capturing && !rec) HIPCHK(hipEventRecord(ev[2], stream));
         // dL/dv grid of the step being processed, and the other one; a fused
         // launch (gather of step t + scatter of step t-1) leaves step t-1's
-        // dL/dv in the other grid, so the next step skips its scatter
+        // dL/dv in the other grid, so the next step skips its scatter.  A
+        // fused launch also clears the third grid (last read by the previous
+        // step's gather), which the next fused launch scatters into.
         void* gcur = ggv;
         void* gnext = ggv2;
+        void* gfree = ggv3;
+        bool gnext_clear = false;
         bool scattered = false;
         for (int t = L - 2; t >= stop; --t) {
             const void* st;
@@ -569,13 +588,23 @@ struct dmpm_sim {
                 Lp.ctl = ctl[t - 1];
                 Lp.ggv = gnext;
                 Lp.nxt = st;
-                op_zero(gnext, 3 * gld);
+                if (!gnext_clear) op_zero(gnext, 3 * gld);
+                // the next step fuses only if it has a previous step
+                gnext_clear = t - 2 >= stop;
+                if (gnext_clear) {
+                    Lc.zero = gfree;
+                    Lc.zero_bytes = 3 * gld * (long long)esz;
+                }
                 op_adj_fused(Lc, Lp);
-                std::swap(gcur, gnext);
+                void* read = gcur;
+                gcur = gnext;
+                gnext = gfree;
+                gfree = read;
                 scattered = true;
             } else {
                 op_kernel(OP_ADJ_GATHER, Lc);
                 scattered = false;
+                gnext_clear = false;
             }
             if (t == stop && want_gctl[t]) {
                 if (!gctl[t]) gctl[t] = dalloc(ctl_bytes());
@@ -844,6 +873,7 @@ int dmpm_create(const dmpm_config* cfg, dmpm_sim** out) {
         s->grid2 = s->dalloc(4 * s->gld * s->esz);
         s->ggv = s->dalloc(3 * s->gld * s->esz);
         s->ggv2 = s->dalloc(3 * s->gld * s->esz);
+        s->ggv3 = s->dalloc(3 * s->gld * s->esz);
         s->target_m = s->dalloc(s->gld * s->esz);
         s->dldm = s->dalloc(s->gld * s->esz);
         s->zero(s->target_m, s->gld);
