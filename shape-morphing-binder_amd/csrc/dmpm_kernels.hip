@@ -155,14 +155,19 @@ DMPM_DI bool batch_block(const int* bstart, int count, int& inst, int& lblock) {
 // A launch's blocks clear, after their own work, a buffer that a later
 // launch on the same stream accumulates into, so that the clear takes no
 // launch of its own and its stores overlap the launch's arithmetic: logical
-// block `lblock` of a.zblk clears its share of the 16-byte chunks.
+// block `lblock` of a.zblk clears its share of the 16-byte chunks, with
+// non-temporal stores (the lines are not read again by this launch; measured
+// slightly faster than plain stores, and than clearing before the block's
+// own work).
 template <typename T>
 DMPM_DI void clear_share(const KP<T>& a, int lblock) {
     if (!a.zero) return;
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
     const long long per = (a.zero16 + a.zblk - 1) / a.zblk;
     const long long c1 = min(a.zero16, (long long)(lblock + 1) * per);
-    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-    for (long long c = (long long)lblock * per + threadIdx.x; c < c1; c += blockDim.x) a.zero[c] = z;
+    u32x4* z = reinterpret_cast<u32x4*>(a.zero);
+    for (long long c = (long long)lblock * per + threadIdx.x; c < c1; c += blockDim.x)
+        __builtin_nontemporal_store((u32x4)(0u), z + c);
 }
 
 // Particles [i0, i1) of logical block `lblock` (block-uniform).  Logical

@@ -356,8 +356,11 @@ struct dmpm_sim {
         seg_base = -1;
         seg[0] = layer[c];
         p2g_into(seg[0], c, seg_grid[0]);
+        // step j also clears the grid step j+1 accumulates into
+        bool g1_clear = false;
         for (int j = c; j < top; ++j) {
-            fused_step(seg[j - c], j, seg_grid[j - c], seg[j - c + 1], seg_grid[j - c + 1]);
+            void* ahead = j + 2 <= top ? seg_grid[j - c + 2] : nullptr;
+            g1_clear = fused_step(seg[j - c], j, seg_grid[j - c], seg[j - c + 1], seg_grid[j - c + 1], g1_clear, ahead);
             fresh[j + 1] = 1;
         }
         seg_base = c;
@@ -462,9 +465,16 @@ struct dmpm_sim {
             p2g_into(cur, start, ga);
             if (kg) gvalid[start] = 1;
             int sc = 0;
-            // With every layer's grid kept, step t's launch clears the grid
-            // of layer t+2 (neither read nor written by it), which step t+1
-            // then accumulates into without a clear of its own.
+            // Where step t+1 accumulates into a grid of its own (every
+            // layer's grid kept, or the segment buffers of the last segment)
+            // step t's launch clears it (neither read nor written by step t),
+            // and step t+1 needs no clear launch.  Not with the two grids
+            // used in turn, one of which step t reads.
+            auto target = [&](int u) -> void* {
+                if (kg) return lgrid[u + 1];
+                if (keep && u + 1 >= c_last) return seg_grid[u + 1 - c_last];
+                return nullptr;
+            };
             bool gb_clear = false;
             for (int t = start; t < L - 1; ++t) {
                 const bool in_last = keep && t + 1 >= c_last;
@@ -481,7 +491,7 @@ struct dmpm_sim {
                     void* gb = kg        ? lgrid[t + 1]
                                : in_last ? seg_grid[t + 1 - c_last]
                                          : (ga == grid ? grid2 : grid);
-                    void* ahead = kg && t + 2 < L - 1 ? lgrid[t + 2] : nullptr;
+                    void* ahead = t + 2 < L - 1 ? target(t + 1) : nullptr;
                     gb_clear = fused_step(cur, t, ga, out, gb, gb_clear, ahead);
                     if (kg) gvalid[t + 1] = 1;
                     ga = gb;
