@@ -199,32 +199,48 @@ struct Win {
     bool any;
 };
 
-// Wave-wide min (MAX = false) or max of v, through DPP: quad swaps, row
-// mirrors, then row broadcasts 15 and 31 fold the four 16-lane rows into
-// lane 63 (no LDS traffic, unlike a shuffle).
-template <bool MAX>
-DMPM_DI int wave_extreme(int v) {
-    auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));  // row_bcast:15 into rows 1, 3
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));  // row_bcast:31 into rows 2, 3
-    return __builtin_amdgcn_readlane(v, 63);
+// Wave-wide min of a0, a1, a2 and max of b0, b1, b2, left in lane 63,
+// through DPP: quad swaps, half-row and row mirrors, then row broadcasts 15
+// and 31 fold the four 16-lane rows (rows left out by the row mask keep
+// their value).  The min / max takes its first operand through the DPP
+// modifier itself, one instruction per step and value, and the six
+// reductions are interleaved so that no DPP operand was written by one of
+// the five instructions before it.  All lanes must be active.
+#define DMPM_BOX_STEP(CTRL)                            \
+    "v_min_i32_dpp %0, %0, %0 " CTRL "\n"             \
+    "v_min_i32_dpp %1, %1, %1 " CTRL "\n"             \
+    "v_min_i32_dpp %2, %2, %2 " CTRL "\n"             \
+    "v_max_i32_dpp %3, %3, %3 " CTRL "\n"             \
+    "v_max_i32_dpp %4, %4, %4 " CTRL "\n"             \
+    "v_max_i32_dpp %5, %5, %5 " CTRL "\n"
+DMPM_DI void wave_box(int& a0, int& a1, int& a2, int& b0, int& b1, int& b2) {
+    asm volatile(
+        "s_nop 4\n"
+        DMPM_BOX_STEP("quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf")
+        DMPM_BOX_STEP("quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf")
+        DMPM_BOX_STEP("row_half_mirror row_mask:0xf bank_mask:0xf")
+        DMPM_BOX_STEP("row_mirror row_mask:0xf bank_mask:0xf")
+        DMPM_BOX_STEP("row_bcast:15 row_mask:0xa bank_mask:0xf")
+        DMPM_BOX_STEP("row_bcast:31 row_mask:0xc bank_mask:0xf")
+        "s_nop 4"
+        : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(b0), "+v"(b1), "+v"(b2));
 }
+#undef DMPM_BOX_STEP
 
 // Bounding box of the block's stencils (base nodes b, width KS).  Every
 // thread of the block must call this (all lanes of each wave active).
 template <int KS>
 DMPM_DI Win block_window(bool valid, int b0, int b1, int b2, int cap) {
     __shared__ int s_red[6][16];
-    const int v0 = wave_extreme<false>(valid ? b0 : INT_MAX);
-    const int v1 = wave_extreme<false>(valid ? b1 : INT_MAX);
-    const int v2 = wave_extreme<false>(valid ? b2 : INT_MAX);
-    const int w0 = wave_extreme<true>(valid ? b0 : INT_MIN);
-    const int w1 = wave_extreme<true>(valid ? b1 : INT_MIN);
-    const int w2 = wave_extreme<true>(valid ? b2 : INT_MIN);
+    int v0 = valid ? b0 : INT_MAX, v1 = valid ? b1 : INT_MAX, v2 = valid ? b2 : INT_MAX;
+    int w0 = valid ? b0 : INT_MIN, w1 = valid ? b1 : INT_MIN, w2 = valid ? b2 : INT_MIN;
+    wave_box(v0, v1, v2, w0, w1, w2);
+    v0 = __builtin_amdgcn_readlane(v0, 63);
+    v1 = __builtin_amdgcn_readlane(v1, 63);
+    v2 = __builtin_amdgcn_readlane(v2, 63);
+    w0 = __builtin_amdgcn_readlane(w0, 63);
+    w1 = __builtin_amdgcn_readlane(w1, 63);
+    w2 = __builtin_amdgcn_readlane(w2, 63);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) {
         s_red[0][wave] = v0; s_red[1][wave] = v1; s_red[2][wave] = v2;
