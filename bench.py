@@ -389,12 +389,17 @@ def main():
     wall = time.perf_counter() - t0
     value = sc.n * T * args.steps / wall
     ms_per_step = 1e3 * wall / args.steps
-    sim.window_stats(reset=True)
     sim.kernel_timer(12 * T + 64)
     run_episodes(sim, 1)
     sim.synchronize()
     kstats = sim.kernel_stats()
     sim.kernel_timer(0)
+    # window statistics from one more episode (counting adds an atomic per block)
+    sim.window_stats(reset=True)
+    sim.count_windows(True)
+    run_episodes(sim, 1)
+    sim.synchronize()
+    sim.count_windows(False)
     win_lds, win_fb = sim.window_stats()
     mem_cur, mem_peak = sim.memory()
     log(f"[bench] f32 K={K}: {ms_per_step:.1f} ms/episode, {value:.4g} particle-steps/s, peak {mem_peak / 1e9:.2f} GB")

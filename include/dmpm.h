@@ -337,9 +337,15 @@ int dmpm_last_timing(const dmpm_sim* sim, double* fwd_ms, double* bwd_ms);
 #define DMPM_KERNEL_ADJ_FUSED 6  /* adjoint gather of step t + adjoint scatter of step t-1 in one launch */
 int dmpm_kernel_timer(dmpm_sim* sim, int capacity);
 int dmpm_kernel_stats(dmpm_sim* sim, int kind, double* total_ms, int* count);
-/* Scatter windows since the last reset: blocks whose stencil box was held
- * in LDS, and blocks that fell back to global atomics. */
+/* Scatter windows counted since the last reset: blocks whose stencil box
+ * was held in LDS, and blocks that fell back to global atomics. */
 int dmpm_window_stats(dmpm_sim* sim, long long* lds_blocks, long long* fallback_blocks, int reset);
+/* Window counting on (1) or off (0, the default).  Counting costs one
+ * device-scope atomic per block of every scatter launch (about 2 % of an
+ * episode at 1M particles); launches recorded into a graph or a batched
+ * episode keep the setting they were recorded with.  No counterpart in the
+ * reference (a diagnostic of this library's LDS windows). */
+int dmpm_window_counters(dmpm_sim* sim, int on);
 /* Device bytes currently allocated by the sim, and the peak. */
 int dmpm_memory(const dmpm_sim* sim, size_t* current, size_t* peak);
 

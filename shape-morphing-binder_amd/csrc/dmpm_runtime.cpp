@@ -135,6 +135,7 @@ struct dmpm_sim {
         return det;
     }
     void* ggv = nullptr;   // 3 planes
+    bool count_windows = false;  // window counters (wstat) updated by the scatter launches
     void* ggv2 = nullptr;  // 3 planes: dL/dv of the previous step, written by the fused adjoint
     void* ggv3 = nullptr;  // 3 planes: the dL/dv grid a fused adjoint launch clears for the next one
     void* carry[2] = {nullptr, nullptr};
@@ -214,7 +215,7 @@ struct dmpm_sim {
         Lc.n = n;
         Lc.ld = ld;
         Lc.gld = gld;
-        Lc.wstat = wstat;
+        Lc.wstat = count_windows ? wstat : nullptr;
         Lc.bst = bst;
         Lc.nblk = nblk;
         // default LDS budget: the windows of 256-particle blocks cut at 512
@@ -2123,6 +2124,13 @@ int dmpm_kernel_stats(dmpm_sim* s, int kind, double* total_ms, int* count) {
         }
         if (total_ms) *total_ms = tot;
         if (count) *count = c;
+    });
+}
+
+int dmpm_window_counters(dmpm_sim* s, int on) {
+    return guard([&] {
+        check_sim(s);
+        s->count_windows = on != 0;
     });
 }
 

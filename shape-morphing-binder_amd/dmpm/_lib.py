@@ -140,6 +140,7 @@ def lib() -> C.CDLL:
         "dmpm_memory": (i, [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
         "dmpm_kernel_timer": (i, [vp, i]),
         "dmpm_window_stats": (i, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), i]),
+        "dmpm_window_counters": (i, [vp, i]),
         "dmpm_graph_capture": (i, [C.POINTER(vp), i, i, i, C.POINTER(vp)]),
         "dmpm_graph_launch": (i, [vp]),
         "dmpm_graph_synchronize": (i, [vp]),
@@ -178,7 +179,7 @@ EXPORTED = [
     "dmpm_set_control", "dmpm_request_control_grad", "dmpm_get_control_grad", "dmpm_control_grad_norm",
     "dmpm_descend_adam", "dmpm_optimize_single_timestep", "dmpm_run_optimization", "dmpm_p2g_mass",
     "dmpm_calculate_volumes", "dmpm_lame_parameters", "dmpm_last_timing", "dmpm_memory", "dmpm_kernel_timer",
-    "dmpm_kernel_stats", "dmpm_window_stats", "dmpm_graph_capture", "dmpm_graph_launch", "dmpm_graph_synchronize", "dmpm_graph_destroy",
+    "dmpm_kernel_stats", "dmpm_window_stats", "dmpm_window_counters", "dmpm_graph_capture", "dmpm_graph_launch", "dmpm_graph_synchronize", "dmpm_graph_destroy",
     "dmpm_get_config", "dmpm_set_adam_state", "dmpm_file_info", "dmpm_write_file", "dmpm_read_file",
     "dmpm_write_layer", "dmpm_read_particles", "dmpm_winding_numbers",
     "dmpm_batch_capture", "dmpm_batch_launch", "dmpm_batch_synchronize", "dmpm_batch_destroy",

@@ -45,6 +45,7 @@ def tune(scene, physics: Physics, precision: str = "f32", steps: int = 20, check
                 times.append(time.perf_counter() - t0)
             dt = min(times[1:])
             sim.window_stats(reset=True)
+            sim.count_windows(True)
             sim.forward(0, keep_controls=True)
             lds_blocks, fb_blocks = sim.window_stats()
             sim.close()

@@ -331,8 +331,14 @@ class Simulator:
             out[name] = (ms.value, c.value)
         return out
 
+    def count_windows(self, on: bool = True) -> None:
+        """Window counting on or off (off by default: one device-scope atomic
+        per block of every scatter launch)."""
+        L.check(L.lib().dmpm_window_counters(self._h, 1 if on else 0))
+
     def window_stats(self, reset: bool = False):
-        """(scatter blocks held in LDS, blocks that fell back to global atomics)."""
+        """(scatter blocks held in LDS, blocks that fell back to global
+        atomics) counted while count_windows is on."""
         a, b = C.c_longlong(), C.c_longlong()
         L.check(L.lib().dmpm_window_stats(self._h, C.byref(a), C.byref(b), 1 if reset else 0))
         return a.value, b.value

@@ -185,6 +185,7 @@ def test_block_table_cuts(name):
         sim.set_state(dict(s, dFc=None))
         sim.set_up(T + 1)
         sim.window_stats(reset=True)
+        sim.count_windows(True)
         sim.forward(0)
         sim.set_last_grad(x=adj["x"], v=adj["v"], F=adj["F"], C_=adj["C"])
         sim.backward(0)
@@ -269,6 +270,7 @@ def test_launch_shapes_match_oracle(name, prec, lds, block):
     sim.set_state(dict(s, dFc=None))
     sim.set_up(T + 1)
     sim.window_stats(reset=True)
+    sim.count_windows(True)
     sim.forward(0)
     sim.set_last_grad(x=adj["x"], v=adj["v"], F=adj["F"], C_=adj["C"])
     sim.backward(0)
