@@ -1,8 +1,9 @@
 """Batched episodes: every kernel of many sims' episodes as one launch over
 all instances (one hipGraph).  Replays must reproduce each instance's eager
 episode: the 64 uneven instances of the batched workload in FP64 with a
-control and its gradient, the reference physics in FP32 on a subset, and a
-mismatched episode structure is rejected."""
+control and its gradient, the reference physics in FP32 on a subset, the
+kept tape (K = 1) in FP32 on a subset, and a mismatched episode structure is
+rejected."""
 import numpy as np
 import pytest
 
@@ -68,6 +69,16 @@ def test_batched_reference_physics_fp32_matches_eager():
     from dmpm.scenes import config4_instances
     _check(config4_instances()[:8], "f32", Physics.reference(dt=2e-4, drag=0.0, smoothing=0.0), T=5, K=2,
            control_layer=None, tol=1e-4)
+
+
+def test_batched_kept_tape_fp32_matches_eager():
+    """Every layer's grid kept (K = 1): each fused step of the batched launch
+    clears the grid two layers ahead and each fused adjoint launch the next
+    dL/dv grid, per instance; two replays must both match the eager
+    episodes."""
+    from dmpm import Physics
+    from dmpm.scenes import config4_instances
+    _check(config4_instances()[:16], "f32", Physics.plain(dt=2.5e-4), T=7, K=1, control_layer=3, tol=1e-4)
 
 
 def test_mismatched_episodes_are_rejected():
