@@ -337,8 +337,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(value is the one GPU's throughput).", file=sys.stderr)
+        print(f"bench.py: the stepper runs on one GPU (sharding across GPUs is designed, not built); "
+              f"--gpus {args.gpus} is not supported (value is the one GPU's throughput).", file=sys.stderr)
         sys.exit(2)
 
     import torch  # (one HIP runtime for torch and libdmpm)
