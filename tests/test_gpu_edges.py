@@ -1,0 +1,75 @@
+"""Edge cases of the particle set against the oracle: one particle, counts
+that leave a partial wave or a partial block, a body inside the sticky-wall
+band next to a grid corner; and an empty particle set is refused."""
+import numpy as np
+import pytest
+
+from helpers import oracle_params, physics_kwargs, random_adjoint, rel_err
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+DT = 1e-3
+
+
+def _subset(sc, k, seed=7):
+    """The first k particles of scene sc, with perturbed F and C."""
+    s = {key: np.asarray(val)[:k].copy() for key, val in sc.state().items()}
+    rng = np.random.default_rng(seed)
+    s["F"] = s["F"] + 0.05 * rng.normal(size=s["F"].shape)
+    s["C"] = 0.5 * rng.normal(size=s["C"].shape)
+    return s
+
+
+def _episode_matches(sc, s, name, T=3):
+    from dmpm import Physics, Simulator
+    n = s["x"].shape[0]
+    adj = random_adjoint(n, seed=3)
+    orc = Oracle("f64")
+    prm = oracle_params(name, sc, DT)
+    layers = orc.rollout(prm, dict(s, dFc=None), T, [np.zeros((n, 3, 3))] * T)
+    g0, _ = orc.rollout_grad(prm, layers, adj)
+    sim = Simulator(sc.dims, sc.dx, sc.grid_min, physics=Physics(**physics_kwargs(name, DT)), precision="f64")
+    sim.set_state(dict(s, dFc=None))
+    sim.set_up(T + 1)
+    sim.forward(0)
+    sim.set_last_grad(x=adj["x"], v=adj["v"], F=adj["F"], C_=adj["C"])
+    sim.backward(0)
+    for k in ("x", "v", "F", "C"):
+        e = rel_err(sim.get(T, k, dtype=np.float64), layers[T][k])
+        assert e < 1e-10, f"{k}: rel err {e:.3e}"
+        e = rel_err(sim.get_grad(0, k, dtype=np.float64), g0[k])
+        assert e < 1e-9, f"d{k}: rel err {e:.3e}"
+    sim.close()
+
+
+@pytest.mark.parametrize("n", [1, 65, 257])
+@pytest.mark.parametrize("name", ["plain", "reference"])
+def test_small_and_ragged_counts(name, n):
+    from dmpm.scenes import Body, make_scene
+    sc = make_scene(Body((0.5, 0.5, 0.5), (0.2, 0.2, 0.2)), 1500, 16, 11, 1e4, 0.3, velocity="sine", amplitude=0.3)
+    print(f"[edges] physics={name} n={n}", flush=True)
+    _episode_matches(sc, _subset(sc, n), name)
+
+
+@pytest.mark.parametrize("name", ["plain", "reference", "quadratic_fcr"])
+def test_body_in_the_wall_band(name):
+    """A body 0.12-0.28 from three faces of the unit box (16 cells): part of
+    it lies within the sticky-wall cells, and its windows reach the grid's
+    first nodes."""
+    from dmpm.scenes import Body, make_scene
+    sc = make_scene(Body((0.2, 0.8, 0.2), (0.08, 0.08, 0.08)), 800, 16, 5, 1e4, 0.3, velocity="sine", amplitude=0.5)
+    assert sc.x.min() > 0.1 and sc.x.max() < 0.9
+    print(f"[edges] wall band physics={name} n={sc.n}", flush=True)
+    _episode_matches(sc, _subset(sc, sc.n), name)
+
+
+def test_empty_particle_set_is_refused():
+    from dmpm import Physics, Simulator
+    from dmpm._lib import DmpmError
+    sim = Simulator((17, 17, 17), 1 / 16, (0.0, 0.0, 0.0), physics=Physics.plain(dt=DT), precision="f32")
+    z = np.zeros((0, 3))
+    with pytest.raises(DmpmError):
+        sim.set_state(dict(x=z, v=z, F=np.zeros((0, 3, 3)), C=np.zeros((0, 3, 3)), m=np.zeros(0), vol=np.zeros(0),
+                           lam=np.zeros(0), mu=np.zeros(0)))
+    sim.close()
