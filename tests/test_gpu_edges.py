@@ -1,6 +1,7 @@
-"""Edge cases of the particle set against the oracle: one particle, counts
-that leave a partial wave or a partial block, a body inside the sticky-wall
-band next to a grid corner; and an empty particle set is refused."""
+"""Edge cases against the oracle: one particle, counts that leave a partial
+wave or a partial block, a body inside the sticky-wall band next to a grid
+corner, a box-shaped grid with its origin off zero; and an empty particle
+set is refused."""
 import numpy as np
 import pytest
 
@@ -61,6 +62,25 @@ def test_body_in_the_wall_band(name):
     sc = make_scene(Body((0.2, 0.8, 0.2), (0.08, 0.08, 0.08)), 800, 16, 5, 1e4, 0.3, velocity="sine", amplitude=0.5)
     assert sc.x.min() > 0.1 and sc.x.max() < 0.9
     print(f"[edges] wall band physics={name} n={sc.n}", flush=True)
+    _episode_matches(sc, _subset(sc, sc.n), name)
+
+
+@pytest.mark.parametrize("name", ["plain", "reference"])
+def test_box_shaped_grid_with_offset_origin(name):
+    """A grid of 25 x 13 x 19 nodes (node index i*ny*nz + j*nz + k with
+    unequal extents) whose origin is not at zero: the window strides and the
+    node indexing must still agree with the oracle."""
+    import dataclasses
+    from dmpm.scenes import Body, make_scene
+    base = make_scene(Body((0.5, 0.5, 0.5), (0.3, 0.14, 0.2)), 1200, 20, 17, 1e4, 0.3, velocity="sine",
+                      amplitude=0.4)
+    shift = np.array([-0.7, 0.35, 0.1])
+    dx = base.dx
+    sc = dataclasses.replace(base, x=base.x + shift, dims=(25, 13, 19),
+                             grid_min=tuple(float(c) for c in shift + np.array([0.5 - 12 * dx, 0.5 - 6 * dx, 0.5 - 9 * dx])))
+    lo = (sc.x - np.array(sc.grid_min)) / dx
+    assert lo.min() > 2.0 and (lo.max(axis=0) < np.array(sc.dims) - 3.0).all()
+    print(f"[edges] box grid physics={name} dims={sc.dims} n={sc.n}", flush=True)
     _episode_matches(sc, _subset(sc, sc.n), name)
 
 
