@@ -421,8 +421,7 @@ DMPM_DI Trust<T> trust_region(const KP<T>& a, const m3<T>& Ft, const m3<T>& Craw
 template <typename T, int ORDER, int MODEL>
 DMPM_DI void p2g_terms(const KP<T>& a, T vol, T lam, T mu, const v3<T>& v, const m3<T>& Ft, const m3<T>& C, T m,
                        m3<T>& A, v3<T>& mv) {
-    const m3<T> P = Model<MODEL>::pk1(Ft, lam, mu);
-    m3<T> str = mmult(P, Ft);
+    m3<T> str = Model<MODEL>::kirchhoff(Ft, lam, mu);  // P Ft^T
     const T tau = a.zeta * mtrace(C);
     str.a[0] += tau; str.a[4] += tau; str.a[8] += tau;
     const T k0 = -Spline<ORDER>::dinv(a.dx) * a.dt * vol;
@@ -1263,13 +1262,12 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
         const T lam = a.stat[SLAM * a.ld + i], mu = a.stat[SMU * a.ld + i];
         const v3<T> vp = load_v3(a.st, a.ld, PV, i);
         const m3<T> C = load_m3(a.st, a.ld, PC, i);
-        const m3<T> P = Model<MODEL>::pk1(Ft, lam, mu);
         const T dinv = Spline<ORDER>::dinv(a.dx);
         const T k0 = dinv * a.dt * vol;
         // Gt = A_p = -k0 (P Ft^T + zeta tr(C) I) + m C
         m3<T> Gt;
         {
-            const m3<T> PFt = mmult(P, Ft);
+            const m3<T> PFt = Model<MODEL>::kirchhoff(Ft, lam, mu);
     #pragma unroll
             for (int q = 0; q < 9; ++q) Gt.a[q] = -k0 * PFt.a[q] + m * C.a[q];
             const T bulk = -k0 * a.zeta * mtrace(C);

@@ -292,6 +292,21 @@ template <> struct Model<1> {
         for (int i = 0; i < 9; ++i) P.a[i] = mu * F.a[i] + k * Fit.a[i];
         return P;
     }
+    // Kirchhoff stress P F^T = mu F F^T + (lam log J - mu) I, since
+    // F^{-T} F^T = I: what the P2G momentum and the adjoint's affine term use,
+    // without the cofactor matrix and 1/J (det F and log J in FP64 as in pk1).
+    template <typename T>
+    static DMPM_DI m3<T> kirchhoff(const m3<T>& F, T lam, T mu) {
+        const double J = mdet(mcast<double>(F));
+        const T k = (T)((double)lam * log(fmax(J, 1e-6)) - (double)mu);
+        m3<T> tau = mmult(F, F);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) tau.a[i] *= mu;
+        tau.a[0] += k;
+        tau.a[4] += k;
+        tau.a[8] += k;
+        return tau;
+    }
     // dP = mu dF + lam (F^{-T}:dF) F^{-T} + (mu - lam log J) F^{-T} dF^T F^{-T}
     // (FP64 as in pk1 for det F, log J and F^{-T}).
     template <typename T>
@@ -310,6 +325,11 @@ template <> struct Model<1> {
 };
 
 template <> struct Model<0> {
+    // Kirchhoff stress P F^T, formed from P as the reference does.
+    template <typename T>
+    static DMPM_DI m3<T> kirchhoff(const m3<T>& F, T lam, T mu) {
+        return mmult(pk1(F, lam, mu), F);
+    }
     // P = 2 mu (F - R) + lam (J-1) J F^{-T}, J = max(det F, 1e-6)
     template <typename T>
     static DMPM_DI m3<T> pk1(const m3<T>& F, T lam, T mu) {

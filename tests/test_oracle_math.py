@@ -93,6 +93,19 @@ def test_stress_closed_forms(model):
     assert rel_err(O64.pk1(model, Q @ F, lam, mu), Q @ O64.pk1(model, F, lam, mu)) < 1e-12
 
 
+def test_neo_hookean_kirchhoff_closed_form():
+    """P F^T = mu F F^T + (lam log J - mu) I for the Neo-Hookean P (the GPU
+    forms the Kirchhoff stress this way, without F^-T)."""
+    lam, mu = 5769.23, 3846.15
+    rng = np.random.default_rng(9)
+    F = np.eye(3) + 0.3 * rng.normal(size=(40, 3, 3))
+    F = F[np.linalg.det(F) > 0.05]
+    tau = O64.pk1(1, F, lam, mu) @ np.swapaxes(F, 1, 2)
+    J = np.linalg.det(F)
+    expect = mu * F @ np.swapaxes(F, 1, 2) + (lam * np.log(J) - mu)[:, None, None] * np.eye(3)
+    assert rel_err(tau, expect) < 1e-12
+
+
 @pytest.mark.parametrize("model", [0, 1])
 def test_stress_is_energy_gradient_and_hessian(model):
     lam, mu = 5769.23, 3846.15
