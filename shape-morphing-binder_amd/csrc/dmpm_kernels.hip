@@ -1409,7 +1409,6 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
             tr2 = trust_region(a, Ft2, Craw2);
         }
         const Phase1<T, REF> p1b = phase1<T, REF>(a, cy2, Ft2, Craw2, tr2);
-        const m3<T> P2 = Model<MODEL>::pk1(Ft2, lam, mu);
         // dL/dx_p: sum s_i grad w - Gt^T sum w h - D^{-1} gCraw^T v' + dL/dx'
         const v3<T> gx1 = cy2.gx;
         const v3<T> Gsh = mtv(Gt, sh);
@@ -1419,9 +1418,9 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
         for (int q = 0; q < 3; ++q) gx.a[q] = dxa.a[q] - Gsh.a[q] - Cv.a[q] + gx1.a[q];
         const T dragf = m * (T(1) - a.dt * a.drag);
         const v3<T> gv = {{dragf * sh.a[0], dragf * sh.a[1], dragf * sh.a[2]}};
-        // A_p = -k0 (P Ft^T + zeta tr C I) + m C
-        const m3<T> dP = mscale(-k0, mmul(gA, Ft2));
-        m3<T> gFt = madd(p1b.gFt, mscale(-k0, mtmul(gA, P2)));
+        // A_p = -k0 (P Ft^T + zeta tr C I) + m C: the stress term's gradient
+        // in Ft is that of <-k0 gA, P(Ft) Ft^T>
+        m3<T> gFt = madd(p1b.gFt, Model<MODEL>::kirchhoff_vjp(Ft2, lam, mu, mscale(-k0, gA)));
         m3<T> gC = mscale(m, gA);
         if (REF) {
             // bulk-viscosity adjoint on the diagonal entries only (:117-123), and
@@ -1439,7 +1438,6 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
             const T tb = -k0 * a.zeta * mtrace(gA);
             gC.a[0] += tb; gC.a[4] += tb; gC.a[8] += tb;
         }
-        gFt = madd(gFt, Model<MODEL>::hvp(Ft2, lam, mu, dP));
         store_v3(a.adj0, a.ld, PX, i, gx);
         store_v3(a.adj0, a.ld, PV, i, gv);
         const m3<T> gF = madd(gFt, p1b.gFdir);

@@ -307,6 +307,25 @@ template <> struct Model<1> {
         tau.a[8] += k;
         return tau;
     }
+    // Gradient of <G, P(F) F^T> in F: G^T P + dP(F; G F), which for this
+    // model reduces to mu (G + G^T) F + lam tr(G) F^{-T} (the log J terms
+    // cancel; F^{-T} from FP64 det F and cofactors as in pk1).
+    template <typename T>
+    static DMPM_DI m3<T> kirchhoff_vjp(const m3<T>& F, T lam, T mu, const m3<T>& G) {
+        const m3<double> Fd = mcast<double>(F);
+        const double J = mdet(Fd);
+        const m3<T> Fit = mcast<T>(mscale(1.0 / J, mcof(Fd)));
+        m3<T> S;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) S.a[3 * r + c] = mu * (G.a[3 * r + c] + G.a[3 * c + r]);
+        m3<T> R = mmul(S, F);
+        const T c = lam * mtrace(G);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) R.a[q] += c * Fit.a[q];
+        return R;
+    }
     // dP = mu dF + lam (F^{-T}:dF) F^{-T} + (mu - lam log J) F^{-T} dF^T F^{-T}
     // (FP64 as in pk1 for det F, log J and F^{-T}).
     template <typename T>
@@ -329,6 +348,11 @@ template <> struct Model<0> {
     template <typename T>
     static DMPM_DI m3<T> kirchhoff(const m3<T>& F, T lam, T mu) {
         return mmult(pk1(F, lam, mu), F);
+    }
+    // Gradient of <G, P(F) F^T> in F: G^T P + dP(F; G F).
+    template <typename T>
+    static DMPM_DI m3<T> kirchhoff_vjp(const m3<T>& F, T lam, T mu, const m3<T>& G) {
+        return madd(mtmul(G, pk1(F, lam, mu)), hvp(F, lam, mu, mmul(G, F)));
     }
     // P = 2 mu (F - R) + lam (J-1) J F^{-T}, J = max(det F, 1e-6)
     template <typename T>

@@ -94,8 +94,8 @@ def test_stress_closed_forms(model):
 
 
 def test_neo_hookean_kirchhoff_closed_form():
-    """P F^T = mu F F^T + (lam log J - mu) I for the Neo-Hookean P (the GPU
-    forms the Kirchhoff stress this way, without F^-T)."""
+    """P F^T = mu F F^T + (lam log J - mu) I for the Neo-Hookean P, and the
+    gradient of <G, P F^T> in closed form (the GPU forms both this way)."""
     lam, mu = 5769.23, 3846.15
     rng = np.random.default_rng(9)
     F = np.eye(3) + 0.3 * rng.normal(size=(40, 3, 3))
@@ -104,6 +104,13 @@ def test_neo_hookean_kirchhoff_closed_form():
     J = np.linalg.det(F)
     expect = mu * F @ np.swapaxes(F, 1, 2) + (lam * np.log(J) - mu)[:, None, None] * np.eye(3)
     assert rel_err(tau, expect) < 1e-12
+    # its gradient: G^T P + dP(F; G F) = mu (G + G^T) F + lam tr(G) F^-T
+    G = rng.normal(size=F.shape)
+    Gt = np.swapaxes(G, 1, 2)
+    vjp = Gt @ O64.pk1(1, F, lam, mu) + O64.pk1_hvp(1, F, lam, mu, G @ F)
+    Fit = np.swapaxes(np.linalg.inv(F), 1, 2)
+    closed = mu * (G + Gt) @ F + lam * np.trace(G, axis1=1, axis2=2)[:, None, None] * Fit
+    assert rel_err(vjp, closed) < 1e-11
 
 
 @pytest.mark.parametrize("model", [0, 1])
