@@ -1471,9 +1471,21 @@ __global__ void __launch_bounds__(256) k_adj_gather(KP<T> a) {
     adj_gather_block<T, ORDER, MODEL, REF, false>(a, nullptr, logical_block(), smem_d);
 }
 
+// Minimum waves per SIMD the compiler must fit the fused adjoint into: four
+// (at most 128 VGPRs) for FP32 with the quadratic B-spline and Neo-Hookean
+// stress, which needs 133 on its own and spills five values to scratch at
+// 128 (measured at 1M particles: 158.3 -> 153.5 us per launch); no bound
+// for the other forms (FP64 and the cubic / fixed-corotated ones hold many
+// more registers, and would spill heavily).
+template <typename T, int ORDER, int MODEL>
+constexpr int adj_fused_waves() {
+    return sizeof(T) == 4 && ORDER == 2 && MODEL == 1 ? 4 : 1;
+}
+
 // Adjoint of step t (gather) and of step t-1 (scatter) in one launch.
 template <typename T, int ORDER, int MODEL, bool REF>
-__global__ void __launch_bounds__(256) k_adj_fused(KP<T> a, KP<T> b) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(adj_fused_waves<T, ORDER, MODEL>())))
+k_adj_fused(KP<T> a, KP<T> b) {
     extern __shared__ double smem_d[];
     const int lb = logical_block();
     adj_gather_block<T, ORDER, MODEL, REF, true>(a, &b, lb, smem_d);
@@ -1492,7 +1504,8 @@ __global__ void __launch_bounds__(256) k_adj_gather_batch(const KP<T>* tab, cons
 // Batched k_adj_fused: tab holds two parameter blocks per instance (step t,
 // then step t-1).
 template <typename T, int ORDER, int MODEL, bool REF>
-__global__ void __launch_bounds__(256) k_adj_fused_batch(const KP<T>* tab, const int* bstart, int count) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(adj_fused_waves<T, ORDER, MODEL>())))
+k_adj_fused_batch(const KP<T>* tab, const int* bstart, int count) {
     extern __shared__ double smem_d[];
     int inst, lblock;
     if (!batch_block(bstart, count, inst, lblock)) return;
