@@ -190,6 +190,10 @@ void volumes(Prec pr, const Launch& L, void* vol_out, hipStream_t s);
 
 // Squared Frobenius norm partials of 9 planes (dL/dF): partial[b]
 void sqnorm9(Prec pr, const void* planes, int n, long long ld, double* partial, int nblocks, hipStream_t s);
+// Sum of squares partials of nc planes of n elements (stride ld) of
+// precision pr: partial[b].  A flat array of n values is one plane.
+void sqnorm_planes(Prec pr, const void* planes, int n, int nc, long long ld, double* partial, int nblocks,
+                   hipStream_t s);
 // Descend_Adam element update on 9-plane arrays.
 void adam(Prec pr, const void* dLdF, void* dFc, void* mom, void* vec, void* vmax, int n, long long ld,
           const double* sqnorm, double alpha, double beta1, double beta2, double eps, int timestep,

@@ -30,7 +30,34 @@
 #include "dmpm_kernels.h"
 #include "dmpm_math.h"
 
+// The build compiles this file once per part, in parallel, with DMPM_PART
+// set: part 0 holds the small kernels and every host entry point declared in
+// dmpm_kernels.h; parts 1-6 each instantiate one group of the big particle
+// kernels for one precision (1/2: forward FP32/FP64, 3/4: adjoint scatter and
+// gather, 5/6: fused adjoint), reached from part 0 through the launch_*<T>
+// templates declared below.  Without DMPM_PART one unit holds everything.
+#ifndef DMPM_PART
+#define DMPM_PART -1
+#endif
+#define DMPM_IN(p) (DMPM_PART < 0 || DMPM_PART == (p))
+
 namespace dmpm {
+
+// Per-part launchers (defined and instantiated in parts 1-6).
+template <typename T> void launch_fwd(int kind, const Launch& L, hipStream_t s);
+template <typename T> void launch_adj(int kind, const Launch& L, hipStream_t s);
+template <typename T> void launch_fused(const Launch& L, const Launch& L2, hipStream_t s);
+// One batched launch of kernel op `kind` over `count` instances (parts 1-6 by
+// kind group and precision).
+template <typename T>
+void batch_fwd(int kind, const Phys& ph, int block, size_t sh, unsigned nb, const void* tab, const int* bstart,
+               int count, hipStream_t s);
+template <typename T>
+void batch_adj(int kind, const Phys& ph, int block, size_t sh, unsigned nb, const void* tab, const int* bstart,
+               int count, hipStream_t s);
+template <typename T>
+void batch_fused(int kind, const Phys& ph, int block, size_t sh, unsigned nb, const void* tab, const int* bstart,
+                 int count, hipStream_t s);
 
 template <typename T>
 struct KP {
@@ -1515,6 +1542,7 @@ k_adj_fused_batch(const KP<T>* tab, const int* bstart, int count) {
     clear_share(a, lblock);
 }
 
+#if DMPM_IN(0)
 // ---------------------------------------------------------------------------
 // Deterministic two-pass scatter.  Pass 1 computes each particle's
 // contribution terms and its base cell; a stable radix sort orders particles
@@ -1916,10 +1944,10 @@ __global__ void k_volumes(KP<T> a, T* vol) {
 }
 
 template <typename T>
-__global__ void k_sqnorm9(const T* p, int n, long long ld, double* partial) {
+__global__ void k_sqnorm(const T* p, int n, int nc, long long ld, double* partial) {
     double acc = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        for (int q = 0; q < 9; ++q) {
+        for (int q = 0; q < nc; ++q) {
             const double v = (double)p[q * ld + i];
             acc += v * v;
         }
@@ -1983,8 +2011,11 @@ __global__ void k_adam(const T* dLdF, T* dFc, T* mom, T* vec, T* vmax, int n, lo
     }
 }
 
+#endif  // DMPM_IN(0)
+
 // ---------------------------------------------------------------------------
 // Launchers.
+#if DMPM_IN(1) || DMPM_IN(2)
 template <typename T>
 static void launch_p2g(const Launch& L, bool mass_only, hipStream_t s) {
     const int na = mass_only ? 1 : 4;
@@ -2026,6 +2057,53 @@ static void launch_step(const Launch& L, hipStream_t s) {
 }
 
 template <typename T>
+void launch_fwd(int kind, const Launch& L, hipStream_t s) {
+    switch (kind) {
+        case OP_P2G: launch_p2g<T>(L, false, s); break;
+        case OP_P2G_MASS: launch_p2g<T>(L, true, s); break;
+        case OP_G2P: launch_g2p<T>(L, s); break;
+        case OP_STEP: launch_step<T>(L, s); break;
+        default: break;
+    }
+}
+
+template <typename T>
+void batch_fwd(int kind, const Phys& ph, int block, size_t sh, unsigned nb, const void* tab_, const int* bstart,
+               int count, hipStream_t s) {
+    const KP<T>* tab = (const KP<T>*)tab_;
+    const int o = ph.order, md = ph.model;
+#define BATCH(KERNEL) hipLaunchKernelGGL(KERNEL, dim3(nb), dim3(block), sh, s, tab, bstart, count)
+    switch (kind) {
+        case OP_P2G:
+            if (o == 3) { if (md == 0) BATCH((k_p2g_batch<T, 3, 0, false>)); else BATCH((k_p2g_batch<T, 3, 1, false>)); }
+            else { if (md == 0) BATCH((k_p2g_batch<T, 2, 0, false>)); else BATCH((k_p2g_batch<T, 2, 1, false>)); }
+            break;
+        case OP_P2G_MASS:
+            if (o == 3) BATCH((k_p2g_batch<T, 3, 0, true>)); else BATCH((k_p2g_batch<T, 2, 0, true>));
+            break;
+        case OP_G2P:
+            if (o == 3) BATCH((k_g2p_batch<T, 3>)); else BATCH((k_g2p_batch<T, 2>));
+            break;
+        case OP_STEP:
+            if (o == 3) { if (md == 0) BATCH((k_step_batch<T, 3, 0>)); else BATCH((k_step_batch<T, 3, 1>)); }
+            else { if (md == 0) BATCH((k_step_batch<T, 2, 0>)); else BATCH((k_step_batch<T, 2, 1>)); }
+            break;
+        default: break;
+    }
+#undef BATCH
+}
+#endif
+#if DMPM_IN(1)
+template void launch_fwd<float>(int, const Launch&, hipStream_t);
+template void batch_fwd<float>(int, const Phys&, int, size_t, unsigned, const void*, const int*, int, hipStream_t);
+#endif
+#if DMPM_IN(2)
+template void launch_fwd<double>(int, const Launch&, hipStream_t);
+template void batch_fwd<double>(int, const Phys&, int, size_t, unsigned, const void*, const int*, int, hipStream_t);
+#endif
+
+#if DMPM_IN(3) || DMPM_IN(4)
+template <typename T>
 static void launch_adj_scatter(const Launch& L, hipStream_t s) {
     KP<T> k = make_kp<T>(L, 3 * (int)sizeof(T) + 3 * (int)sizeof(Acc) + 8);
     const unsigned nb = launch_blocks(L);
@@ -2056,6 +2134,48 @@ static void launch_adj_gather(const Launch& L, hipStream_t s) {
 }
 
 template <typename T>
+void launch_adj(int kind, const Launch& L, hipStream_t s) {
+    if (kind == OP_ADJ_SCATTER) launch_adj_scatter<T>(L, s);
+    else if (kind == OP_ADJ_GATHER) launch_adj_gather<T>(L, s);
+}
+
+template <typename T>
+void batch_adj(int kind, const Phys& ph, int block, size_t sh, unsigned nb, const void* tab_, const int* bstart,
+               int count, hipStream_t s) {
+    const KP<T>* tab = (const KP<T>*)tab_;
+    const int o = ph.order, md = ph.model;
+    const bool ref = ph.adjoint == 1;
+#define BATCH(KERNEL) hipLaunchKernelGGL(KERNEL, dim3(nb), dim3(block), sh, s, tab, bstart, count)
+    switch (kind) {
+        case OP_ADJ_SCATTER:
+            if (o == 3) { if (ref) BATCH((k_adj_scatter_batch<T, 3, true>)); else BATCH((k_adj_scatter_batch<T, 3, false>)); }
+            else { if (ref) BATCH((k_adj_scatter_batch<T, 2, true>)); else BATCH((k_adj_scatter_batch<T, 2, false>)); }
+            break;
+        case OP_ADJ_GATHER:
+            if (o == 3) {
+                if (md == 0) { if (ref) BATCH((k_adj_gather_batch<T, 3, 0, true>)); else BATCH((k_adj_gather_batch<T, 3, 0, false>)); }
+                else { if (ref) BATCH((k_adj_gather_batch<T, 3, 1, true>)); else BATCH((k_adj_gather_batch<T, 3, 1, false>)); }
+            } else {
+                if (md == 0) { if (ref) BATCH((k_adj_gather_batch<T, 2, 0, true>)); else BATCH((k_adj_gather_batch<T, 2, 0, false>)); }
+                else { if (ref) BATCH((k_adj_gather_batch<T, 2, 1, true>)); else BATCH((k_adj_gather_batch<T, 2, 1, false>)); }
+            }
+            break;
+        default: break;
+    }
+#undef BATCH
+}
+#endif
+#if DMPM_IN(3)
+template void launch_adj<float>(int, const Launch&, hipStream_t);
+template void batch_adj<float>(int, const Phys&, int, size_t, unsigned, const void*, const int*, int, hipStream_t);
+#endif
+#if DMPM_IN(4)
+template void launch_adj<double>(int, const Launch&, hipStream_t);
+template void batch_adj<double>(int, const Phys&, int, size_t, unsigned, const void*, const int*, int, hipStream_t);
+#endif
+
+#if DMPM_IN(5) || DMPM_IN(6)
+template <typename T>
 static void launch_adj_fused(const Launch& L, const Launch& L2, hipStream_t s) {
     KP<T> k = make_kp<T>(L, 7 * (int)sizeof(T));
     KP<T> k2 = make_kp<T>(L2, 3 * (int)sizeof(Acc));
@@ -2074,6 +2194,39 @@ static void launch_adj_fused(const Launch& L, const Launch& L2, hipStream_t s) {
 #undef AF_CASE
 }
 
+template <typename T>
+void launch_fused(const Launch& L, const Launch& L2, hipStream_t s) {
+    launch_adj_fused<T>(L, L2, s);
+}
+
+template <typename T>
+void batch_fused(int kind, const Phys& ph, int block, size_t sh, unsigned nb, const void* tab_, const int* bstart,
+                 int count, hipStream_t s) {
+    const KP<T>* tab = (const KP<T>*)tab_;
+    const int o = ph.order, md = ph.model;
+    const bool ref = ph.adjoint == 1;
+    if (kind != OP_ADJ_FUSED) return;
+#define BATCH(KERNEL) hipLaunchKernelGGL(KERNEL, dim3(nb), dim3(block), sh, s, tab, bstart, count)
+    if (o == 3) {
+        if (md == 0) { if (ref) BATCH((k_adj_fused_batch<T, 3, 0, true>)); else BATCH((k_adj_fused_batch<T, 3, 0, false>)); }
+        else { if (ref) BATCH((k_adj_fused_batch<T, 3, 1, true>)); else BATCH((k_adj_fused_batch<T, 3, 1, false>)); }
+    } else {
+        if (md == 0) { if (ref) BATCH((k_adj_fused_batch<T, 2, 0, true>)); else BATCH((k_adj_fused_batch<T, 2, 0, false>)); }
+        else { if (ref) BATCH((k_adj_fused_batch<T, 2, 1, true>)); else BATCH((k_adj_fused_batch<T, 2, 1, false>)); }
+    }
+#undef BATCH
+}
+#endif
+#if DMPM_IN(5)
+template void launch_fused<float>(const Launch&, const Launch&, hipStream_t);
+template void batch_fused<float>(int, const Phys&, int, size_t, unsigned, const void*, const int*, int, hipStream_t);
+#endif
+#if DMPM_IN(6)
+template void launch_fused<double>(const Launch&, const Launch&, hipStream_t);
+template void batch_fused<double>(int, const Phys&, int, size_t, unsigned, const void*, const int*, int, hipStream_t);
+#endif
+
+#if DMPM_IN(0)
 // ---- batched launches --------------------------------------------------------
 // Bytes of LDS per window node of each kernel op (the window capacity is the
 // LDS budget divided by this), and the dynamic LDS a block of it takes.
@@ -2121,56 +2274,22 @@ int instance_blocks(const Launch& L) {
 }
 
 template <typename T>
-static void launch_batch_t(int kind, const Phys& ph, int lds_bytes, int block, const void* tab_, const int* bstart,
+static void launch_batch_t(int kind, const Phys& ph, int lds_bytes, int block, const void* tab, const int* bstart,
                            int count, int total, hipStream_t s) {
-    const KP<T>* tab = (const KP<T>*)tab_;
     const unsigned nb = (unsigned)std::max(8, (total + 7) & ~7);
     const int cap = lds_bytes / node_bytes<T>(kind);
     size_t sh = block_lds<T>(kind, cap);
     if (kind == OP_ADJ_FUSED) sh = std::max(sh, (size_t)(lds_bytes / (3 * (int)sizeof(Acc))) * 3 * sizeof(Acc));
-    const int o = ph.order, md = ph.model;
-    const bool ref = ph.adjoint == 1;
-#define BATCH(KERNEL) hipLaunchKernelGGL(KERNEL, dim3(nb), dim3(block), sh, s, tab, bstart, count)
     switch (kind) {
         case OP_P2G:
-            if (o == 3) { if (md == 0) BATCH((k_p2g_batch<T, 3, 0, false>)); else BATCH((k_p2g_batch<T, 3, 1, false>)); }
-            else { if (md == 0) BATCH((k_p2g_batch<T, 2, 0, false>)); else BATCH((k_p2g_batch<T, 2, 1, false>)); }
-            break;
         case OP_P2G_MASS:
-            if (o == 3) BATCH((k_p2g_batch<T, 3, 0, true>)); else BATCH((k_p2g_batch<T, 2, 0, true>));
-            break;
         case OP_G2P:
-            if (o == 3) BATCH((k_g2p_batch<T, 3>)); else BATCH((k_g2p_batch<T, 2>));
-            break;
-        case OP_STEP:
-            if (o == 3) { if (md == 0) BATCH((k_step_batch<T, 3, 0>)); else BATCH((k_step_batch<T, 3, 1>)); }
-            else { if (md == 0) BATCH((k_step_batch<T, 2, 0>)); else BATCH((k_step_batch<T, 2, 1>)); }
-            break;
+        case OP_STEP: batch_fwd<T>(kind, ph, block, sh, nb, tab, bstart, count, s); break;
         case OP_ADJ_SCATTER:
-            if (o == 3) { if (ref) BATCH((k_adj_scatter_batch<T, 3, true>)); else BATCH((k_adj_scatter_batch<T, 3, false>)); }
-            else { if (ref) BATCH((k_adj_scatter_batch<T, 2, true>)); else BATCH((k_adj_scatter_batch<T, 2, false>)); }
-            break;
-        case OP_ADJ_GATHER:
-            if (o == 3) {
-                if (md == 0) { if (ref) BATCH((k_adj_gather_batch<T, 3, 0, true>)); else BATCH((k_adj_gather_batch<T, 3, 0, false>)); }
-                else { if (ref) BATCH((k_adj_gather_batch<T, 3, 1, true>)); else BATCH((k_adj_gather_batch<T, 3, 1, false>)); }
-            } else {
-                if (md == 0) { if (ref) BATCH((k_adj_gather_batch<T, 2, 0, true>)); else BATCH((k_adj_gather_batch<T, 2, 0, false>)); }
-                else { if (ref) BATCH((k_adj_gather_batch<T, 2, 1, true>)); else BATCH((k_adj_gather_batch<T, 2, 1, false>)); }
-            }
-            break;
-        case OP_ADJ_FUSED:
-            if (o == 3) {
-                if (md == 0) { if (ref) BATCH((k_adj_fused_batch<T, 3, 0, true>)); else BATCH((k_adj_fused_batch<T, 3, 0, false>)); }
-                else { if (ref) BATCH((k_adj_fused_batch<T, 3, 1, true>)); else BATCH((k_adj_fused_batch<T, 3, 1, false>)); }
-            } else {
-                if (md == 0) { if (ref) BATCH((k_adj_fused_batch<T, 2, 0, true>)); else BATCH((k_adj_fused_batch<T, 2, 0, false>)); }
-                else { if (ref) BATCH((k_adj_fused_batch<T, 2, 1, true>)); else BATCH((k_adj_fused_batch<T, 2, 1, false>)); }
-            }
-            break;
+        case OP_ADJ_GATHER: batch_adj<T>(kind, ph, block, sh, nb, tab, bstart, count, s); break;
+        case OP_ADJ_FUSED: batch_fused<T>(kind, ph, block, sh, nb, tab, bstart, count, s); break;
         default: break;
     }
-#undef BATCH
 }
 
 void launch_batch(Prec pr, int kind, const Phys& ph, int lds_bytes, int block, const void* tab, const int* bstart,
@@ -2226,28 +2345,29 @@ static inline unsigned ew_blocks(long long n, int bs = 256) {
 }
 
 void p2g(Prec pr, const Launch& L, bool mass_only, hipStream_t s) {
-    if (pr == PREC_F32) launch_p2g<float>(L, mass_only, s);
-    else launch_p2g<double>(L, mass_only, s);
+    const int kind = mass_only ? OP_P2G_MASS : OP_P2G;
+    if (pr == PREC_F32) launch_fwd<float>(kind, L, s);
+    else launch_fwd<double>(kind, L, s);
 }
 void g2p(Prec pr, const Launch& L, hipStream_t s) {
-    if (pr == PREC_F32) launch_g2p<float>(L, s);
-    else launch_g2p<double>(L, s);
+    if (pr == PREC_F32) launch_fwd<float>(OP_G2P, L, s);
+    else launch_fwd<double>(OP_G2P, L, s);
 }
 void step(Prec pr, const Launch& L, hipStream_t s) {
-    if (pr == PREC_F32) launch_step<float>(L, s);
-    else launch_step<double>(L, s);
+    if (pr == PREC_F32) launch_fwd<float>(OP_STEP, L, s);
+    else launch_fwd<double>(OP_STEP, L, s);
 }
 void adj_scatter(Prec pr, const Launch& L, hipStream_t s) {
-    if (pr == PREC_F32) launch_adj_scatter<float>(L, s);
-    else launch_adj_scatter<double>(L, s);
+    if (pr == PREC_F32) launch_adj<float>(OP_ADJ_SCATTER, L, s);
+    else launch_adj<double>(OP_ADJ_SCATTER, L, s);
 }
 void adj_gather(Prec pr, const Launch& L, hipStream_t s) {
-    if (pr == PREC_F32) launch_adj_gather<float>(L, s);
-    else launch_adj_gather<double>(L, s);
+    if (pr == PREC_F32) launch_adj<float>(OP_ADJ_GATHER, L, s);
+    else launch_adj<double>(OP_ADJ_GATHER, L, s);
 }
 void adj_fused(Prec pr, const Launch& L, const Launch& L2, hipStream_t s) {
-    if (pr == PREC_F32) launch_adj_fused<float>(L, L2, s);
-    else launch_adj_fused<double>(L, L2, s);
+    if (pr == PREC_F32) launch_fused<float>(L, L2, s);
+    else launch_fused<double>(L, L2, s);
 }
 
 void winding_numbers(const double* tri, int nf, const double* P, int np, double* partial, int chunks, double* w,
@@ -2420,9 +2540,15 @@ void volumes(Prec pr, const Launch& L, void* vol, hipStream_t s) {
     }
 }
 
+void sqnorm_planes(Prec pr, const void* p, int n, int nc, long long ld, double* partial, int nblocks, hipStream_t s) {
+    if (pr == PREC_F32)
+        hipLaunchKernelGGL(k_sqnorm<float>, dim3(nblocks), dim3(256), 0, s, (const float*)p, n, nc, ld, partial);
+    else
+        hipLaunchKernelGGL(k_sqnorm<double>, dim3(nblocks), dim3(256), 0, s, (const double*)p, n, nc, ld, partial);
+}
+
 void sqnorm9(Prec pr, const void* p, int n, long long ld, double* partial, int nblocks, hipStream_t s) {
-    if (pr == PREC_F32) hipLaunchKernelGGL(k_sqnorm9<float>, dim3(nblocks), dim3(256), 0, s, (const float*)p, n, ld, partial);
-    else hipLaunchKernelGGL(k_sqnorm9<double>, dim3(nblocks), dim3(256), 0, s, (const double*)p, n, ld, partial);
+    sqnorm_planes(pr, p, n, 9, ld, partial, nblocks, s);
 }
 
 void adam(Prec pr, const void* dLdF, void* dFc, void* mom, void* vec, void* vmax, int n, long long ld,
@@ -2538,5 +2664,7 @@ void det_adj_scatter(Prec pr, const Launch& L, void* scratch, hipStream_t s) {
     if (pr == PREC_F32) launch_det_adj<float>(L, scratch, s);
     else launch_det_adj<double>(L, scratch, s);
 }
+
+#endif  // DMPM_IN(0)
 
 }  // namespace dmpm

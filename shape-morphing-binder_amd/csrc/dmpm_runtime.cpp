@@ -153,6 +153,8 @@ struct dmpm_sim {
     bool has_rg = false;
     int rg_n = 0;
     double rg_normF = 0, rg_normx = 0;
+    double* rgn = nullptr;        // device: squared norms of the last set (F, x)
+    bool rg_norms_host = true;    // rg_normF/x hold them
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double fwd_ms = 0, bwd_ms = 0;
@@ -164,12 +166,59 @@ struct dmpm_sim {
     // pointers (shared with them, so that one destroyed after the sim still
     // counts down safely)
     std::shared_ptr<int> captures = std::make_shared<int>(0);
+    // cleared when the sim is destroyed; a graph or batch refuses to launch
+    // once any sim it captured is gone
+    std::shared_ptr<bool> alive = std::make_shared<bool>(true);
     std::vector<hipEvent_t> kpool;
     size_t knext = 0;
     std::vector<std::pair<int, size_t>> krec;  // (kind, index of start event)
     int koverflow = 0;
 
+    // A captured graph or batched episode holds this sim's device pointers
+    // (layers, grids, carries, controls, render gradients, Adam-free
+    // buffers) and the pointers and sizes baked into its launches.  Calls
+    // that would free, move or resize such a buffer, or change a parameter
+    // baked into a launch, are refused while one is alive.
+    bool captured() const { return *captures > 0; }
+    void refuse_if_captured(const char* what) const {
+        if (captured())
+            throw Error(DMPM_ERR_STATE, std::string(what) + " is refused while a captured graph or batched episode "
+                                                            "holds this sim's buffers; destroy it first");
+    }
+
+    // Host bookkeeping after an episode (which layers, grids and segment
+    // buffers hold results, where the adjoint carry is).  A capture runs the
+    // bookkeeping of the episode it records; each launch of the capture
+    // restores it, since a replay leaves the device in that same state
+    // whatever eager calls ran in between.
+    struct Book {
+        std::vector<char> valid, fresh, gvalid;
+        int seg_base = -1, carry_cur = 0, carry_layer = -1;
+        void* seg0 = nullptr;
+    };
+    Book book() const {
+        Book b;
+        b.valid = valid;
+        b.fresh = fresh;
+        b.gvalid = gvalid;
+        b.seg_base = seg_base;
+        b.carry_cur = carry_cur;
+        b.carry_layer = carry_layer;
+        b.seg0 = seg.empty() ? nullptr : seg[0];
+        return b;
+    }
+    void restore(const Book& b) {
+        valid = b.valid;
+        fresh = b.fresh;
+        gvalid = b.gvalid;
+        seg_base = b.seg_base;
+        carry_cur = b.carry_cur;
+        carry_layer = b.carry_layer;
+        if (!seg.empty()) seg[0] = b.seg0;
+    }
+
     ~dmpm_sim() {
+        *alive = false;
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto& a : allocs) (void)hipFree(a.first);
         for (auto& e : ev)
@@ -448,9 +497,18 @@ struct dmpm_sim {
         if (start < 0 || start >= L) throw Error(DMPM_ERR_INDEX, "start layer out of range");
         const void* cur = state(start);
         // The reference zeroes dFc of every layer it writes (ForwardSimulation.cpp:156);
-        // a rollout with per-step controls keeps them.
+        // a rollout with per-step controls keeps them.  While a capture holds
+        // the control buffers they are zeroed in place instead of freed.
         if (zero_controls)
-            for (int t = start + 1; t < L; ++t) dfree(ctl[t]);
+            for (int t = start + 1; t < L; ++t) {
+                if (!ctl[t]) continue;
+                if (captured()) {
+                    zero(ctl[t], 9 * ld);
+                    grids_stale(t);
+                } else {
+                    dfree(ctl[t]);
+                }
+            }
         // The segment the backward pass enters first (holding layer L-2) is
         // written straight into the segment buffers -- states and grids --
         // when this pass computes all of it, so the backward pass need not
@@ -890,6 +948,7 @@ int dmpm_create(const dmpm_config* cfg, dmpm_sim** out) {
         s->zero(s->target_m, s->gld);
         s->partial = (double*)s->dalloc(4 * 1024 * sizeof(double));
         s->red = (double*)s->dalloc(8 * sizeof(double));
+        s->rgn = (double*)s->dalloc(2 * sizeof(double));
         s->wstat = (unsigned long long*)s->dalloc(2 * sizeof(unsigned long long));
         s->zero(s->wstat, 4);  // 16 bytes
         *out = s.release();
@@ -1040,6 +1099,7 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
                        int on_device) {
     return guard([&] {
         check_sim(s);
+        s->refuse_if_captured("dmpm_set_particles");
         if (n <= 0) throw Error(DMPM_ERR_ARG, "n must be positive");
         if (!x || !v || !F || !C || !m || !vol || !lam || !mu) throw Error(DMPM_ERR_ARG, "null particle array");
         const size_t es = dsize(dtype);
@@ -1143,6 +1203,8 @@ int dmpm_set_target_grid_mass(dmpm_sim* s, const void* m, int dtype, int on_devi
 int dmpm_set_sim_params(dmpm_sim* s, double dt, double drag, double smoothing, const double f_ext[3]) {
     return guard([&] {
         check_sim(s);
+        // captured launches carry dt, drag, smoothing and f_ext by value
+        s->refuse_if_captured("dmpm_set_sim_params");
         s->cfg.dt = dt;
         s->cfg.drag = drag;
         s->cfg.smoothing = smoothing;
@@ -1156,6 +1218,7 @@ int dmpm_set_sim_params(dmpm_sim* s, double dt, double drag, double smoothing, c
 int dmpm_set_up_comp_graph(dmpm_sim* s, int num_layers) {
     return guard([&] {
         check_sim(s);
+        s->refuse_if_captured("dmpm_set_up_comp_graph");
         if (s->n <= 0) throw Error(DMPM_ERR_STATE, "no particles");
         if (num_layers <= 0) throw Error(DMPM_ERR_ARG, "num_layers must be positive");
         for (int t = 1; t < (int)s->ctl.size(); ++t) {
@@ -1369,51 +1432,61 @@ int dmpm_set_render_gradients(dmpm_sim* s, int n, const void* dLdF, const void* 
         check_sim(s);
         if (!dLdF || !dLdx || n < 0) throw Error(DMPM_ERR_ARG, "bad render gradients");
         const size_t es = dsize(dtype);
-        // norms (computed from the caller's data, bind.cpp:413-422)
-        std::vector<char> h((size_t)n * 12 * es);
-        if (on_device) {
-            HIPCHK(hipMemcpy(h.data(), dLdF, (size_t)n * 9 * es, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(h.data() + (size_t)n * 9 * es, dLdx, (size_t)n * 3 * es, hipMemcpyDeviceToHost));
-        } else {
-            std::memcpy(h.data(), dLdF, (size_t)n * 9 * es);
-            std::memcpy(h.data() + (size_t)n * 9 * es, dLdx, (size_t)n * 3 * es);
-        }
-        double nf = 0, nx = 0;
-        for (size_t i = 0; i < (size_t)n * 9; ++i) nf += host_get(h.data(), dtype, i) * host_get(h.data(), dtype, i);
-        for (size_t i = 0; i < (size_t)n * 3; ++i) {
-            double v = host_get(h.data() + (size_t)n * 9 * es, dtype, i);
-            nx += v * v;
-        }
-        s->rg_normF = std::sqrt(nf);
-        s->rg_normx = std::sqrt(nx);
+        const int f64 = dtype == DMPM_DTYPE_F64;
+        const Prec dp = f64 ? PREC_F64 : PREC_F32;
+        // A capture injects from the buffer it recorded: only an update in place is possible.
+        if (s->captured() && (!s->rg || n != s->n))
+            s->refuse_if_captured("dmpm_set_render_gradients without a buffer of this size at capture");
+        const void* dF = s->stage(dLdF, (size_t)n * 9 * es, on_device);
+        const void* dX = s->stage(dLdx, (size_t)n * 3 * es, on_device);
+        // The norms of the caller's data (bind.cpp:413-422), summed on the
+        // device in a fixed order; read back only by dmpm_render_gradient_info.
+        const int nb = red_blocks_for(std::max<long long>((long long)n * 9, 1));
+        sqnorm_planes(dp, dF, n * 9, 1, 0, s->partial, nb, s->stream);
+        reduce_partials(s->partial, nb, 1, s->rgn, s->stream);
+        sqnorm_planes(dp, dX, n * 3, 1, 0, s->partial + 1024, nb, s->stream);
+        reduce_partials(s->partial + 1024, nb, 1, s->rgn + 1, s->stream);
+        s->rg_norms_host = false;
         s->rg_n = n;
         s->has_rg = true;
-        s->dfree(s->rg);
         if (n == s->n) {
-            s->rg = s->dalloc((size_t)12 * s->ld * s->esz);
-            const void* dF = s->stage(dLdF, (size_t)n * 9 * es, on_device);
-            const void* dX = s->stage(dLdx, (size_t)n * 3 * es, on_device);
-            import_planes(s->pr, dF, dtype == DMPM_DTYPE_F64, s->rg, s->perm, n, 9, s->ld, s->stream);
-            import_planes(s->pr, dX, dtype == DMPM_DTYPE_F64, (char*)s->rg + 9 * s->ld * s->esz, s->perm, n, 3,
-                          s->ld, s->stream);
-            s->unstage();
+            if (!s->rg) s->rg = s->dalloc((size_t)12 * s->ld * s->esz);
+            import_planes(s->pr, dF, f64, s->rg, s->perm, n, 9, s->ld, s->stream);
+            import_planes(s->pr, dX, f64, (char*)s->rg + 9 * s->ld * s->esz, s->perm, n, 3, s->ld, s->stream);
+        } else {
+            // stored for the info call only: the backward pass skips a
+            // mismatched set (CompGraph.cpp:148-160)
+            s->dfree(s->rg);
         }
+        s->unstage();
     });
 }
 
 int dmpm_clear_render_gradients(dmpm_sim* s) {
     return guard([&] {
         check_sim(s);
-        s->dfree(s->rg);
+        // a capture keeps injecting from its buffer: zeros inject nothing
+        if (s->captured() && s->rg) s->zero(s->rg, 12 * s->ld);
+        else s->dfree(s->rg);
         s->has_rg = false;
         s->rg_n = 0;
         s->rg_normF = s->rg_normx = 0;
+        s->rg_norms_host = true;
     });
 }
 
-int dmpm_render_gradient_info(const dmpm_sim* s, int* has, int* num_points, double* nF, double* nx) {
+int dmpm_render_gradient_info(const dmpm_sim* cs, int* has, int* num_points, double* nF, double* nx) {
     return guard([&] {
-        check_sim(s);
+        check_sim(cs);
+        dmpm_sim* s = const_cast<dmpm_sim*>(cs);
+        if (s->has_rg && !s->rg_norms_host) {
+            double h[2] = {0, 0};
+            HIPCHK(hipMemcpyAsync(h, s->rgn, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+            HIPCHK(hipStreamSynchronize(s->stream));
+            s->rg_normF = std::sqrt(h[0]);
+            s->rg_normx = std::sqrt(h[1]);
+            s->rg_norms_host = true;
+        }
         if (has) *has = s->has_rg;
         if (num_points) *num_points = s->rg_n;
         if (nF) *nF = s->has_rg ? s->rg_normF : 0.0;
@@ -1481,6 +1554,7 @@ int dmpm_sort_particles(dmpm_sim* s, int layer) {
 int dmpm_promote_last_as_initial(dmpm_sim* s, int /*carry_grid*/) {
     return guard([&] {
         check_sim(s);
+        s->refuse_if_captured("dmpm_promote_last_as_initial");
         if (s->L <= 0) return;
         const int last = s->L - 1;
         if (last > 0) {
@@ -1521,10 +1595,17 @@ int dmpm_set_control(dmpm_sim* s, int layer, const void* dFc, int dtype, int on_
         check_sim(s);
         if (layer < 0 || layer >= s->L) throw Error(DMPM_ERR_INDEX, "Layer index out of range.");
         if (!dFc) {
-            s->dfree(s->ctl[layer]);
+            // a capture keeps reading the buffer: zero it in place
+            if (s->captured() && s->ctl[layer]) s->zero(s->ctl[layer], 9 * s->ld);
+            else s->dfree(s->ctl[layer]);
+            s->grids_stale(layer);
             return;
         }
-        if (!s->ctl[layer]) s->ctl[layer] = s->dalloc(s->ctl_bytes());
+        if (!s->ctl[layer]) {
+            // a capture recorded this layer without a control and would ignore a new buffer
+            s->refuse_if_captured("dmpm_set_control on a layer that had no control when captured");
+            s->ctl[layer] = s->dalloc(s->ctl_bytes());
+        }
         const void* d = s->stage(dFc, (size_t)s->n * 9 * dsize(dtype), on_device);
         import_planes(s->pr, d, dtype == DMPM_DTYPE_F64, s->ctl[layer], s->perm, s->n, 9, s->ld, s->stream);
         s->unstage();
@@ -1556,7 +1637,7 @@ int dmpm_request_control_grad(dmpm_sim* s, int layer, int on) {
         check_sim(s);
         if (layer < 0 || layer >= s->L) throw Error(DMPM_ERR_INDEX, "Layer index out of range.");
         s->want_gctl[layer] = on ? 1 : 0;
-        if (!on) s->dfree(s->gctl[layer]);
+        if (!on && !s->captured()) s->dfree(s->gctl[layer]);
     });
 }
 
@@ -1600,6 +1681,7 @@ int dmpm_optimize_single_timestep(dmpm_sim* s, int t, int max_gd_iters, int /*ep
                                   int max_ls) {
     return guard([&] {
         check_sim(s);
+        s->refuse_if_captured("dmpm_optimize_single_timestep");
         if (t < 0 || t >= s->L - 1) {
             fprintf(stderr, "Invalid timestep index: %d\n", t);
             return;
@@ -1648,6 +1730,7 @@ int dmpm_run_optimization(dmpm_sim* s, int num_steps, double dt, double drag, co
                           double smoothing, int /*current_episodes*/, double* final_loss) {
     return guard([&] {
         check_sim(s);
+        s->refuse_if_captured("dmpm_run_optimization");
         if (num_steps <= 0) throw Error(DMPM_ERR_ARG, "num_steps must be positive");
         if (control_stride <= 0) throw Error(DMPM_ERR_ARG, "control_stride must be positive");
         s->cfg.dt = dt;
@@ -1832,13 +1915,27 @@ int dmpm_last_timing(const dmpm_sim* s, double* fwd_ms, double* bwd_ms) {
 // sim's own stream, forked from and joined back to an origin stream.  The
 // branches are independent, so the GPU runs the instances concurrently.
 // Holds one count on each captured sim while alive.
+// The sims' host bookkeeping after the captured episode is kept and put back
+// at every launch; a launch is refused once one of the sims is destroyed.
 struct CaptureHold {
     std::vector<std::shared_ptr<int>> counts;
-    void hold(dmpm_sim** sims, int n) {
+    std::vector<std::shared_ptr<bool>> alive;
+    std::vector<dmpm_sim*> sims;
+    std::vector<dmpm_sim::Book> books;
+    void hold(dmpm_sim** s, int n) {
         for (int i = 0; i < n; ++i) {
-            counts.push_back(sims[i]->captures);
+            counts.push_back(s[i]->captures);
             ++*counts.back();
+            alive.push_back(s[i]->alive);
+            sims.push_back(s[i]);
+            books.push_back(s[i]->book());
         }
+    }
+    // before a launch
+    void renew() {
+        for (auto& a : alive)
+            if (!*a) throw Error(DMPM_ERR_STATE, "a sim of this capture has been destroyed");
+        for (size_t i = 0; i < sims.size(); ++i) sims[i]->restore(books[i]);
     }
     ~CaptureHold() {
         for (auto& c : counts) --*c;
@@ -1920,6 +2017,7 @@ int dmpm_graph_capture(dmpm_sim** sims, int count, int start_layer, int stop_lay
 int dmpm_graph_launch(dmpm_graph* g) {
     return guard([&] {
         if (!g || !g->exec) throw Error(DMPM_ERR_ARG, "null graph");
+        g->hold.renew();
         HIPCHK(hipGraphLaunch(g->exec, g->origin));
     });
 }
@@ -2076,6 +2174,7 @@ int dmpm_batch_capture(dmpm_sim** sims, int count, int start_layer, int stop_lay
 int dmpm_batch_launch(dmpm_batch* b) {
     return guard([&] {
         if (!b || !b->exec) throw Error(DMPM_ERR_ARG, "null batch");
+        b->hold.renew();
         HIPCHK(hipGraphLaunch(b->exec, b->stream));
     });
 }

@@ -21,6 +21,9 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libdmpm.so")
 
 SOURCES = ["dmpm_kernels.hip", "dmpm_runtime.cpp", "dmpm_io.cpp", "dmpm_mesh.cpp"]
+# dmpm_kernels.hip is compiled once per part (DMPM_PART = 0..6, see the file's
+# header), in parallel: each part instantiates one group of kernels.
+KERNEL_PARTS = 7
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -48,12 +51,18 @@ def build(verbose: bool = True, extra: list | None = None) -> str:
     objs = []
     procs = []
     t0 = time.time()
+    units = []
     for src in SOURCES:
-        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
-        cmd = [hipcc(), *HIPCC_FLAGS, *(extra or []), "-I", INCLUDE, "-c", os.path.join(CSRC, src), "-o", obj]
+        if src == "dmpm_kernels.hip":
+            units += [(src, f"dmpm_kernels_p{k}.o", [f"-DDMPM_PART={k}"]) for k in range(KERNEL_PARTS)]
+        else:
+            units.append((src, os.path.splitext(src)[0] + ".o", []))
+    for src, oname, defs in units:
+        obj = os.path.join(objdir, oname)
+        cmd = [hipcc(), *HIPCC_FLAGS, *(extra or []), *defs, "-I", INCLUDE, "-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
             cmd.insert(1, "-xhip")
-        procs.append((src, cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        procs.append((oname, cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
         objs.append(obj)
     for src, cmd, p in procs:
         out, _ = p.communicate()

@@ -37,11 +37,15 @@ class _Rollout(torch.autograd.Function):
         out = [o if isinstance(o, torch.Tensor) else torch.from_numpy(o) for o in out]
         ctx.sim, ctx.steps, ctx.ctl_layers, ctx.dev = sim, steps, ctl_layers, dev
         ctx.dtype = x0.dtype
+        ctx.generation = sim.generation
         return tuple(o.to(dtype=x0.dtype, device=dev) for o in out)
 
     @staticmethod
     def backward(ctx, gx, gv, gF, gC):
         sim = ctx.sim
+        if sim.generation != ctx.generation:
+            raise RuntimeError("the simulator ran another rollout (or forward pass) after this one; its tape is "
+                               "gone -- use one Simulator per rollout that is still to be differentiated")
         n = sim.n
         zeros = lambda *s: torch.zeros((n,) + s, dtype=ctx.dtype, device=ctx.dev)  # noqa: E731
         gx = zeros(3) if gx is None else gx

@@ -73,6 +73,26 @@ def _is_torch(a) -> bool:
     return type(a).__module__.startswith("torch")
 
 
+def _on_one_device(arrays, code):
+    """Borrowed pointers for several arrays that one C call takes with a
+    single on_device flag: when any is a CUDA tensor, host arrays are copied
+    to that device first; otherwise all stay on the host."""
+    dev = None
+    for a in arrays:
+        if a is not None and _is_torch(a) and a.is_cuda:
+            dev = a.device
+            break
+    if dev is not None:
+        import torch
+        arrays = [a if a is None or (_is_torch(a) and a.is_cuda) else
+                  torch.as_tensor(np.asarray(a) if not _is_torch(a) else a).to(dev) for a in arrays]
+    out = [_Arr(a, code) for a in arrays]
+    flag = 1 if dev is not None else 0
+    if any(a.ptr is not None and a.dev != flag for a in out):
+        raise ValueError("arrays are on more than one device")
+    return out, flag
+
+
 class _Arr:
     """A borrowed pointer to caller data for one C call."""
 
@@ -140,6 +160,10 @@ class Simulator:
         L.check(L.lib().dmpm_create(C.byref(self.cfg), C.byref(h)))
         self._h = h
         self.n = 0
+        # bumped by every call that replaces or changes the tape (particles,
+        # layer count, parameters, controls, forward pass), so that a differentiable rollout can tell
+        # whether the tape it recorded is still the sim's
+        self.generation = 0
 
     # -- lifetime ------------------------------------------------------------
     def close(self):
@@ -178,6 +202,7 @@ class Simulator:
             raise ValueError("particle arrays must all be on the host or all on the device")
         L.check(L.lib().dmpm_set_particles(self._h, n, *[a.ptr for a in arrs], code, dev))
         self.n = n
+        self.generation += 1
 
     def set_state(self, state: dict):
         lam = state.get("lam")
@@ -192,10 +217,12 @@ class Simulator:
     def set_sim_params(self, dt, drag, smoothing, f_ext=(0.0, 0.0, 0.0)):
         fe = (C.c_double * 3)(*[float(a) for a in f_ext])
         L.check(L.lib().dmpm_set_sim_params(self._h, float(dt), float(drag), float(smoothing), fe))
+        self.generation += 1
 
     # -- CompGraph -------------------------------------------------------------
     def set_up(self, num_layers: int):
         L.check(L.lib().dmpm_set_up_comp_graph(self._h, int(num_layers)))
+        self.generation += 1
 
     @property
     def num_layers(self) -> int:
@@ -211,6 +238,7 @@ class Simulator:
             L.check(L.lib().dmpm_rollout_forward(self._h, int(start)))
         else:
             L.check(L.lib().dmpm_compute_forward_pass(self._h, int(start)))
+        self.generation += 1
 
     def mass_loss(self) -> float:
         v = C.c_double()
@@ -244,6 +272,9 @@ class Simulator:
     def get_grad(self, layer: int, field: str, dtype=np.float32, device=None):
         f = FIELDS[field]
         out, ptr, dev = self._out(_NCOMP[f], dtype, device)
+        if dev:
+            import torch
+            torch.cuda.current_stream().synchronize()
         L.check(L.lib().dmpm_get_layer_grad(self._h, int(layer), f, ptr,
                                             L.DTYPE_F64 if dtype == np.float64 else L.DTYPE_F32, dev))
         if dev:
@@ -252,17 +283,16 @@ class Simulator:
 
     def set_last_grad(self, x=None, v=None, F=None, C_=None):
         code = L.DTYPE_F64 if self.precision == "f64" else L.DTYPE_F32
-        a = [_Arr(t, code) for t in (x, v, F, C_)]
-        dev = max(t.dev for t in a)
+        a, dev = _on_one_device((x, v, F, C_), code)
         L.check(L.lib().dmpm_set_last_layer_grad(self._h, *[t.ptr for t in a], code, dev))
 
     def accumulate_last_grad(self, F=None, x=None, v=None, C_=None):
         code = L.DTYPE_F64 if self.precision == "f64" else L.DTYPE_F32
-        a = [_Arr(t, code) for t in (F, x, v, C_)]
-        dev = max(t.dev for t in a)
+        a, dev = _on_one_device((F, x, v, C_), code)
         L.check(L.lib().dmpm_accumulate_last_layer_grad(self._h, *[t.ptr for t in a], code, dev))
 
     def set_control(self, layer: int, dFc):
+        self.generation += 1
         if dFc is None:
             L.check(L.lib().dmpm_set_control(self._h, int(layer), None, 0, 0))
             return
@@ -272,9 +302,8 @@ class Simulator:
 
     def set_adam_state(self, layer: int, m, v, vmax):
         code = L.DTYPE_F64 if self.precision == "f64" else L.DTYPE_F32
-        a = [_Arr(t, code) for t in (m, v, vmax)]
-        L.check(L.lib().dmpm_set_adam_state(self._h, int(layer), a[0].ptr, a[1].ptr, a[2].ptr, a[0].code,
-                                            a[0].dev))
+        a, dev = _on_one_device((m, v, vmax), code)
+        L.check(L.lib().dmpm_set_adam_state(self._h, int(layer), a[0].ptr, a[1].ptr, a[2].ptr, code, dev))
 
     # -- on-disk state -------------------------------------------------------
     def write_layer(self, layer: int, path: str, format: str = "native"):
@@ -298,6 +327,9 @@ class Simulator:
 
     def get_control_grad(self, layer: int, dtype=np.float32, device=None):
         out, ptr, dev = self._out(9, dtype, device)
+        if dev:
+            import torch
+            torch.cuda.current_stream().synchronize()
         L.check(L.lib().dmpm_get_control_grad(self._h, int(layer), ptr,
                                               L.DTYPE_F64 if dtype == np.float64 else L.DTYPE_F32, dev))
         if dev:
@@ -384,9 +416,12 @@ def lame_parameters(E, nu):
 class EpisodeGraph:
     """One hipGraph holding a whole episode -- the forward pass from
     `start` (keeping per-layer controls) and the backward pass to `stop` --
-    for each of `sims`, as concurrent branches.  launch() replays it; the
-    sims' inputs (particles, controls, last-layer seed) may be changed between
-    launches through the usual calls."""
+    for each of `sims`, as concurrent branches.  launch() replays it.
+    Between launches the controls of layers that had one at capture, the
+    last-layer seed and render gradients set before the capture may be
+    changed through the usual calls; calls that would reallocate the sims'
+    buffers (new particles, a new layer count, new parameters) are refused
+    until the graph is closed."""
 
     def __init__(self, sims, start: int = 0, stop: int = 0):
         arr = (C.c_void_p * len(sims))(*[s.handle for s in sims])

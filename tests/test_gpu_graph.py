@@ -77,3 +77,93 @@ def test_sort_refused_while_a_graph_holds_the_sim():
     g.close()
     sim.sort_particles(2)
     sim.close()
+
+
+def test_graph_inputs_between_replays_and_refusals():
+    """Between replays a capture takes new controls (on layers that had one),
+    a new last-layer seed and new render gradients, each updated in place; a
+    replay then matches an eager episode of a fresh sim on the same inputs.
+    Calls that would reallocate the captured buffers or change the captured
+    parameters are refused while the graph lives, and a graph whose sim was
+    destroyed refuses to launch."""
+    from dmpm import EpisodeGraph
+    from dmpm._lib import DmpmError
+    import ctypes as C
+    from dmpm import _lib as L
+
+    a, T = _instance(1500, 16, 1, (0.2, 0.22, 0.18), 1e4, 0.3)
+    n = a.n
+    rng = np.random.default_rng(77)
+    rgF0 = 1e-3 * rng.normal(size=(n, 3, 3))
+    rgx0 = 1e-2 * rng.normal(size=(n, 3))
+
+    def set_rg(sim, gF, gx):
+        gF = np.ascontiguousarray(gF)
+        gx = np.ascontiguousarray(gx)
+        L.check(L.lib().dmpm_set_render_gradients(sim.handle, n, gF.ctypes.data_as(C.c_void_p),
+                                                   gx.ctypes.data_as(C.c_void_p), L.DTYPE_F64, 0))
+
+    set_rg(a, rgF0, rgx0)
+    print("[graph_inputs] capture", flush=True)
+    g = EpisodeGraph([a], start=0, stop=0)
+    # new inputs, updated in place
+    ctl1 = 2e-3 * rng.normal(size=(n, 3, 3))
+    seed_x = rng.normal(size=(n, 3))
+    rgF1 = 1e-3 * rng.normal(size=(n, 3, 3))
+    rgx1 = 1e-2 * rng.normal(size=(n, 3))
+    a.set_control(1, ctl1)
+    a.set_last_grad(x=seed_x)
+    set_rg(a, rgF1, rgx1)
+    # the eager episode of a fresh sim with the same inputs
+    b, _ = _instance(1500, 16, 1, (0.2, 0.22, 0.18), 1e4, 0.3)
+    b.set_control(1, ctl1)
+    b.set_last_grad(x=seed_x)
+    set_rg(b, rgF1, rgx1)
+    b.forward(0, keep_controls=True)
+    b.backward(0)
+    want = _results(b, T)
+    for rep in range(2):
+        print(f"[graph_inputs] replay {rep}", flush=True)
+        g.launch()
+        g.synchronize()
+        got = _results(a, T)
+        for k in want:
+            assert rel_err(got[k], want[k]) < 1e-12, (rep, k)
+        # eager work on the captured sim itself between replays
+        a.forward(0, keep_controls=True)
+        a.backward(0)
+        a.set_last_grad(x=seed_x)
+        a.synchronize()
+    # a control cleared between replays is zeroed in place
+    a.set_control(1, None)
+    b.set_control(1, None)
+    b.set_last_grad(x=seed_x)
+    b.forward(0, keep_controls=True)
+    b.backward(0)
+    want = _results(b, T)
+    g.launch()
+    g.synchronize()
+    got = _results(a, T)
+    for k in ("x", "F", "gx", "gv", "gF"):
+        assert rel_err(got[k], want[k]) < 1e-12, k
+    # refused while the graph lives
+    s = a.get(0, "x", dtype=np.float64)
+    for call in (lambda: a.set_up(T + 1),
+                 lambda: a.set_particles(s, s, np.zeros((n, 3, 3)), np.zeros((n, 3, 3)), np.ones(n), np.ones(n),
+                                         np.ones(n), np.ones(n)),
+                 lambda: a.set_control(3, ctl1),
+                 lambda: a.set_sim_params(1e-3, 0.0, 0.0),
+                 lambda: a.sort_particles(1)):
+        with pytest.raises(DmpmError):
+            call()
+    g.close()
+    a.set_up(T + 1)  # allowed again
+    # a graph whose sim is gone refuses to launch
+    c, _ = _instance(900, 12, 3, (0.2, 0.2, 0.2), 1e4, 0.3)
+    g2 = EpisodeGraph([c], start=0, stop=0)
+    c.close()
+    with pytest.raises(DmpmError):
+        g2.launch()
+    g2.close()
+    for sim in (a, b):
+        sim.close()
