@@ -217,6 +217,11 @@ int dmpm_descend_adam(dmpm_sim* sim, int layer, double alpha, double grad_norm, 
 /* optimize_single_timestep(...) (bind.cpp:294-317, CompGraph.cpp:369-465). */
 int dmpm_optimize_single_timestep(dmpm_sim* sim, int timestep_idx, int max_gd_iters, int current_episode,
                                   double initial_alpha, int max_line_search_iters);
+/* The Adam step count OptimizeSingleTimestep shares across calls and
+ * graphs: the reference keeps it in a function-local static
+ * (CompGraph.cpp:396), so it lives for the process.  Writes it to *count
+ * (may be NULL), then sets it to 0 when reset is 1 (a fresh process). */
+int dmpm_single_timestep_adam_count(int* count, int reset);
 /* run_optimization(opt) (bind.cpp:165-172, CompGraph.cpp:213-368). */
 int dmpm_run_optimization(dmpm_sim* sim, int num_steps, double dt, double drag, const double f_ext[3],
                           int control_stride, int max_gd_iters, int max_line_search_iters, double initial_alpha,

@@ -401,6 +401,53 @@ class Ref:
         o["loss"] = float(loss)
         return o
 
+    def rollout(self, prm: Params, s: dict, steps: int, seed: Optional[dict] = None) -> dict:
+        """`steps` ForwardTimeStep calls from layer 0 (dFc of layer 0 as
+        given, zero after) and, with `seed` (keys x, v, F, C: the last layer's
+        adjoint), `steps` Back_Timestep calls.  Returns the last layer's x, v,
+        F, C and, with a seed, layer 0's gradients gx, gv, gF, gC."""
+        n = s["x"].shape[0]
+        dims, mn = self._grid(prm)
+        args = self._state(s)
+        o = {k: np.zeros((n, 3), np.float32) for k in ("x", "v", "gx", "gv")}
+        o.update({k: np.zeros((n, 3, 3), np.float32) for k in ("F", "C", "gF", "gC")})
+        sd = [None] * 4 if seed is None else [self._f3(seed[k]) for k in ("x", "v", "F", "C")]
+        fe = self._f3(prm.f_ext)
+        self._lib.ref_rollout(n, *[_ptr(a) for a in args], dims, C.c_float(prm.dx), _ptr(mn), int(steps),
+                              C.c_float(prm.smoothing), C.c_float(prm.dt), C.c_float(prm.drag), _ptr(fe),
+                              *[_ptr(a) for a in sd], _ptr(o["x"]), _ptr(o["v"]), _ptr(o["F"]), _ptr(o["C"]),
+                              _ptr(o["gx"]), _ptr(o["gv"]), _ptr(o["gF"]), _ptr(o["gC"]))
+        if seed is None:
+            for k in ("gx", "gv", "gF", "gC"):
+                del o[k]
+        return o
+
+    def optimize(self, prm: Params, s: dict, target_m, num_steps: int, control_stride: int, max_gd_iters: int,
+                 max_ls_iters: int, initial_alpha: float, gd_tol: float, single: Optional[dict] = None) -> dict:
+        """CompGraph::OptimizeDefGradControlSequence with prm's dt, drag,
+        f_ext and smoothing, then (with `single` = dict(t, iters, alpha, ls))
+        CompGraph::OptimizeSingleTimestep on the same graph.  Returns the
+        end-layer loss after each, every layer's dFc after each, and the
+        single-timestep layer's Adam state."""
+        n = s["x"].shape[0]
+        dims, mn = self._grid(prm)
+        args = self._state(s)
+        del args[4]  # no dFc
+        fe = self._f3(prm.f_ext)
+        loss = np.zeros(2, np.float32)
+        d_run = np.zeros((num_steps, n, 3, 3), np.float32)
+        d_single = np.zeros((num_steps, n, 3, 3), np.float32)
+        adam = np.zeros((3, n, 3, 3), np.float32)
+        sg = single or dict(t=0, iters=0, alpha=1.0, ls=1)
+        self._lib.ref_optimize(n, *[_ptr(a) for a in args], dims, C.c_float(prm.dx), _ptr(mn),
+                               _ptr(self._f3(target_m)), int(num_steps), C.c_float(prm.dt), C.c_float(prm.drag),
+                               _ptr(fe), int(control_stride), int(max_gd_iters), int(max_ls_iters),
+                               C.c_float(initial_alpha), C.c_float(gd_tol), C.c_float(prm.smoothing), int(sg["t"]),
+                               int(sg["iters"]), C.c_float(sg["alpha"]), int(sg["ls"]), _ptr(loss), _ptr(d_run),
+                               _ptr(d_single), _ptr(adam))
+        return dict(loss_run=float(loss[0]), loss_single=float(loss[1]), dFc_run=d_run, dFc_single=d_single,
+                    adam_m=adam[0], adam_v=adam[1], adam_vmax=adam[2])
+
     def mass_loss(self, prm: Params, s: dict, target_m):
         n = s["x"].shape[0]
         dims, mn = self._grid(prm)

@@ -436,3 +436,103 @@ void ref_lame(float E, float nu, float* lam, float* mu) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// `steps` ForwardTimeStep calls from one layer (ForwardSimulation.cpp:159-185,
+// each from a layer whose dFc is the given one for the first step and zero
+// after, as P_op_2 writes it, :156), then -- when a seed is given -- the
+// seed as the last layer's adjoint and `steps` Back_Timestep calls
+// (BackPropagation.cpp:9-144) with the per-layer gradient reset of
+// ComputeBackwardPass (CompGraph.cpp:199-209).  Outputs: the last layer's
+// x, v, F, C and (with a seed) layer 0's dLdx, dLdv, dLdF, dLdC.
+void ref_rollout(int n, const float* x, const float* v, const float* F, const float* C, const float* dFc,
+                 const float* m, const float* vol, const float* lam, const float* mu, const int* dims, float dx,
+                 const float* grid_min, int steps, float smoothing, float dt, float drag, const float* f_ext,
+                 const float* sx, const float* sv, const float* sF, const float* sC, float* out_x, float* out_v,
+                 float* out_F, float* out_C, float* gx, float* gv, float* gF, float* gC) {
+    RefParticles s{n, x, v, F, C, dFc, m, vol, lam, mu};
+    std::vector<CompGraphLayer> layers(steps + 1);
+    layers[0].point_cloud = make_pc(s);
+    layers[0].grid = make_grid(dims, dx, grid_min);
+    for (int i = 1; i <= steps; ++i) {
+        layers[i].point_cloud = std::make_shared<PointCloud>(*layers[0].point_cloud);
+        layers[i].grid = std::make_shared<Grid>(*layers[0].grid);
+    }
+    for (int i = 0; i < steps; ++i)
+        SingleThreadMPM::ForwardTimeStep(*layers[i + 1].point_cloud, *layers[i].point_cloud, *layers[i].grid,
+                                         smoothing, dt, drag, load_v3(f_ext), 0);
+    const PointCloud& last = *layers[steps].point_cloud;
+    for (int p = 0; p < n; ++p) {
+        store_v3(last.points[p].x, out_x + 3 * p);
+        store_v3(last.points[p].v, out_v + 3 * p);
+        store_m3(last.points[p].F, out_F + 9 * p);
+        store_m3(last.points[p].C, out_C + 9 * p);
+    }
+    if (!sx) return;
+    PointCloud& lp = *layers[steps].point_cloud;
+    lp.ResetGradients();
+    for (int p = 0; p < n; ++p) {
+        lp.points[p].dLdx = load_v3(sx + 3 * p);
+        lp.points[p].dLdv = load_v3(sv + 3 * p);
+        lp.points[p].dLdF = load_m3(sF + 9 * p);
+        lp.points[p].dLdC = load_m3(sC + 9 * p);
+    }
+    for (int i = steps - 1; i >= 0; --i) {
+        layers[i].grid->ResetGradients();
+        layers[i].point_cloud->ResetGradients();
+        Back_Timestep(layers[i + 1], layers[i], drag, dt, smoothing);
+    }
+    const PointCloud& l0 = *layers[0].point_cloud;
+    for (int p = 0; p < n; ++p) {
+        store_v3(l0.points[p].dLdx, gx + 3 * p);
+        store_v3(l0.points[p].dLdv, gv + 3 * p);
+        store_m3(l0.points[p].dLdF, gF + 9 * p);
+        store_m3(l0.points[p].dLdC, gC + 9 * p);
+    }
+}
+
+// CompGraph::OptimizeDefGradControlSequence (CompGraph.cpp:213-368; bound as
+// run_optimization, bind.cpp:165-172) on CompGraph(pc, grid, target), then,
+// when st_iters > 0, CompGraph::OptimizeSingleTimestep(st_t, st_iters, 0,
+// st_alpha, st_ls) (CompGraph.cpp:369-465) on the same graph, whose
+// simulation parameters the first call set.  Outputs: the end-layer mass
+// loss after each call (out_loss[0], out_loss[1]); every layer's dFc after
+// the first call (num_steps x n x 9) and after the second (same layout);
+// the Adam state (momentum, vector, vector_max) of layer st_t after the
+// second call (3 x n x 9).
+void ref_optimize(int n, const float* x, const float* v, const float* F, const float* C, const float* m,
+                  const float* vol, const float* lam, const float* mu, const int* dims, float dx,
+                  const float* grid_min, const float* target_m, int num_steps, float dt, float drag,
+                  const float* f_ext, int control_stride, int max_gd_iters, int max_ls_iters, float initial_alpha,
+                  float gd_tol, float smoothing, int st_t, int st_iters, float st_alpha, int st_ls, float* out_loss,
+                  float* dfc_run, float* dfc_single, float* adam_single) {
+    RefParticles s{n, x, v, F, C, nullptr, m, vol, lam, mu};
+    auto pc0 = make_pc(s);
+    auto grid0 = make_grid(dims, dx, grid_min);
+    auto tgt = make_grid(dims, dx, grid_min);
+    const size_t G = (size_t)dims[0] * dims[1] * dims[2];
+    for (size_t i = 0; i < G; ++i) tgt->nodes[i].m = target_m[i];
+    CompGraph cg(pc0, grid0, tgt);
+    cg.OptimizeDefGradControlSequence(num_steps, dt, drag, load_v3(f_ext), control_stride, max_gd_iters,
+                                      max_ls_iters, initial_alpha, gd_tol, smoothing, 0);
+    auto dump_dfc = [&](float* out) {
+        for (int l = 0; l < num_steps; ++l)
+            for (int p = 0; p < n; ++p)
+                store_m3(cg.layers[l].point_cloud->points[p].dFc, out + ((size_t)l * n + p) * 9);
+    };
+    out_loss[0] = cg.EndLayerMassLoss();
+    dump_dfc(dfc_run);
+    if (st_iters <= 0) return;
+    cg.OptimizeSingleTimestep(st_t, st_iters, 0, st_alpha, st_ls);
+    out_loss[1] = cg.EndLayerMassLoss();
+    dump_dfc(dfc_single);
+    const PointCloud& pc = *cg.layers[st_t].point_cloud;
+    for (int p = 0; p < n; ++p) {
+        store_m3(pc.points[p].momentum, adam_single + ((size_t)0 * n + p) * 9);
+        store_m3(pc.points[p].vector, adam_single + ((size_t)1 * n + p) * 9);
+        store_m3(pc.points[p].vector_max, adam_single + ((size_t)2 * n + p) * 9);
+    }
+}
+
+}  // extern "C"

@@ -217,10 +217,14 @@ DMPM_DI void svd3(const m3<T>& F, m3<T>& U, v3<T>& sig, m3<T>& V) {
         jacobi_rot(S.a, V, 0, 2);
         jacobi_rot(S.a, V, 1, 2);
     }
+    // Sort by adjacent swaps only, so that equal eigenvalues keep their
+    // order: with |sigma_2| = |sigma_3| and det F < 0 the closest rotation is
+    // not unique, and this order gives the reference's choice
+    // (diag(0.5, -0.5, 1) -> R = I, as ImplicitQRSVD.h's polar factor).
     T e[3] = {S.a[0], S.a[4], S.a[8]};
     if (e[0] < e[1]) swapcols(V, e, 0, 1);
-    if (e[0] < e[2]) swapcols(V, e, 0, 2);
     if (e[1] < e[2]) swapcols(V, e, 1, 2);
+    if (e[0] < e[1]) swapcols(V, e, 0, 1);
     m3<T> B = mmul(F, V);
     v3<T> b0 = {{B.a[0], B.a[3], B.a[6]}}, b1 = {{B.a[1], B.a[4], B.a[7]}}, b2 = {{B.a[2], B.a[5], B.a[8]}};
     T n0 = sqrt(vdot(b0, b0));

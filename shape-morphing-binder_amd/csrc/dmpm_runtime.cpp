@@ -1725,6 +1725,13 @@ int dmpm_optimize_single_timestep(dmpm_sim* s, int t, int max_gd_iters, int /*ep
     });
 }
 
+int dmpm_single_timestep_adam_count(int* count, int reset) {
+    return guard([&] {
+        if (count) *count = g_single_timestep_adam_t;
+        if (reset) g_single_timestep_adam_t = 0;
+    });
+}
+
 int dmpm_run_optimization(dmpm_sim* s, int num_steps, double dt, double drag, const double f_ext[3],
                           int control_stride, int max_gd_iters, int max_ls, double initial_alpha, double gd_tol,
                           double smoothing, int /*current_episodes*/, double* final_loss) {

@@ -133,6 +133,7 @@ def lib() -> C.CDLL:
         "dmpm_descend_adam": (i, [vp, i, d, d, d, d, d, i]),
         "dmpm_optimize_single_timestep": (i, [vp, i, i, i, d, i]),
         "dmpm_run_optimization": (i, [vp, i, d, d, dp, i, i, i, d, d, d, i, dp]),
+        "dmpm_single_timestep_adam_count": (i, [ip, i]),
         "dmpm_p2g_mass": (i, [cfgp, i, vp, vp, vp, i, i]),
         "dmpm_calculate_volumes": (i, [cfgp, i, vp, vp, vp, vp, i, i]),
         "dmpm_lame_parameters": (i, [d, d, dp, dp]),
@@ -177,7 +178,8 @@ EXPORTED = [
     "dmpm_accumulate_last_layer_grad", "dmpm_set_last_layer_grad", "dmpm_set_render_gradients",
     "dmpm_clear_render_gradients", "dmpm_render_gradient_info", "dmpm_promote_last_as_initial", "dmpm_sort_particles",
     "dmpm_set_control", "dmpm_request_control_grad", "dmpm_get_control_grad", "dmpm_control_grad_norm",
-    "dmpm_descend_adam", "dmpm_optimize_single_timestep", "dmpm_run_optimization", "dmpm_p2g_mass",
+    "dmpm_descend_adam", "dmpm_optimize_single_timestep", "dmpm_run_optimization",
+    "dmpm_single_timestep_adam_count", "dmpm_p2g_mass",
     "dmpm_calculate_volumes", "dmpm_lame_parameters", "dmpm_last_timing", "dmpm_memory", "dmpm_kernel_timer",
     "dmpm_kernel_stats", "dmpm_window_stats", "dmpm_window_counters", "dmpm_graph_capture", "dmpm_graph_launch", "dmpm_graph_synchronize", "dmpm_graph_destroy",
     "dmpm_get_config", "dmpm_set_adam_state", "dmpm_file_info", "dmpm_write_file", "dmpm_read_file",

@@ -38,6 +38,154 @@ def scene(n=400, cells=16, seed=7):
     return sc, s
 
 
+def compressive_scene(n=400, cells=16, seed=31, a=20.0):
+    """A compressed, converging body for the J trust region
+    (ForwardSimulation.cpp:136-152): det F uniform in [0.55, 0.7], a control
+    dFc that moves det(F + dFc) by a few percent, and a velocity field
+    -a g(x) (x - c) whose convergence g runs from 1.1 to -0.3 across x, so
+    that with dt = 2e-3 some particles expand (tr C > 0), some have
+    J0 < J_min (s clamped to 0) and some J_min < J0 with dt tr C below
+    J_min/J0 - 1 (s in (0, 1))."""
+    sc = make_scene(Body((0.5, 0.5, 0.5), (0.2, 0.2, 0.2)), n, cells, seed, 1e4, 0.3, h=1 / 32)
+    st = {k: np.asarray(v, np.float64) for k, v in sc.state().items()}
+    rng = np.random.default_rng(seed + 1)
+    J = rng.uniform(0.55, 0.7, size=n)
+    st["F"] = (J ** (1 / 3))[:, None, None] * (np.eye(3) + 0.04 * rng.normal(size=(n, 3, 3)))
+    r = st["x"] - 0.5
+    g = 1.1 - 1.4 * (r[:, 0] / 0.2 + 1) / 2
+    st["v"] = -a * g[:, None] * r + 0.2 * rng.normal(size=(n, 3))
+    st["C"] = -a * g[:, None, None] * np.eye(3) + 2.0 * rng.normal(size=(n, 3, 3))
+    st["dFc"] = 0.02 * rng.normal(size=(n, 3, 3))
+    return sc, {k: v.astype(np.float32) for k, v in st.items()}
+
+
+def trust_fixtures():
+    """11. The J trust region in the reference's own step, adjoint and
+    CompGraph episode, on the compressive scene (dt = 2e-3)."""
+    ref = Ref(threads=1)
+    sc, s = compressive_scene()
+    prm = reference_params(dims=sc.dims, dx=sc.dx, dt=2e-3, drag=0.5, smoothing=0.1, f_ext=(0.0, -9.8, 0.0))
+    meta = dict(dims=np.array(prm.dims), dx=np.float64(prm.dx), dt=np.float64(prm.dt), drag=np.float64(prm.drag),
+                smoothing=np.float64(prm.smoothing), f_ext=np.array(prm.f_ext))
+    ins = {f"in_{k}": v for k, v in s.items()}
+    o = ref.forward_step(prm, s)
+    np.savez_compressed(os.path.join(HERE, "ref_trust_forward.npz"), **meta, **ins,
+                        **{f"out_{k}": o[k] for k in ("x", "v", "F", "C", "P", "grid_m", "grid_v")})
+    rng = np.random.default_rng(41)
+    adj = dict(x=rng.normal(size=(sc.n, 3)), v=rng.normal(size=(sc.n, 3)), F=rng.normal(size=(sc.n, 3, 3)),
+               C=rng.normal(size=(sc.n, 3, 3)))
+    adj = {k: v.astype(np.float32) for k, v in adj.items()}
+    b = ref.backward_step(prm, s, adj)
+    np.savez_compressed(os.path.join(HERE, "ref_trust_backward.npz"), **meta, **ins,
+                        **{f"adj_{k}": v for k, v in adj.items()},
+                        **{f"out_{k}": b[k] for k in ("x", "v", "F", "C", "P")})
+    tsc = make_scene(Body((0.5, 0.5, 0.5), (0.17, 0.17, 0.17)), 400, 16, 42, 1e4, 0.3, h=1 / 32)
+    tm = ref.p2g_mass(prm, tsc.x.astype(np.float32), tsc.m.astype(np.float32))
+    rgF = (1e-3 * rng.normal(size=(sc.n, 3, 3))).astype(np.float32)
+    rgx = (1e-2 * rng.normal(size=(sc.n, 3))).astype(np.float32)
+    e = ref.episode(prm, s, tm, num_layers=4, control_layer=0, render_gF=rgF, render_gx=rgx)
+    np.savez_compressed(os.path.join(HERE, "ref_trust_episode.npz"), **meta, **ins, target_m=tm, render_gF=rgF,
+                        render_gx=rgx, num_layers=np.int64(4), out_loss=np.float64(e["loss"]),
+                        **{f"out_{k}": e[k] for k in ("x", "v", "F", "C", "gx", "gv", "gF", "gC")})
+    print("wrote the trust-region fixtures")
+
+
+def degenerate_elasticity():
+    """12. PK_FixedCorotatedElasticity, d2_FCE_psi_dF2_mult_by_dF and the
+    polar decomposition (Elasticity.cpp:45-117, ImplicitQRSVD.h:852-875) on
+    degenerate F: scaled rotations small enough that |det(tr S I - S)| <
+    1e-6 (the +1e-4 I regularisation, Elasticity.cpp:97-100); repeated
+    singular values; exactly representable diagonal and permutation inputs,
+    inverted ones with sigma_2 + sigma_3 = 0 among them; and inverted F with
+    sigma_2 + sigma_3 = +-1e-2, +-2e-2 (near the degeneracy, where the
+    reference's float32 polar factor is good to about 1e-7 / 1e-2)."""
+    ref = Ref(1)
+    rng = np.random.default_rng(12)
+
+    def rot(k):
+        Q = np.linalg.qr(rng.normal(size=(k, 3, 3)))[0]
+        Q[np.linalg.det(Q) < 0, :, 0] *= -1
+        return Q
+
+    groups = {}
+    Q = rot(6)
+    groups["regularised"] = np.array([c * Q[i] for i, c in enumerate([0.004, 0.0035, 0.003, 0.002, 0.0045, 0.001])])
+    U, V = rot(6), rot(6)
+    sig = [(0.8, 0.8, 1.2), (1.2, 0.9, 0.9), (0.7, 0.7, 0.7), (1.3, 1.3, 1.3), (1.1, 1.1, 0.6), (0.9, 1.4, 1.4)]
+    groups["repeated"] = np.array([U[i] @ np.diag(sg) @ V[i].T for i, sg in enumerate(sig)])
+    groups["exact"] = np.array([np.eye(3), np.diag([1.1, 0.7, 0.7]), np.diag([2.0, 2.0, 2.0]),
+                                np.diag([-1.0, -1.0, 1.0]), np.diag([1.1, 0.7, -0.7]), np.diag([1.0, 1.0, -1.0]),
+                                np.diag([0.5, -0.5, 1.0]), np.array([[0, 1.0, 0], [1.0, 0, 0], [0, 0, 1.0]])])
+    U, V = rot(4), rot(4)
+    groups["near"] = np.array([U[i] @ np.diag([1.2, 0.5, -0.5 + d]) @ V[i].T for i, d in
+                               enumerate([1e-2, -1e-2, 2e-2, -2e-2])])
+    lam, mu = ref.lame(1e4, 0.3)
+    out = dict(lam=np.float64(lam), mu=np.float64(mu))
+    for name, F in groups.items():
+        F = F.astype(np.float32)
+        dF = rng.normal(size=F.shape).astype(np.float32)
+        R, S = ref.polar(F)
+        out.update({f"{name}_F": F, f"{name}_dF": dF, f"{name}_R": R, f"{name}_S": S,
+                    f"{name}_P": ref.pk_fcr(F, lam, mu), f"{name}_hvp": ref.pk_fcr_hvp(F, lam, mu, dF)})
+    np.savez_compressed(os.path.join(HERE, "ref_elasticity_degenerate.npz"), **out)
+    print("wrote the degenerate elasticity fixture")
+
+
+def c1_fixture():
+    """13. The reference's gradient at the 8,192-particle size: the sphere of
+    radius 0.2 on 32^3 cells (h = dx/2), E = 1e4, nu = 0.3, set in motion by
+    a seeded sine velocity field (amplitude 0.5), in the reference's own
+    physics (gravity, drag 0.5, smoothing 0.1), dt = 5e-4, 5 ForwardTimeSteps,
+    then 5 Back_Timesteps seeded with the gradient of the positional loss
+    sum |x_T - x*|^2, x* = c + diag(1.2, 0.8, 1.0)(x_0 - c)."""
+    ref = Ref(1)
+    sc = make_scene(Body((0.5, 0.5, 0.5), (0.2, 0.2, 0.2)), 8192, 32, 1, 1e4, 0.3, h=1.0 / 64, velocity="sine",
+                    amplitude=0.5)
+    s = {k: np.asarray(v, np.float32) for k, v in sc.state().items()}
+    prm = reference_params(dims=sc.dims, dx=sc.dx, dt=5e-4, drag=0.5, smoothing=0.1, f_ext=(0.0, -9.8, 0.0))
+    T = 5
+    fwd = ref.rollout(prm, s, T)
+    c = np.float32(0.5)
+    xs = (c + (s["x"] - c) * np.array([1.2, 0.8, 1.0], np.float32)).astype(np.float32)
+    seed = dict(x=(2 * (fwd["x"] - xs)).astype(np.float32), v=np.zeros((sc.n, 3), np.float32),
+                F=np.zeros((sc.n, 3, 3), np.float32), C=np.zeros((sc.n, 3, 3), np.float32))
+    o = ref.rollout(prm, s, T, seed)
+    for k in ("x", "v", "F", "C"):
+        assert np.array_equal(o[k], fwd[k]), "the reference's single-threaded rollout must repeat exactly"
+    np.savez_compressed(os.path.join(HERE, "ref_c1_rollout.npz"), dims=np.array(prm.dims), dx=np.float64(prm.dx),
+                        dt=np.float64(prm.dt), drag=np.float64(prm.drag), smoothing=np.float64(prm.smoothing),
+                        f_ext=np.array(prm.f_ext), steps=np.int64(T), in_x=s["x"], in_v=s["v"], in_m=s["m"],
+                        in_vol=s["vol"], in_lam=s["lam"], in_mu=s["mu"], target_x=xs,
+                        **{f"out_{k}": o[k] for k in ("x", "v", "F", "C", "gx", "gv", "gF", "gC")})
+    print("wrote the 8,192-particle reference rollout")
+
+
+def optimizer_fixture():
+    """14. CompGraph::OptimizeDefGradControlSequence (run_optimization) and
+    then OptimizeSingleTimestep on the same graph (CompGraph.cpp:213-465), on
+    a 400-particle body whose target is the same body shifted and squashed:
+    4 layers, control stride 1, 2 GD iterations of at most 4 line-search
+    tries, alpha 0.2, gd_tol 1e-3, dt 1e-3, drag 0.5, smoothing 0.1, gravity;
+    then layer 1 for 3 GD iterations of at most 5 tries at alpha 0.1."""
+    ref = Ref(1)
+    sc = make_scene(Body((0.5, 0.5, 0.5), (0.2, 0.2, 0.2)), 400, 16, 51, 1e4, 0.3, h=1 / 32, velocity="sine",
+                    amplitude=0.3)
+    s = {k: np.asarray(v, np.float32) for k, v in sc.state().items()}
+    prm = reference_params(dims=sc.dims, dx=sc.dx, dt=1e-3, drag=0.5, smoothing=0.1, f_ext=(0.0, -9.8, 0.0))
+    tsc = make_scene(Body((0.52, 0.49, 0.5), (0.22, 0.17, 0.2)), 400, 16, 52, 1e4, 0.3, h=1 / 32)
+    tm = ref.p2g_mass(prm, tsc.x.astype(np.float32), tsc.m.astype(np.float32))
+    args = dict(num_steps=4, control_stride=1, max_gd_iters=2, max_ls_iters=4, initial_alpha=0.2, gd_tol=1e-3)
+    single = dict(t=1, iters=3, alpha=0.1, ls=5)
+    o = ref.optimize(prm, s, tm, single=single, **args)
+    np.savez_compressed(os.path.join(HERE, "ref_optimize.npz"), dims=np.array(prm.dims), dx=np.float64(prm.dx),
+                        dt=np.float64(prm.dt), drag=np.float64(prm.drag), smoothing=np.float64(prm.smoothing),
+                        f_ext=np.array(prm.f_ext), **{f"in_{k}": v for k, v in s.items()}, target_m=tm,
+                        **{f"arg_{k}": np.float64(v) for k, v in args.items()},
+                        **{f"single_{k}": np.float64(v) for k, v in single.items()},
+                        **{f"out_{k}": (np.float64(v) if np.isscalar(v) else v) for k, v in o.items()})
+    print("wrote the optimiser fixture: losses", o["loss_run"], o["loss_single"])
+
+
 def main():
     ref = Ref(threads=1)
     sc, s = scene()
@@ -110,6 +258,10 @@ def main():
                         mu=np.float64(mu), R=R, S=S, P=ref.pk_fcr(F, lam, mu), hvp=ref.pk_fcr_hvp(F, lam, mu, dF),
                         psi=ref.energy_fcr(F, lam, mu))
     print("wrote golden vectors to", HERE)
+    trust_fixtures()
+    degenerate_elasticity()
+    c1_fixture()
+    optimizer_fixture()
     io_fixture()
     mesh_fixture()
     assets_fixture()
@@ -204,7 +356,11 @@ def assets_fixture():
 
 
 if __name__ == "__main__":
-    if sys.argv[1:] == ["assets"]:
+    extra = {"trust": trust_fixtures, "degenerate": degenerate_elasticity, "c1": c1_fixture,
+             "optimize": optimizer_fixture}
+    if sys.argv[1:] and sys.argv[1] in extra:
+        extra[sys.argv[1]]()
+    elif sys.argv[1:] == ["assets"]:
         assets_fixture()
     elif sys.argv[1:] == ["io"]:
         io_fixture()

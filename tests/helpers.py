@@ -57,3 +57,22 @@ def rel_err(a, b, floor=1e-30):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.abs(a - b).max() / max(np.abs(b).max(), floor))
+
+
+def trust_branches(prm: Params, s: dict):
+    """Which particles take which branch of the J trust region
+    (ForwardSimulation.cpp:136-152) in one step from state s: the step's G2P
+    C' is computed with the trust region off (FP64 oracle), J0 = max(det(F +
+    dFc), 1e-6), s_J = (J_min / J0 - 1) / min(-1e-6, dt tr C').  Returns
+    boolean masks (scaled: tr C' < 0 and 0 < s_J < 1; clamped: tr C' < 0
+    and s_J <= 0)."""
+    from dataclasses import replace
+    from oracle import Oracle
+    s64 = {k: (None if v is None else np.asarray(v, np.float64)) for k, v in s.items()}
+    raw = Oracle("f64").forward_step(replace(prm, trust_region=0), s64)
+    Ft = s64["F"] + (s64["dFc"] if s64.get("dFc") is not None else 0.0)
+    J0 = np.maximum(np.linalg.det(Ft), 1e-6)
+    trC = np.trace(raw["C"], axis1=1, axis2=2)
+    sj = (prm.jmin / J0 - 1.0) / np.minimum(-1e-6, prm.dt * trC)
+    neg = trC < 0
+    return neg & (sj > 0) & (sj < 1), neg & (sj <= 0)

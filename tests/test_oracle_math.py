@@ -226,3 +226,53 @@ def test_reference_adjoint_doubles_the_F_term():
     assert rel_err(rf["F"] - ex["F"], extra) < 1e-10
     for k in ("x", "v", "C"):
         assert rel_err(rf[k], ex[k]) < 1e-12, k
+
+
+def test_exact_adjoint_through_the_scaled_trust_region():
+    """Central differences of a 3-step FP64 rollout of the compressive body
+    (the reference-physics fixture's inputs, dt = 2e-3) under the exact
+    adjoint, where more than 10 % of the particles take the scaled branch of
+    the J trust region in the first step: the derivative of s_J through
+    tr C' and det(F + dFc) (ForwardSimulation.cpp:136-152) is exercised in
+    every input direction, the control dFc included."""
+    import os
+    from helpers import trust_branches
+    d = dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_trust_forward.npz"),
+                     allow_pickle=False))
+    s0 = {k[3:]: np.asarray(d[k], np.float64) for k in d if k.startswith("in_")}
+    prm = Params(dims=tuple(int(a) for a in d["dims"]), dx=float(d["dx"]), dt=float(d["dt"]),
+                 drag=float(d["drag"]), smoothing=float(d["smoothing"]), f_ext=tuple(float(a) for a in d["f_ext"]),
+                 order=3, model=0, trust_region=1, zeta=0.25, jmin=0.6, wall_cells=0, adjoint=0)
+    scaled, clamped = trust_branches(prm, s0)
+    n = s0["x"].shape[0]
+    assert scaled.sum() > 0.1 * n and clamped.sum() > 0.1 * n
+    steps = 3
+    dfc0 = s0.pop("dFc")
+    rng = np.random.default_rng(19)
+    ctl = [dfc0] + [1e-3 * rng.normal(size=(n, 3, 3)) for _ in range(steps - 1)]
+    gT = random_adjoint(n, seed=23)
+
+    def L(s, c):
+        lay = O64.rollout(prm, s, steps, c)
+        return sum((lay[-1][k] * gT[k]).sum() for k in gT)
+
+    layers = O64.rollout(prm, dict(s0, dFc=None), steps, ctl)
+    adj, gctl = O64.rollout_grad(prm, layers, gT)
+    h = 1e-7
+    for key in ("x", "v", "F", "C", "ctl0", "ctl1"):
+        dd = rng.normal(size=(n, 3) if key in ("x", "v") else (n, 3, 3))
+        if key.startswith("ctl"):
+            t = int(key[3:])
+            cp = [c.copy() for c in ctl]
+            cm = [c.copy() for c in ctl]
+            cp[t] += h * dd
+            cm[t] -= h * dd
+            fd = (L(s0, cp) - L(s0, cm)) / (2 * h)
+            an = (gctl[t] * dd).sum()
+        else:
+            sp, sm = dict(s0), dict(s0)
+            sp[key] = s0[key] + h * dd
+            sm[key] = s0[key] - h * dd
+            fd = (L(sp, ctl) - L(sm, ctl)) / (2 * h)
+            an = (adj[key] * dd).sum()
+        assert abs(fd - an) <= 1e-5 * max(abs(fd), 1e-8), f"d{key}: fd {fd:.10e} adjoint {an:.10e}"
