@@ -39,9 +39,25 @@ sys.path.insert(0, os.path.join(ROOT, "shape-morphing-binder_amd"))
 METRIC = "fwd+bwd particle-steps/s at 1M pts,128³ grid; % HBM peak; grad err vs FP64"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per launch, FP32 state (4-byte elements): per particle and
-# per active grid node (a node any particle's stencil touches).  The adjoint
-# kernels read G2P's v' and C' of step t from layer t+1 (no recompute).
+# Algorithmic bytes of one step's forward pass and one step's adjoint (FP32
+# state): per particle and per active grid node (a node with m > 0, i.e. one
+# that some particle's stencil touches).  A fused forward launch (k_step) is
+# one step's forward pass, a fused adjoint launch (k_adj_fused: the adjoint
+# gather of step t with the adjoint scatter of step t-1) one step's adjoint.
+#   forward: read x, v, F, C (96 B) + m, vol (8 B), write x, v, F, C (96 B) per
+#            particle; per node write m, p (16 B), read m, p + write v (28 B),
+#            read v (12 B)
+#   adjoint: read layer t's x, v, F, C, m, vol (104 B) + C of layer t+1 (36 B)
+#            and the adjoint of layer t+1 (96 B), write the adjoint of layer t
+#            (96 B) per particle; per node read m, v (16 B), scatter dL/dv
+#            (12 B), read dL/dv + write dL/dp, dL/dm (28 B), read dL/dp, dL/dm
+#            (16 B)
+# The whole fwd+bwd step is 532 B per particle + 128 B per active node.
+ALGO_BYTES = {"step": (200, 56), "adj_fused": (332, 72)}
+
+# The kernels' own byte models, FP32 (4-byte elements): every word each kernel
+# reads or writes once, as built (the adjoint kernels read G2P's v' and C' of
+# step t from layer t+1 instead of recomputing them).
 #   p2g         reads x, v, F, C, m, vol, lam, mu (28 floats); writes m, p per node
 #   g2p         reads x, F (12 floats) and m, p per node; writes x, v, F, C (24 floats)
 #   adj_scatter reads x, F (12), the adjoint carry (24) and C of layer t+1 (9); writes dL/dv per node
@@ -62,18 +78,24 @@ BYTES = {
 }
 
 
-def profiled_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary of the default workload (None if absent)."""
-    path = os.path.join(ROOT, "profiles", "kernels_1M_128cubed_hbm_traffic.md")
+def profiled(key: str, kernel: str, name: str = "kernels_1M_128cubed_hbm_traffic.md"):
+    """A per-kernel figure of the default workload from a committed rocprofv3
+    summary under profiles/ (a line `key kernel value`), or None."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
             f = line.split()
-            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
-                return int(f[2])
+            if len(f) == 3 and f[0] == key and f[1] == kernel:
+                return float(f[2])
     except OSError:
         pass
     return None
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the FETCH_SIZE/WRITE_SIZE runs."""
+    v = profiled("traffic_bytes_per_launch", kernel)
+    return None if v is None else int(v)
 
 
 def log(*a):
@@ -81,11 +103,32 @@ def log(*a):
 
 
 def cpu_threads() -> int:
+    """Threads for the CPU baseline: OMP_NUM_THREADS when the host sets it
+    (the share of the host's cores this process is given), else every core
+    this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
     try:
-        n = len(os.sched_getaffinity(0))
+        return max(1, len(os.sched_getaffinity(0)))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        return max(1, os.cpu_count() or 1)
+
+
+def host_cpu() -> dict:
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = None
+    return {"model": model, "logical_cpus": os.cpu_count(), "cpus_available": aff}
 
 
 @contextlib.contextmanager
@@ -104,7 +147,7 @@ def stdout_to_stderr():
         os.close(saved)
 
 
-def cpu_baseline(sc, seed_gx, steps: int = 2):
+def cpu_baseline(sc, seed_gx, steps: int = 5):
     """The reference's own OpenMP C++ path (compiled from its sources into
     oracle/_ref) on a bounded sample of the same workload: all particles,
     `steps` forward steps and `steps` adjoint steps, its own physics
@@ -126,7 +169,7 @@ def cpu_baseline(sc, seed_gx, steps: int = 2):
                 "sample": f"{sc.n} particles, 128^3 grid, {steps} forward + {steps} adjoint steps of the "
                           f"reference's ForwardTimeStep/Back_Timestep (its physics: cubic B-spline, "
                           f"fixed-corotated), {threads} OpenMP threads",
-                "fwd_s": fwd, "bwd_s": bwd}
+                "fwd_s": fwd, "bwd_s": bwd, "host": host_cpu()}
     n = min(sc.n, 65536)
     sub = {k: v[:n] for k, v in s.items()}
     orc = Oracle("f32")
@@ -136,8 +179,9 @@ def cpu_baseline(sc, seed_gx, steps: int = 2):
     gT = dict(x=seed_gx[:n], v=np.zeros((n, 3)), F=np.zeros((n, 3, 3)), C=np.zeros((n, 3, 3)))
     orc.rollout_grad(prm, layers, gT)
     dt = time.perf_counter() - t0
-    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} particles, {steps} fwd+bwd steps of the oracle (plain physics)"}
+    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} particles, {steps} fwd+bwd steps of the oracle (plain physics, one thread)",
+            "host": host_cpu()}
 
 
 def stream_copy_GBps(mib: int = 1024, reps: int = 5) -> float:
@@ -311,6 +355,64 @@ def full_scale_episode(T: int = 500, K: int = 25, controls: int = 20) -> dict:
     return out
 
 
+def fp32_errors(o32, g32, o64, g64, n) -> dict:
+    """Max and RMS of |x32 - x64| (the box is the unit cube) and |F32 - F64|_F
+    at the last layer; relative L2 error of each layer-0 gradient."""
+    dx = np.linalg.norm(o32["x"] - o64["x"], axis=1)
+    dF = np.linalg.norm((o32["F"] - o64["F"]).reshape(n, 9), axis=1)
+    return {"x_max": float(dx.max()), "x_rms": float(np.sqrt((dx ** 2).mean())),
+            "F_max": float(dF.max()), "F_rms": float(np.sqrt((dF ** 2).mean())),
+            **{f"grad_{k}_rel_l2": float(np.linalg.norm(g32[k] - g64[k]) / max(np.linalg.norm(g64[k]), 1e-300))
+               for k in g32}}
+
+
+def implosion_workload(T: int, lds_bytes: int, block: int) -> dict:
+    """The 1M-particle sphere with a radial implosion (22/s) on top of its
+    sine field, which compresses the body to det F = 0.30 before it rebounds
+    (the sine field alone keeps det F above 0.97).  The FP32 path (FP64
+    det/log J/F^-T) with the whole tape and checkpointed (K = 25) against a
+    pure FP64 run with the whole tape; and, to measure how strongly the
+    dynamics amplify small differences, a second FP64 run whose initial
+    positions are perturbed by a relative 1e-7 (about one FP32 ulp)."""
+    from dmpm.scenes import config3_implosion
+    rng = np.random.default_rng(7)
+    sc = config3_implosion()
+    gx = 1e-3 * rng.normal(size=(sc.n, 3))
+    gF = 1e-4 * rng.normal(size=(sc.n, 3, 3))
+
+    def run(prec, K, scene, det_every=0):
+        s = episode_sim(scene, prec, K, T, gx, gF, lds_bytes=lds_bytes if prec == "f32" else 0,
+                        block=block if prec == "f32" else 0)
+        warm_up(s, 1, T)
+        t0 = time.perf_counter()
+        run_episodes(s, 1)
+        s.synchronize()
+        w = time.perf_counter() - t0
+        o = {k: s.get(T, k, dtype=np.float64) for k in ("x", "F")}
+        g = {k: s.get_grad(0, k, dtype=np.float64) for k in ("x", "v", "F", "C")}
+        jmin = [float(np.linalg.det(s.get(t, "F", dtype=np.float64)).min()) for t in range(0, T + 1, det_every)] \
+            if det_every else None
+        s.close()
+        return w, o, g, jmin
+
+    w64, o64, g64, jmin = run("f64", 1, sc, det_every=10)
+    scp = config3_implosion()
+    scp.x = scp.x * (1 + 1e-7 * np.random.default_rng(9).normal(size=scp.x.shape))
+    _, op, gp, _ = run("f64", 1, scp)
+    w1, o1, g1, _ = run("f32", 1, sc)
+    w25, o25, g25, _ = run("f32", 25, sc)
+    out = {"scene": "sphere r=0.3, 1,048,576 particles, 128^3, sine field (0.5) + radial implosion 22/s",
+           "detF_min_over_episode": min(jmin), "detF_min_every_10_layers": jmin,
+           "fp64_ms": 1e3 * w64, "fp32_ms": 1e3 * w1, "fp32_checkpointed_ms": 1e3 * w25,
+           "errors_vs_fp64": fp32_errors(o1, g1, o64, g64, sc.n),
+           "checkpointed_errors_vs_fp64": fp32_errors(o25, g25, o64, g64, sc.n),
+           "fp64_x0_perturbed_1e-7_vs_fp64": fp32_errors(op, gp, o64, g64, sc.n)}
+    log(f"[bench] implosion: det F min {min(jmin):.3f}; FP64 {1e3 * w64:.1f} ms, FP32 {1e3 * w1:.1f} ms, "
+        f"FP32 K=25 {1e3 * w25:.1f} ms; FP32 errors {out['errors_vs_fp64']}; FP64 x0*(1+1e-7) "
+        f"{out['fp64_x0_perturbed_1e-7_vs_fp64']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,6 +431,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="use the default LDS budget and block size")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-instance batched workload")
     ap.add_argument("--no-full-scale", action="store_true", help="skip the 4M-particle, 256^3, 500-step episode")
+    ap.add_argument("--no-implosion", action="store_true",
+                    help="skip the strongly compressed companion scene (FP32 vs FP64 at det F ~ 0.3)")
     ap.add_argument("--no-sort", action="store_true",
                     help="keep the upload order (no dmpm_sort_particles after the first warm-up episode)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -410,20 +514,35 @@ def main():
             continue
         avg = ms / cnt
         entry = {"total_ms": ms, "launches": cnt, "avg_us": 1e3 * avg}
-        if name in BYTES:
-            bp, bn = BYTES[name]
+        if name in ALGO_BYTES:
+            bp, bn = ALGO_BYTES[name]
             b = bp * sc.n + bn * gact
             entry["bytes_per_launch"] = b
             entry["GBps"] = b / (avg * 1e-3) / 1e9
+        if name in BYTES:
+            bp, bn = BYTES[name]
+            b = bp * sc.n + bn * gact
+            entry["kernel_model_bytes_per_launch"] = b
+            entry["kernel_model_GBps"] = b / (avg * 1e-3) / 1e9
         kernels[name] = entry
-    dom = max((k for k in kernels if k in BYTES), key=lambda k: kernels[k]["total_ms"])
+    dom = max((k for k in kernels if k in ALGO_BYTES), key=lambda k: kernels[k]["total_ms"])
     d = kernels[dom]
     default_workload = (args.particles, args.cells, T) == (1048576, 128, 300)
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    valu = None
+    if default_workload:
+        mix = "kernels_1M_128cubed_instruction_mix.md"
+        vi = profiled("valu_insts_per_launch", "k_" + dom, mix)
+        vb = profiled("valu_busy_frac", "k_" + dom, mix)
+        if vi is not None:
+            valu = {"insts_per_launch": vi, "busy_frac": vb, "source": "profiles/" + mix}
+    roofline = {"bound": "hbm", "kernel": "k_" + dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": d["GBps"] / HBM_PEAK_GBS,
                 "traffic": profiled_traffic("k_" + dom) if default_workload else None,
-                "bytes_per_launch": d["bytes_per_launch"], "avg_launch_us": d["avg_us"],
-                "achievable_copy_GBps": copy_bw}
+                "bytes_per_launch": d["bytes_per_launch"],
+                "bytes_model": f"{ALGO_BYTES[dom][0]} B per particle + {ALGO_BYTES[dom][1]} B per active node "
+                               f"({sc.n} particles, {gact} active nodes)",
+                "kernel_model_bytes_per_launch": d.get("kernel_model_bytes_per_launch"),
+                "avg_launch_us": d["avg_us"], "achievable_copy_GBps": copy_bw, "valu": valu}
 
     # whole-step algorithmic roofline: 532 B per particle-step + 128 B per active node
     step_bytes = 532 * sc.n + 128 * gact
@@ -442,9 +561,12 @@ def main():
         simc.synchronize()
         wc = (time.perf_counter() - t0) / args.steps
         _, peak_c = simc.memory()
+        simc_out = {k: simc.get(T, k, dtype=np.float64) for k in ("x", "F")}
+        simc_out.update({"g" + k: simc.get_grad(0, k, dtype=np.float64) for k in ("x", "v", "F", "C")})
         simc.close()
         extra["checkpointed"] = {"checkpoint_interval": 25, "value": sc.n * T / wc, "ms_per_step": 1e3 * wc,
                                  "peak_device_bytes": peak_c}
+        out25 = {k: simc_out[k] for k in simc_out}
         log(f"[bench] f32 K=25: {1e3 * wc:.1f} ms/episode, {sc.n * T / wc:.4g} particle-steps/s, "
             f"peak {peak_c / 1e9:.2f} GB")
     if not args.no_fp64:
@@ -457,19 +579,20 @@ def main():
         out64 = {k: sim64.get(T, k, dtype=np.float64) for k in ("x", "F")}
         g64 = {k: sim64.get_grad(0, k, dtype=np.float64) for k in ("x", "v", "F", "C")}
         sim64.close()
-        dx = np.linalg.norm(out32["x"] - out64["x"], axis=1)
-        dF = np.linalg.norm((out32["F"] - out64["F"]).reshape(sc.n, 9), axis=1)
         J = np.linalg.det(out64["F"])
         extra["fp64"] = {"value": sc.n * T / w64, "ms_per_step": 1e3 * w64,
                          "speedup_fp32_over_fp64": value / (sc.n * T / w64)}
-        extra["errors_vs_fp64"] = {
-            "x_max": float(dx.max()), "x_rms": float(np.sqrt((dx ** 2).mean())),
-            "F_max": float(dF.max()), "F_rms": float(np.sqrt((dF ** 2).mean())),
-            **{f"grad_{k}_rel_l2": float(np.linalg.norm(g32[k] - g64[k]) / max(np.linalg.norm(g64[k]), 1e-300))
-               for k in g32},
-            "detF_min": float(J.min()), "detF_p01": float(np.percentile(J, 1)), "detF_p50": float(np.median(J)),
-        }
+        extra["errors_vs_fp64"] = {**fp32_errors(out32, g32, out64, g64, sc.n),
+                                   "detF_min": float(J.min()), "detF_p01": float(np.percentile(J, 1)),
+                                   "detF_p50": float(np.median(J))}
+        if "checkpointed" in extra:
+            gk = {k: out25["g" + k] for k in ("x", "v", "F", "C")}
+            extra["checkpointed"]["errors_vs_fp64"] = fp32_errors(out25, gk, out64, g64, sc.n)
+            w = extra["fp64"]["ms_per_step"] / extra["checkpointed"]["ms_per_step"]
+            extra["checkpointed"]["speedup_over_fp64"] = w
         log(f"[bench] fp64: {1e3 * w64:.1f} ms/episode; errors {extra['errors_vs_fp64']}")
+    if not args.no_implosion and default_workload:
+        extra["implosion"] = implosion_workload(T, lds_bytes, block)
 
     if not args.no_reference_physics:
         simr = episode_sim(sc, "f32", K, T, seed_gx, seed_gF,
@@ -491,10 +614,15 @@ def main():
         extra["full_scale"] = full_scale_episode()
 
     cpu = None
+    vs_cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(sc, seed_gx)
         if "reference_physics" in extra and cpu["kind"] == "reference":
-            extra["reference_physics"]["vs_cpu_reference"] = extra["reference_physics"]["value"] / cpu["value"]
+            # like for like: the GPU and the reference's CPU path on the same
+            # physics (cubic B-spline, fixed-corotated, reference adjoint)
+            vs_cpu = {"value": extra["reference_physics"]["value"] / cpu["value"],
+                      "physics": "reference (cubic B-spline, fixed-corotated) on both",
+                      "plain_physics_gpu_over_reference_cpu": value / cpu["value"]}
 
     rec = {
         "metric": METRIC,
@@ -528,7 +656,7 @@ def main():
         "step_roofline": {"bytes_per_particle_step": step_bytes / sc.n, "achieved_GBps":
                           step_bytes * value / sc.n / 1e9, "frac": step_frac},
         "cpu_baseline": cpu,
-        "vs_cpu": (value / cpu["value"]) if cpu else None,
+        "vs_cpu": vs_cpu,
         "kernels": kernels,
         **extra,
     }
