@@ -95,3 +95,53 @@ def test_mismatched_episodes_are_rejected():
         BatchedEpisode([a, c])
     for s in (a, b, c):
         s.close()
+
+
+def test_batched_instances_against_oracle_and_conservation():
+    """All 64 uneven instances of the batched workload, FP64, the full
+    100-step episode (dt = 2.5e-4, K = 10) with a control at layer 3, run as
+    one batched episode: every instance conserves its mass and linear
+    momentum (no wall contact, no external force); the 4 smallest instances'
+    final states, layer-0 gradients and control gradients match the oracle
+    run on the same inputs (1e-10)."""
+    from dmpm import BatchedEpisode, Physics, p2g_mass
+    from dmpm.scenes import config4_instances, instance_scene
+    from helpers import oracle_params
+    from oracle import Oracle
+    T, K, ctl_layer, dt = 100, 10, 3, 2.5e-4
+    ph = Physics.plain(dt=dt)
+    specs = config4_instances()
+    sims = [_make(spec, "f64", ph, T, K, ctl_layer) for spec in specs]
+    print(f"[batch_oracle] {len(sims)} instances, {sum(s.n for s in sims)} particles", flush=True)
+    b = BatchedEpisode(sims, start=0, stop=0)
+    b.launch()
+    b.synchronize()
+    b.close()
+    for spec, sim in zip(specs, sims):
+        sc = instance_scene(spec)
+        m = sc.m.astype(np.float64)
+        xT, vT = sim.get(T, "x", dtype=np.float64), sim.get(T, "v", dtype=np.float64)
+        gm = p2g_mass(sc.dims, sc.dx, sc.grid_min, xT, m, order=2, precision="f64")
+        assert abs(gm.sum() - m.sum()) <= 1e-10 * m.sum(), spec["seed"]
+        p0, pT = (m[:, None] * sc.v).sum(0), (m[:, None] * vT).sum(0)
+        assert np.abs(pT - p0).max() <= 1e-10 * (m[:, None] * np.abs(sc.v)).sum(), spec["seed"]
+    smallest = sorted(range(len(specs)), key=lambda i: specs[i]["n"])[:4]
+    orc = Oracle("f64")
+    for i in smallest:
+        spec, sim = specs[i], sims[i]
+        sc = instance_scene(spec)
+        rng = np.random.default_rng(spec["seed"])
+        ctl = 1e-3 * rng.normal(size=(sc.n, 3, 3))
+        gT = dict(x=rng.normal(size=(sc.n, 3)), v=np.zeros((sc.n, 3)), F=1e-2 * rng.normal(size=(sc.n, 3, 3)),
+                  C=np.zeros((sc.n, 3, 3)))
+        prm = oracle_params("plain", sc, dt)
+        controls = [ctl if t == ctl_layer else None for t in range(T)]
+        layers = orc.rollout(prm, dict(sc.state(), dFc=None), T, controls)
+        g0, gctl = orc.rollout_grad(prm, layers, gT)
+        print(f"[batch_oracle] instance {i}: n={sc.n} cells={spec['cells']}", flush=True)
+        for k in ("x", "v", "F", "C"):
+            assert rel_err(sim.get(T, k, dtype=np.float64), layers[-1][k]) < 1e-10, (i, k)
+            assert rel_err(sim.get_grad(0, k, dtype=np.float64), g0[k]) < 1e-10, (i, "d" + k)
+        assert rel_err(sim.get_control_grad(ctl_layer, dtype=np.float64), gctl[ctl_layer]) < 1e-10, i
+    for sim in sims:
+        sim.close()

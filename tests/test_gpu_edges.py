@@ -40,7 +40,7 @@ def _episode_matches(sc, s, name, T=3):
         e = rel_err(sim.get(T, k, dtype=np.float64), layers[T][k])
         assert e < 1e-10, f"{k}: rel err {e:.3e}"
         e = rel_err(sim.get_grad(0, k, dtype=np.float64), g0[k])
-        assert e < 1e-9, f"d{k}: rel err {e:.3e}"
+        assert e < 1e-10, f"d{k}: rel err {e:.3e}"
     sim.close()
 
 

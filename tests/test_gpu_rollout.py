@@ -42,9 +42,9 @@ def test_autograd_rollout_matches_oracle():
     assert rel_err(xT.detach().cpu().numpy(), layers[-1]["x"]) < 1e-10
     g0, gctl = orc.rollout_grad(prm, layers, {k: w[k].cpu().numpy() for k in w})
     for k in ("x", "v", "F", "C"):
-        assert rel_err(st[k].grad.cpu().numpy(), g0[k]) < 1e-9, k
+        assert rel_err(st[k].grad.cpu().numpy(), g0[k]) < 1e-10, k
     for t in ctl:
-        assert rel_err(ctl[t].grad.cpu().numpy(), gctl[t]) < 1e-9, t
+        assert rel_err(ctl[t].grad.cpu().numpy(), gctl[t]) < 1e-10, t
     sim.close()
 
 

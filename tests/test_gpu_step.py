@@ -115,9 +115,9 @@ def test_rollout_checkpointing(K, recompute):
         sim.backward(0)
         for k in ("x", "v", "F", "C"):
             e = rel_err(sim.get_grad(0, k, dtype=np.float64), g0[k])
-            assert e < 1e-9, f"{case}: d{k}: {e:.3e}"
+            assert e < 1e-10, f"{case}: d{k}: {e:.3e}"
         for t in live:
-            assert rel_err(sim.get_control_grad(t, dtype=np.float64), gctl[t]) < 1e-9, (case, t)
+            assert rel_err(sim.get_control_grad(t, dtype=np.float64), gctl[t]) < 1e-10, (case, t)
     sim.close()
 
 
@@ -280,7 +280,7 @@ def test_launch_shapes_match_oracle(name, prec, lds, block):
     else:
         assert in_lds > 0 and fell_back == 0, (in_lds, fell_back)
     dt = np.float64 if prec == "f64" else np.float32
-    tol_s, tol_g = (1e-10, 1e-9) if prec == "f64" else (1e-3, 1e-2)
+    tol_s, tol_g = (1e-10, 1e-10) if prec == "f64" else (1e-4, 1e-3)
     for k in ("x", "v", "F", "C"):
         e = rel_err(sim.get(T, k, dtype=dt), layers[T][k])
         assert e < tol_s, f"{k}: rel err {e:.3e}"
@@ -319,6 +319,6 @@ def test_unfused_kernels_match_oracle(name, K):
     for k in ("x", "v", "F", "C"):
         assert rel_err(sim.get(T, k, dtype=np.float64), layers[T][k]) < 1e-10, k
         e = rel_err(sim.get_grad(0, k, dtype=np.float64), g0[k])
-        assert e < 1e-9, f"d{k}: rel err {e:.3e}"
-    assert rel_err(sim.get_control_grad(1, dtype=np.float64), gctl[1]) < 1e-9
+        assert e < 1e-10, f"d{k}: rel err {e:.3e}"
+    assert rel_err(sim.get_control_grad(1, dtype=np.float64), gctl[1]) < 1e-10
     sim.close()
