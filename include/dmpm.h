@@ -228,11 +228,18 @@ int dmpm_run_optimization(dmpm_sim* sim, int num_steps, double dt, double drag, 
                           double gd_tol, double smoothing, int current_episodes, double* final_loss);
 
 /* --- free functions (bind.cpp:531-559) ------------------------------------- */
-/* p2g(pc, grid): node masses of a dt = 0 P2G (bind.cpp:546-549). */
+/* p2g(pc, grid) (bind.cpp:546-549): the node masses of a dt = 0 P2G of the
+ * n particles x (n,3), m (n) are ADDED to grid_m (G = nx*ny*nz values), as
+ * the reference's P2G adds onto the Grid it is given.  Computed in the
+ * precision of dtype; with on_device = 1 every array is device memory and
+ * nothing is copied to the host. */
 int dmpm_p2g_mass(const dmpm_config* cfg, int n, const void* x, const void* m, void* grid_m, int dtype,
                   int on_device);
 /* calculate_point_cloud_volumes(pc, grid) (bind.cpp:551-554,
- * ForwardSimulation.cpp:234-271): writes vol and the grid's node masses. */
+ * ForwardSimulation.cpp:234-271): the P2G of dmpm_p2g_mass adds the masses
+ * to grid_m (zeros for a fresh Grid), then vol_p = m_p dx^3 / sum_i w_ip m_i
+ * over the summed node masses (0 where that sum is not above FLT_EPSILON
+ * dx^3). */
 int dmpm_calculate_volumes(const dmpm_config* cfg, int n, const void* x, const void* m, void* vol, void* grid_m,
                            int dtype, int on_device);
 /* calculate_lame_parameters(E, nu) (bind.cpp:540-544, Elasticity.cpp:149-153). */

@@ -187,6 +187,20 @@ void reduce_partials(const double* partial, int nblocks, int stride, double* out
 
 // CalculatePointCloudVolumes particle pass: vol_p = m_p dx^3 / sum_i w m_i
 void volumes(Prec pr, const Launch& L, void* vol_out, hipStream_t s);
+}  // namespace dmpm
+
+#include "../../include/dmpm.h"
+
+namespace dmpm {
+// P2G of the masses of n particles (device arrays x (n,3) and m (n) of
+// precision pr, in the caller's order; no block table, so each thread block
+// takes 256 consecutive particles) added into the grid's m plane grid_m
+// (ForwardSimulation.cpp:234-237 adds onto the Grid it is given), then, when
+// vol is non-null, vol_p = m_p dx^3 / sum_i w_ip m_i (:238-270).  `planes`
+// is device scratch of 4 planes of ld >= n elements (x0..x2, m), fully
+// written before it is read.  Queued on st; nothing is allocated.
+void mass_volume_pass(Prec pr, const dmpm_config& cfg, int n, long long ld, const void* x, const void* m,
+                      void* planes, void* grid_m, void* vol, hipStream_t st);
 
 // Squared Frobenius norm partials of 9 planes (dL/dF): partial[b]
 void sqnorm9(Prec pr, const void* planes, int n, long long ld, double* partial, int nblocks, hipStream_t s);

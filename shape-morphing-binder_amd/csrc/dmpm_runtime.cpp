@@ -886,7 +886,89 @@ void restore_layer(dmpm_sim* s, int t, const void* buf) {
     copy_elems(s->pr, s->avmax[t], (const char*)buf + 3 * e * s->esz, e, s->stream);
 }
 
+// dmpm_p2g_mass / dmpm_calculate_volumes: the particles' masses P2G'd into
+// grid_m (added to what it holds) and, when vol is non-null, their volumes
+// from the summed node masses, computed in the precision of dtype with
+// device memory for the planes (and, for host arrays, staging) allocated
+// for the call.
+void mass_volume_call(const dmpm_config* cfg, int n, const void* x, const void* m, void* vol, void* grid_m,
+                      int dtype, int on_device) {
+    if (!cfg || !x || !m || !grid_m) throw Error(DMPM_ERR_ARG, "null argument");
+    if (n < 0) throw Error(DMPM_ERR_ARG, "n must be non-negative");
+    validate(*cfg);
+    const Prec pr = dtype == DMPM_DTYPE_F64 ? PREC_F64 : PREC_F32;
+    const size_t es = dsize(dtype);
+    const long long ld = round_up(std::max(n, 1), 64);
+    const long long G = (long long)cfg->dims[0] * cfg->dims[1] * cfg->dims[2];
+    hipStream_t st = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    std::vector<void*> bufs;
+    auto dalloc = [&](size_t bytes) {
+        void* p = nullptr;
+        const hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 16));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(DMPM_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        }
+        bufs.push_back(p);
+        return p;
+    };
+    auto release = [&] {
+        (void)hipStreamSynchronize(st);
+        for (void* p : bufs) (void)hipFree(p);
+        (void)hipStreamDestroy(st);
+    };
+    try {
+        void* planes = dalloc((size_t)4 * ld * es);
+        const void *dx = x, *dm = m;
+        void *dg = grid_m, *dv = vol;
+        if (!on_device) {
+            void* hx = dalloc((size_t)n * 3 * es);
+            void* hm = dalloc((size_t)n * es);
+            dg = dalloc((size_t)G * es);
+            if (vol) dv = dalloc((size_t)n * es);
+            HIPCHK(hipMemcpyAsync(hx, x, (size_t)n * 3 * es, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(hm, m, (size_t)n * es, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(dg, grid_m, (size_t)G * es, hipMemcpyHostToDevice, st));
+            dx = hx;
+            dm = hm;
+        }
+        mass_volume_pass(pr, *cfg, n, ld, dx, dm, planes, dg, dv, st);
+        if (!on_device) {
+            HIPCHK(hipMemcpyAsync(grid_m, dg, (size_t)G * es, hipMemcpyDeviceToHost, st));
+            if (vol) HIPCHK(hipMemcpyAsync(vol, dv, (size_t)n * es, hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+}
+
 }  // namespace
+
+// Shared with the allocation-free entry point (dmpm_noalloc.cpp).
+void dmpm::mass_volume_pass(Prec pr, const dmpm_config& cfg, int n, long long ld, const void* x, const void* m,
+                            void* planes, void* grid_m, void* vol, hipStream_t st) {
+    if (n <= 0) return;
+    const size_t es = pr == PREC_F64 ? 8 : 4;
+    char* P = (char*)planes;
+    import_planes(pr, x, pr == PREC_F64, P, nullptr, n, 3, ld, st);
+    import_planes(pr, m, pr == PREC_F64, P + 3 * ld * es, nullptr, n, 1, ld, st);
+    Launch Lc{};
+    Lc.ph = phys_of(cfg);
+    Lc.n = n;
+    Lc.ld = ld;
+    Lc.gld = round_up((long long)cfg.dims[0] * cfg.dims[1] * cfg.dims[2], 64);
+    Lc.st = P;
+    Lc.stat = P + 3 * ld * es;  // the statics' first plane is m
+    Lc.grid = grid_m;           // only the m plane is touched by a mass-only P2G
+    Lc.lds_bytes = (pr == PREC_F32 ? 30 : 48) * 1024;
+    Lc.block = 256;
+    p2g(pr, Lc, true, st);
+    if (vol) volumes(pr, Lc, vol, st);
+}
 
 // ============================================================================
 extern "C" {
@@ -1797,97 +1879,14 @@ int dmpm_run_optimization(dmpm_sim* s, int num_steps, double dt, double drag, co
 
 int dmpm_p2g_mass(const dmpm_config* cfg, int n, const void* x, const void* m, void* grid_m, int dtype,
                   int on_device) {
-    return guard([&] {
-        if (!cfg || !x || !m || !grid_m) throw Error(DMPM_ERR_ARG, "null argument");
-        double vol_unused = 0;
-        (void)vol_unused;
-        dmpm_config c = *cfg;
-        validate(c);
-        dmpm_sim* s = nullptr;
-        if (dmpm_create(&c, &s) != DMPM_OK) throw Error(DMPM_ERR_HIP, g_err);
-        std::unique_ptr<dmpm_sim> own(s);
-        // particles with only x and m
-        const size_t es = dsize(dtype);
-        std::vector<char> zeros3((size_t)n * 9 * es, 0), ones((size_t)n * es);
-        for (int i = 0; i < n; ++i) {
-            if (dtype == DMPM_DTYPE_F64) ((double*)ones.data())[i] = 1.0;
-            else ((float*)ones.data())[i] = 1.0f;
-        }
-        const void* xx = x;
-        const void* mm = m;
-        std::vector<char> xh, mh;
-        if (on_device) {
-            xh.resize((size_t)n * 3 * es);
-            mh.resize((size_t)n * es);
-            HIPCHK(hipMemcpy(xh.data(), x, xh.size(), hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(mh.data(), m, mh.size(), hipMemcpyDeviceToHost));
-            xx = xh.data();
-            mm = mh.data();
-        }
-        if (dmpm_set_particles(s, n, xx, zeros3.data(), zeros3.data(), zeros3.data(), nullptr, mm, ones.data(),
-                               ones.data(), ones.data(), dtype, 0) != DMPM_OK)
-            throw Error(DMPM_ERR_HIP, g_err);
-        Launch Lc = s->launch();
-        Lc.st = s->layer[0];
-        Lc.stat = s->stat;
-        Lc.grid = s->grid;
-        s->zero(s->grid, s->gld);
-        p2g(s->pr, Lc, true, s->stream);
-        void* dst = on_device ? grid_m : s->dalloc((size_t)s->G * es);
-        export_planes(s->pr, s->grid, dst, dtype == DMPM_DTYPE_F64, nullptr, (int)s->G, 1, s->gld, s->stream);
-        if (!on_device) {
-            HIPCHK(hipMemcpyAsync(grid_m, dst, (size_t)s->G * es, hipMemcpyDeviceToHost, s->stream));
-        }
-        HIPCHK(hipStreamSynchronize(s->stream));
-    });
+    return guard([&] { mass_volume_call(cfg, n, x, m, nullptr, grid_m, dtype, on_device); });
 }
 
 int dmpm_calculate_volumes(const dmpm_config* cfg, int n, const void* x, const void* m, void* vol, void* grid_m,
                            int dtype, int on_device) {
     return guard([&] {
-        if (!cfg || !x || !m || !vol) throw Error(DMPM_ERR_ARG, "null argument");
-        dmpm_config c = *cfg;
-        validate(c);
-        dmpm_sim* s = nullptr;
-        if (dmpm_create(&c, &s) != DMPM_OK) throw Error(DMPM_ERR_HIP, g_err);
-        std::unique_ptr<dmpm_sim> own(s);
-        const size_t es = dsize(dtype);
-        std::vector<char> zeros((size_t)n * 9 * es, 0), ones((size_t)n * es);
-        for (int i = 0; i < n; ++i) {
-            if (dtype == DMPM_DTYPE_F64) ((double*)ones.data())[i] = 1.0;
-            else ((float*)ones.data())[i] = 1.0f;
-        }
-        const void* xx = x;
-        const void* mm = m;
-        std::vector<char> xh, mh;
-        if (on_device) {
-            xh.resize((size_t)n * 3 * es);
-            mh.resize((size_t)n * es);
-            HIPCHK(hipMemcpy(xh.data(), x, xh.size(), hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(mh.data(), m, mh.size(), hipMemcpyDeviceToHost));
-            xx = xh.data();
-            mm = mh.data();
-        }
-        if (dmpm_set_particles(s, n, xx, zeros.data(), zeros.data(), zeros.data(), nullptr, mm, ones.data(),
-                               ones.data(), ones.data(), dtype, 0) != DMPM_OK)
-            throw Error(DMPM_ERR_HIP, g_err);
-        Launch Lc = s->launch();
-        Lc.st = s->layer[0];
-        Lc.stat = s->stat;
-        Lc.grid = s->grid;
-        s->zero(s->grid, s->gld);
-        p2g(s->pr, Lc, true, s->stream);
-        void* vint = s->dalloc((size_t)s->ld * s->esz);
-        volumes(s->pr, Lc, vint, s->stream);
-        void* dv = on_device ? vol : s->dalloc((size_t)n * es);
-        export_planes(s->pr, vint, dv, dtype == DMPM_DTYPE_F64, s->perm, n, 1, s->ld, s->stream);
-        if (!on_device) HIPCHK(hipMemcpyAsync(vol, dv, (size_t)n * es, hipMemcpyDeviceToHost, s->stream));
-        if (grid_m) {
-            void* dg = on_device ? grid_m : s->dalloc((size_t)s->G * es);
-            export_planes(s->pr, s->grid, dg, dtype == DMPM_DTYPE_F64, nullptr, (int)s->G, 1, s->gld, s->stream);
-            if (!on_device) HIPCHK(hipMemcpyAsync(grid_m, dg, (size_t)s->G * es, hipMemcpyDeviceToHost, s->stream));
-        }
-        HIPCHK(hipStreamSynchronize(s->stream));
+        if (!vol) throw Error(DMPM_ERR_ARG, "null argument");
+        mass_volume_call(cfg, n, x, m, vol, grid_m, dtype, on_device);
     });
 }
 

@@ -370,11 +370,9 @@ def calculate_point_cloud_volumes(pc: PointCloud, grid: Grid):
         raise RuntimeError("PointCloud or Grid is null.")
     if pc._arr is None:
         raise RuntimeError("calculate_point_cloud_volumes needs a host PointCloud")
-    if np.any(grid.m != 0):
-        # the reference would P2G on top of the masses already there; only a
-        # fresh Grid (the way run.py calls it, run.py:1278-1279) is supported
-        raise RuntimeError("calculate_point_cloud_volumes expects a fresh Grid (no prior P2G)")
-    vol, gm = _volumes(grid.dims, grid.dx, tuple(float(a) for a in grid.min_point), pc._get("x"), pc._get("m"))
+    # the P2G adds onto the masses the grid already holds, as the reference's
+    vol, gm = _volumes(grid.dims, grid.dx, tuple(float(a) for a in grid.min_point), pc._get("x"), pc._get("m"),
+                       grid_m=grid.m)
     pc._arr["vol"] = vol.astype(np.float32)
     grid.m = gm.astype(np.float32)
 

@@ -392,16 +392,17 @@ def p2g_mass(dims, dx, grid_min, x, m, order=L.ORDER_CUBIC, precision="f32"):
     return out
 
 
-def calculate_volumes(dims, dx, grid_min, x, m, order=L.ORDER_CUBIC, precision="f32"):
+def calculate_volumes(dims, dx, grid_min, x, m, order=L.ORDER_CUBIC, precision="f32", grid_m=None):
     """CalculatePointCloudVolumes (ForwardSimulation.cpp:234-271): returns
-    (vol per particle, node masses)."""
+    (vol per particle, node masses).  grid_m holds the node masses the grid
+    had before (the P2G adds to them; zeros when None)."""
     cfg = make_config(dims, dx, grid_min, Physics(order=order), precision)
     code = L.DTYPE_F64 if precision == "f64" else L.DTYPE_F32
     ax, am = _Arr(x, code), _Arr(m, code)
     n = int(np.shape(x)[0])
     dt = np.float64 if code == L.DTYPE_F64 else np.float32
     vol = np.zeros(n, dt)
-    gm = np.zeros(int(np.prod(dims)), dt)
+    gm = np.zeros(int(np.prod(dims)), dt) if grid_m is None else np.array(grid_m, dtype=dt).reshape(-1)
     L.check(L.lib().dmpm_calculate_volumes(C.byref(cfg), n, ax.ptr, am.ptr, vol.ctypes.data_as(C.c_void_p),
                                            gm.ctypes.data_as(C.c_void_p), code, ax.dev))
     return vol, gm

@@ -190,3 +190,37 @@ def test_optimize_single_timestep_decreases_loss():
     l1 = cg.end_layer_mass_loss()
     assert l1 <= l0
     assert np.abs(cg.get_point_cloud(0).get_def_grads_morph()).max() > 0
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_volumes_accumulate_onto_the_grid(prec):
+    """calculate_point_cloud_volumes P2Gs onto the masses the Grid already
+    holds (ForwardSimulation.cpp:234-237 calls P2G on the Grid it is given):
+    with the cloud's own masses already there, the node masses double and
+    every volume halves.  The device path (CUDA tensors, on_device = 1)
+    gives the host path's results."""
+    import ctypes as C
+    import torch
+    from dmpm import _lib as L
+    from dmpm.sim import calculate_volumes, make_config, Physics
+    d = load("ref_volumes.npz")
+    dims = tuple(int(a) for a in d["dims"])
+    dt = np.float64 if prec == "f64" else np.float32
+    x, m = d["in_x"].astype(dt), d["in_m"].astype(dt)
+    vol1, gm1 = calculate_volumes(dims, float(d["dx"]), (0.0, 0.0, 0.0), x, m, precision=prec)
+    vol2, gm2 = calculate_volumes(dims, float(d["dx"]), (0.0, 0.0, 0.0), x, m, precision=prec, grid_m=gm1)
+    tol = 1e-12 if prec == "f64" else 1e-6
+    assert rel_err(gm2, 2 * gm1) < tol
+    assert rel_err(vol2, 0.5 * vol1) < tol
+    dev = torch.device("cuda:0")
+    tx, tm = torch.tensor(x, device=dev), torch.tensor(m, device=dev)
+    tv = torch.zeros(x.shape[0], dtype=tx.dtype, device=dev)
+    tg = torch.tensor(gm1, device=dev)
+    cfg = make_config(dims, float(d["dx"]), (0.0, 0.0, 0.0), Physics(order=3), prec)
+    code = L.DTYPE_F64 if prec == "f64" else L.DTYPE_F32
+    torch.cuda.synchronize()
+    L.check(L.lib().dmpm_calculate_volumes(C.byref(cfg), x.shape[0], C.c_void_p(tx.data_ptr()),
+                                           C.c_void_p(tm.data_ptr()), C.c_void_p(tv.data_ptr()),
+                                           C.c_void_p(tg.data_ptr()), code, 1))
+    assert rel_err(tg.cpu().numpy(), gm2) < tol
+    assert rel_err(tv.cpu().numpy(), vol2) < tol
