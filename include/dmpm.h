@@ -242,6 +242,20 @@ int dmpm_p2g_mass(const dmpm_config* cfg, int n, const void* x, const void* m, v
  * dx^3). */
 int dmpm_calculate_volumes(const dmpm_config* cfg, int n, const void* x, const void* m, void* vol, void* grid_m,
                            int dtype, int on_device);
+/* dmpm_calculate_volumes for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); no counterpart in the reference.  Same results as
+ * dmpm_calculate_volumes with on_device = 1: x (n,3), m (n), vol (n) and
+ * grid_m (nx*ny*nz, the P2G adds to it) are device arrays of dtype; the work
+ * is queued on `stream` (a hipStream_t, 0 = the null stream) and the call
+ * returns without waiting.  The particle planes the kernels use live in one
+ * device array per GPU of DMPM_NOALLOC_SCRATCH_BYTES (4 planes: at most
+ * 1,048,576 particles in FP32, 524,288 in FP64; more is refused with
+ * DMPM_ERR_ARG); nothing is allocated.  Calls that overlap on one GPU share
+ * that array and race: serialise them (one stream, or events). */
+#define DMPM_NOALLOC_SCRATCH_BYTES (16u << 20)
+int dmpm_calculate_volumes_noalloc(const dmpm_config* cfg, int n, const void* x, const void* m, void* vol,
+                                   void* grid_m, int dtype, void* stream);
 /* calculate_lame_parameters(E, nu) (bind.cpp:540-544, Elasticity.cpp:149-153). */
 int dmpm_lame_parameters(double E, double nu, double* lam, double* mu);
 

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "libdmpm.so")
 
 DMPM_OK = 0
 DTYPE_F32, DTYPE_F64 = 0, 1
+# bytes of the device array behind dmpm_calculate_volumes_noalloc (include/dmpm.h)
+DMPM_NOALLOC_SCRATCH_BYTES = 16 << 20
 PREC_F32, PREC_F64 = 0, 1
 ORDER_QUADRATIC, ORDER_CUBIC = 2, 3
 MODEL_FCR, MODEL_NEOHOOKEAN = 0, 1
@@ -136,6 +138,7 @@ def lib() -> C.CDLL:
         "dmpm_single_timestep_adam_count": (i, [ip, i]),
         "dmpm_p2g_mass": (i, [cfgp, i, vp, vp, vp, i, i]),
         "dmpm_calculate_volumes": (i, [cfgp, i, vp, vp, vp, vp, i, i]),
+        "dmpm_calculate_volumes_noalloc": (i, [cfgp, i, vp, vp, vp, vp, i, vp]),
         "dmpm_lame_parameters": (i, [d, d, dp, dp]),
         "dmpm_last_timing": (i, [vp, dp, dp]),
         "dmpm_memory": (i, [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
@@ -180,7 +183,7 @@ EXPORTED = [
     "dmpm_set_control", "dmpm_request_control_grad", "dmpm_get_control_grad", "dmpm_control_grad_norm",
     "dmpm_descend_adam", "dmpm_optimize_single_timestep", "dmpm_run_optimization",
     "dmpm_single_timestep_adam_count", "dmpm_p2g_mass",
-    "dmpm_calculate_volumes", "dmpm_lame_parameters", "dmpm_last_timing", "dmpm_memory", "dmpm_kernel_timer",
+    "dmpm_calculate_volumes", "dmpm_calculate_volumes_noalloc", "dmpm_lame_parameters", "dmpm_last_timing", "dmpm_memory", "dmpm_kernel_timer",
     "dmpm_kernel_stats", "dmpm_window_stats", "dmpm_window_counters", "dmpm_graph_capture", "dmpm_graph_launch", "dmpm_graph_synchronize", "dmpm_graph_destroy",
     "dmpm_get_config", "dmpm_set_adam_state", "dmpm_file_info", "dmpm_write_file", "dmpm_read_file",
     "dmpm_write_layer", "dmpm_read_particles", "dmpm_winding_numbers",

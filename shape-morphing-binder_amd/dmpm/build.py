@@ -20,7 +20,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libdmpm.so")
 
-SOURCES = ["dmpm_kernels.hip", "dmpm_runtime.cpp", "dmpm_io.cpp", "dmpm_mesh.cpp"]
+SOURCES = ["dmpm_kernels.hip", "dmpm_runtime.cpp", "dmpm_io.cpp", "dmpm_mesh.cpp", "dmpm_noalloc.cpp"]
 # dmpm_kernels.hip is compiled once per part (DMPM_PART = 0..6, see the file's
 # header), in parallel: each part instantiates one group of kernels.
 KERNEL_PARTS = 7
