@@ -119,6 +119,62 @@ struct Seg {
 };
 void launch_segs(Prec pr, int kind, const Seg* segs, int count, long long max_n, hipStream_t s);
 
+// Position of a cell along a 3-D Hilbert curve over a 1024^3 lattice (10
+// bits per axis), after J. Skilling, "Programming the Hilbert curve" (AIP
+// Conf. Proc. 707, 2004): the coordinates are transformed in place into the
+// curve's "transposed" index, whose bits are then interleaved.  Unlike a
+// Morton (Z-order) curve the Hilbert curve has no jumps -- consecutive cells
+// are face neighbours -- so any run of ~60 cells lies inside two adjacent
+// 4x4x4 sub-cubes, which bounds the block windows of the particle kernels.
+__host__ __device__ inline uint32_t hilbert3(uint32_t x0, uint32_t x1, uint32_t x2) {
+    const int bits = 10;
+    uint32_t X[3] = {x0 & 1023u, x1 & 1023u, x2 & 1023u};
+    const uint32_t M = 1u << (bits - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    for (int i = 0; i < 3; ++i) X[i] ^= t;
+    uint32_t h = 0;
+    for (int b = bits - 1; b >= 0; --b)
+        for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
+    return h;
+}
+
+// Storage order and thread blocks of n particles, computed on the device
+// from positions that are either the caller's (n,3) array (soa = 0) or three
+// planes of stride ld (soa = 1), float or double (x_f64):
+//  * the particles sorted (stable radix sort) by the Hilbert index of their
+//    stencil base cell, floor(x/dx - 1/2) for the quadratic B-spline and
+//    floor(x/dx) - 1 for the cubic one (clamped to the 1024^3 lattice), so
+//    that particles sharing a stencil are neighbours and the scatters can
+//    merge their sums across lanes;
+//  * thread blocks of up to `block` consecutive particles, each ended
+//    before the particle that would grow the box of its stencils beyond
+//    split_nodes grid nodes (0: 512; negative: no cuts, consecutive blocks);
+//  * inside each block, its runs of equal base cells re-ordered so that few
+//    cross a 16-lane row (best fit: each row takes the longest remaining run
+//    that fits its free lanes, else the longest remaining run).
+// perm (n ints) receives the source index of each stored particle and
+// starts (n + 1 ints of room) the blocks' first particles and n; returns the
+// block count (0 without cuts).  Synchronises the stream (the count).
+size_t order_scratch_bytes(int n);
+int order_particles_device(int order, int block, int split_nodes, const double grid_min[3], double dx,
+                           const void* x, int x_f64, int soa, long long ld, int n, int* perm, int* starts,
+                           void* scratch, size_t scratch_bytes, hipStream_t s);
+
 // Record the message dmpm_last_error() returns (for library parts outside
 // the runtime's own error guard).
 void set_last_error(const std::string& msg);

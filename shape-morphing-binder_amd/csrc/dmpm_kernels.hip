@@ -2561,6 +2561,214 @@ void adam(Prec pr, const void* dLdF, void* dFc, void* mom, void* vec, void* vmax
         hipLaunchKernelGGL(k_adam<double>, dim3(nb), dim3(256), 0, s, (const double*)dLdF, (double*)dFc, (double*)mom, (double*)vec, (double*)vmax, n, ld, sqn, alpha, beta1, beta2, eps, timestep);
 }
 
+// ---- particle ordering on the device ----------------------------------------
+template <typename S, bool SOA>
+DMPM_DI double order_pos(const S* x, long long ld, int i, int a) {
+    return SOA ? (double)x[(long long)a * ld + i] : (double)x[3 * (long long)i + a];
+}
+
+template <typename S, bool SOA>
+__global__ void k_order_keys(const S* x, long long ld, int n, double mn0, double mn1, double mn2, double dx,
+                             double shift, unsigned* keys, int* vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double mn[3] = {mn0, mn1, mn2};
+    uint32_t c[3];
+    for (int a = 0; a < 3; ++a) {
+        const double f = (order_pos<S, SOA>(x, ld, i, a) - mn[a]) / dx;
+        long long ci = (long long)floor(f - shift) + 2;
+        ci = ci < 0 ? 0 : (ci > 1023 ? 1023 : ci);
+        c[a] = (uint32_t)ci;
+    }
+    keys[i] = hilbert3(c[0], c[1], c[2]);
+    vals[i] = i;
+}
+
+// Base cell of the j-th particle in sorted order.
+template <typename S, bool SOA>
+__global__ void k_order_cells(const S* x, long long ld, int n, const int* perm, double mn0, double mn1, double mn2,
+                              double dx, double shift, int* cell) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double mn[3] = {mn0, mn1, mn2};
+    const int p = perm[j];
+    for (int a = 0; a < 3; ++a) {
+        const double f = (order_pos<S, SOA>(x, ld, p, a) - mn[a]) / dx;
+        cell[3 * (long long)j + a] = (int)fmax(-1e9, fmin(1e9, floor(f - shift)));
+    }
+}
+
+// End of the block that would start at particle i: it takes particles while
+// it holds fewer than B and the box of their stencils stays within split nodes.
+__global__ void k_block_next(const int* cell, int n, int B, int KS, long long split, int* next) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) lo[a] = hi[a] = cell[3 * (long long)i + a];
+    int j = i + 1;
+    for (; j < n && j - i < B; ++j) {
+        int l2[3], h2[3];
+        long long vol = 1;
+        for (int a = 0; a < 3; ++a) {
+            const int c = cell[3 * (long long)j + a];
+            l2[a] = min(lo[a], c);
+            h2[a] = max(hi[a], c);
+            vol *= (long long)h2[a] - l2[a] + KS;
+        }
+        if (vol > split) break;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = l2[a];
+            hi[a] = h2[a];
+        }
+    }
+    next[i] = j;
+}
+
+// The blocks actually used: 0, next(0), next(next(0)), ... (one thread).
+__global__ void k_block_chain(const int* next, int n, int* starts, int* nblk) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int k = 0, i = 0;
+    starts[0] = 0;
+    while (i < n) {
+        i = next[i];
+        starts[++k] = i;
+    }
+    *nblk = k;
+}
+
+// One thread per block: its runs of equal base cells placed row by row.
+__global__ void k_pack_rows(const int* cell, const int* starts, int nblk, const int* perm_in, int* perm_out,
+                            int* rlen, int* rstart, unsigned char* used) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblk) return;
+    const int s0 = starts[b], s1 = starts[b + 1];
+    int R = 0;
+    for (int r = s0; r < s1;) {
+        int e = r + 1;
+        while (e < s1 && cell[3 * (long long)e] == cell[3 * (long long)r] &&
+               cell[3 * (long long)e + 1] == cell[3 * (long long)r + 1] &&
+               cell[3 * (long long)e + 2] == cell[3 * (long long)r + 2])
+            ++e;
+        rlen[s0 + R] = e - r;
+        rstart[s0 + R] = r;
+        ++R;
+        r = e;
+    }
+    // stable insertion sort, longest runs first
+    for (int q = 1; q < R; ++q) {
+        const int L = rlen[s0 + q], S = rstart[s0 + q];
+        int p = q;
+        for (; p > 0 && rlen[s0 + p - 1] < L; --p) {
+            rlen[s0 + p] = rlen[s0 + p - 1];
+            rstart[s0 + p] = rstart[s0 + p - 1];
+        }
+        rlen[s0 + p] = L;
+        rstart[s0 + p] = S;
+    }
+    for (int q = 0; q < R; ++q) used[s0 + q] = 0;
+    int out = s0;
+    for (int placed = 0; placed < R; ++placed) {
+        const int room = 16 - ((out - s0) % 16);
+        int pick = R;
+        for (int q = 0; q < R; ++q)
+            if (!used[s0 + q] && rlen[s0 + q] <= room) {
+                pick = q;
+                break;
+            }
+        if (pick == R)
+            for (int q = 0; q < R; ++q)
+                if (!used[s0 + q]) {
+                    pick = q;
+                    break;
+                }
+        used[s0 + pick] = 1;
+        for (int k = 0; k < rlen[s0 + pick]; ++k) perm_out[out++] = perm_in[rstart[s0 + pick] + k];
+    }
+}
+
+namespace {
+struct OrderLayout {
+    unsigned *kin, *kout;
+    int *vin, *vout, *cell, *next, *rlen, *rstart, *nblk;
+    unsigned char* used;
+    void* temp;
+    size_t temp_bytes;
+};
+size_t order_cub_bytes(int n) {
+    size_t temp = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                             (const int*)nullptr, (int*)nullptr, n, 0, 30, (hipStream_t)0);
+    return temp;
+}
+OrderLayout order_layout(void* base, int n) {
+    OrderLayout L{};
+    char* p = (char*)base;
+    auto take = [&](size_t bytes) {
+        char* q = p;
+        p += (bytes + 255) & ~size_t(255);
+        return q;
+    };
+    const size_t nn = (size_t)std::max(n, 1);
+    L.kin = (unsigned*)take(4 * nn);
+    L.kout = (unsigned*)take(4 * nn);
+    L.vin = (int*)take(4 * nn);
+    L.vout = (int*)take(4 * nn);
+    L.cell = (int*)take(12 * nn);
+    L.next = (int*)take(4 * nn);
+    L.rlen = (int*)take(4 * nn);
+    L.rstart = (int*)take(4 * nn);
+    L.used = (unsigned char*)take(nn);
+    L.nblk = (int*)take(4);
+    L.temp_bytes = order_cub_bytes(n);
+    L.temp = take(L.temp_bytes);
+    return L;
+}
+}  // namespace
+
+size_t order_scratch_bytes(int n) { return 45 * (size_t)std::max(n, 1) + order_cub_bytes(n) + 12 * 256; }
+
+int order_particles_device(int order, int block, int split_nodes, const double grid_min[3], double dx,
+                           const void* x, int x_f64, int soa, long long ld, int n, int* perm, int* starts,
+                           void* scratch, size_t scratch_bytes, hipStream_t s) {
+    if (n <= 0) return 0;
+    if (scratch_bytes < order_scratch_bytes(n)) throw std::runtime_error("particle ordering: scratch too small");
+    OrderLayout L = order_layout(scratch, n);
+    const double shift = order == 2 ? 0.5 : 0.0;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+#define ORDER_LAUNCH(KERNEL, ...)                                                                           \
+    do {                                                                                                    \
+        if (x_f64) {                                                                                        \
+            if (soa) hipLaunchKernelGGL((KERNEL<double, true>), dim3(nb), dim3(256), 0, s, (const double*)x, __VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<double, false>), dim3(nb), dim3(256), 0, s, (const double*)x, __VA_ARGS__);   \
+        } else {                                                                                            \
+            if (soa) hipLaunchKernelGGL((KERNEL<float, true>), dim3(nb), dim3(256), 0, s, (const float*)x, __VA_ARGS__);   \
+            else hipLaunchKernelGGL((KERNEL<float, false>), dim3(nb), dim3(256), 0, s, (const float*)x, __VA_ARGS__);     \
+        }                                                                                                   \
+    } while (0)
+    ORDER_LAUNCH(k_order_keys, ld, n, grid_min[0], grid_min[1], grid_min[2], dx, shift, L.kin, L.vin);
+    size_t tb = L.temp_bytes;
+    if (hipcub::DeviceRadixSort::SortPairs(L.temp, tb, L.kin, L.kout, L.vin, L.vout, n, 0, 30, s) != hipSuccess)
+        throw std::runtime_error("particle ordering: radix sort failed");
+    const long long split = split_nodes == 0 ? 512 : split_nodes;
+    if (split <= 0) {
+        if (hipMemcpyAsync(perm, L.vout, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, s) != hipSuccess)
+            throw std::runtime_error("particle ordering: copy failed");
+        return 0;
+    }
+    ORDER_LAUNCH(k_order_cells, ld, n, L.vout, grid_min[0], grid_min[1], grid_min[2], dx, shift, L.cell);
+#undef ORDER_LAUNCH
+    const int B = block > 0 ? block : 256;
+    hipLaunchKernelGGL(k_block_next, dim3(nb), dim3(256), 0, s, L.cell, n, B, order == 2 ? 3 : 4, split, L.next);
+    hipLaunchKernelGGL(k_block_chain, dim3(1), dim3(64), 0, s, L.next, n, starts, L.nblk);
+    int nblk = 0;
+    if (hipMemcpyAsync(&nblk, L.nblk, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        throw std::runtime_error("particle ordering: block count read failed");
+    hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)((nblk + 63) / 64)), dim3(64), 0, s, L.cell, starts, nblk, L.vout,
+                       perm, L.rlen, L.rstart, L.used);
+    return nblk;
+}
+
 // ---- deterministic mode ----------------------------------------------------
 size_t det_scratch_bytes(int n, const int* dims, int esize) {
     const long long ncells = (long long)(dims[0] + 8) * (dims[1] + 8) * (dims[2] + 8);

@@ -756,40 +756,6 @@ void validate(const dmpm_config& c) {
     if (c.lds_bytes < 0 || c.lds_bytes > 64 * 1024) throw Error(DMPM_ERR_ARG, "lds_bytes must be in [0, 65536]");
 }
 
-// Position of a cell along a 3-D Hilbert curve over a 1024^3 lattice (10
-// bits per axis), after J. Skilling, "Programming the Hilbert curve" (AIP
-// Conf. Proc. 707, 2004): the coordinates are transformed in place into the
-// curve's "transposed" index, whose bits are then interleaved.  Unlike a
-// Morton (Z-order) curve the Hilbert curve has no jumps -- consecutive cells
-// are face neighbours -- so any run of ~60 cells lies inside two adjacent
-// 4x4x4 sub-cubes, which bounds the block windows of the particle kernels.
-uint32_t hilbert3(uint32_t x0, uint32_t x1, uint32_t x2) {
-    const int bits = 10;
-    uint32_t X[3] = {x0 & 1023u, x1 & 1023u, x2 & 1023u};
-    const uint32_t M = 1u << (bits - 1);
-    for (uint32_t Q = M; Q > 1; Q >>= 1) {
-        const uint32_t P = Q - 1;
-        for (int i = 0; i < 3; ++i) {
-            if (X[i] & Q) {
-                X[0] ^= P;
-            } else {
-                const uint32_t t = (X[0] ^ X[i]) & P;
-                X[0] ^= t;
-                X[i] ^= t;
-            }
-        }
-    }
-    for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
-    uint32_t t = 0;
-    for (uint32_t Q = M; Q > 1; Q >>= 1)
-        if (X[2] & Q) t ^= Q - 1;
-    for (int i = 0; i < 3; ++i) X[i] ^= t;
-    uint32_t h = 0;
-    for (int b = bits - 1; b >= 0; --b)
-        for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
-    return h;
-}
-
 size_t dsize(int dtype) { return dtype == DMPM_DTYPE_F64 ? 8 : 4; }
 
 double host_get(const void* p, int dtype, size_t i) {
@@ -1060,121 +1026,48 @@ int dmpm_synchronize(dmpm_sim* s) {
     });
 }
 
-namespace {
-// Order n particles for the particle kernels.  pos(i, a) is coordinate a of
-// particle i in the caller's numbering.  order[j] = the particle stored j-th;
-// starts = the thread blocks' first particles (nblk + 1 entries, the last n),
-// or empty when blocks are not cut (split_nodes < 0).
-void order_particles(const dmpm_config& cfg, int n, const std::function<double(int, int)>& pos,
-                     std::vector<int>& perm, std::vector<int>& starts) {
-    // Hilbert order of the stencil base cell of each particle (host): the
-    // base cell is floor(x/dx - 1/2) for the quadratic B-spline and
-    // floor(x/dx) - 1 for the cubic one, so particles sharing a stencil
-    // end up adjacent and the scatter kernels can merge their sums
-    // across lanes before touching LDS.
-    const double shift = cfg.order == 2 ? 0.5 : 0.0;
-    std::vector<uint64_t> key(n);
-    for (int i = 0; i < n; ++i) {
-        uint32_t c[3];
-        for (int a = 0; a < 3; ++a) {
-            double f = (pos(i, a) - cfg.grid_min[a]) / cfg.dx;
-            long long ci = (long long)std::floor(f - shift) + 2;
-            ci = std::max(0LL, std::min(ci, 1023LL));
-            c[a] = (uint32_t)ci;
-        }
-        key[i] = ((uint64_t)hilbert3(c[0], c[1], c[2]) << 32) | (uint32_t)i;
-    }
-    std::sort(key.begin(), key.end());
-    perm.resize(n);
-    for (int i = 0; i < n; ++i) perm[i] = (int)(key[i] & 0xffffffffu);
-    starts.clear();
+}  // extern "C"
 
-    // Thread blocks of the window kernels: runs of up to `block`
-    // consecutive particles (in Hilbert order), cut where the box of the
-    // run's stencils would exceed split_nodes nodes.  The curve is
-    // continuous, but where it leaves the body and re-enters it elsewhere
-    // consecutive particles lie far apart; a block spanning such a jump
-    // would have a window too large for LDS and fall back to global
-    // atomics (and run several times longer than the others).
-    const long long split = cfg.split_nodes == 0 ? 512 : cfg.split_nodes;
-    if (split > 0) {
-        const int B = cfg.block > 0 ? cfg.block : 256;
-        const long long KS = cfg.order == 2 ? 3 : 4;
-        std::vector<int> cell((size_t)n * 3);
-        for (int j = 0; j < n; ++j)
-            for (int a = 0; a < 3; ++a) {
-                const double f = (pos(perm[j], a) - cfg.grid_min[a]) / cfg.dx;
-                cell[3 * (size_t)j + a] = (int)std::max(-1e9, std::min(1e9, std::floor(f - shift)));
-            }
-        starts.assign(1, 0);
-        for (int i = 0; i < n;) {
-            int lo[3], hi[3];
-            for (int a = 0; a < 3; ++a) lo[a] = hi[a] = cell[3 * (size_t)i + a];
-            int j = i + 1;
-            for (; j < n && j - i < B; ++j) {
-                int l2[3], h2[3];
-                long long vol = 1;
-                for (int a = 0; a < 3; ++a) {
-                    l2[a] = std::min(lo[a], cell[3 * (size_t)j + a]);
-                    h2[a] = std::max(hi[a], cell[3 * (size_t)j + a]);
-                    vol *= (long long)h2[a] - l2[a] + KS;
-                }
-                if (vol > split) break;
-                for (int a = 0; a < 3; ++a) {
-                    lo[a] = l2[a];
-                    hi[a] = h2[a];
-                }
-            }
-            starts.push_back(j);
-            i = j;
-        }
-        // Inside each block, order the runs of particles that share a
-        // base cell so that few of them cross a 16-lane row: the scatter
-        // kernels sum a run across lanes inside a row, and a run cut by a
-        // row boundary leaves two lanes adding to the same LDS address in
-        // one instruction, which the LDS serialises.  Best fit: fill each
-        // row with the longest remaining run that fits its free lanes;
-        // when none fits, the longest remaining run starts there.
-        std::vector<int> tmp;
-        for (size_t b = 0; b + 1 < starts.size(); ++b) {
-            const int s0 = starts[b], s1 = starts[b + 1];
-            std::vector<std::pair<int, int>> runs;  // (length, start)
-            for (int r = s0; r < s1;) {
-                int e = r + 1;
-                while (e < s1 && cell[3 * (size_t)e] == cell[3 * (size_t)r] &&
-                       cell[3 * (size_t)e + 1] == cell[3 * (size_t)r + 1] &&
-                       cell[3 * (size_t)e + 2] == cell[3 * (size_t)r + 2])
-                    ++e;
-                runs.emplace_back(e - r, r);
-                r = e;
-            }
-            std::stable_sort(runs.begin(), runs.end(),
-                             [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first > y.first; });
-            std::vector<char> used(runs.size(), 0);
-            tmp.clear();
-            for (size_t placed = 0; placed < runs.size(); ++placed) {
-                const int room = 16 - (int)(tmp.size() % 16);
-                size_t pick = runs.size();
-                for (size_t q = 0; q < runs.size(); ++q)
-                    if (!used[q] && runs[q].first <= room) {
-                        pick = q;
-                        break;
-                    }
-                if (pick == runs.size())
-                    for (size_t q = 0; q < runs.size(); ++q)
-                        if (!used[q]) {
-                            pick = q;
-                            break;
-                        }
-                used[pick] = 1;
-                for (int k = 0; k < runs[pick].first; ++k) tmp.push_back(perm[runs[pick].second + k]);
-            }
-            std::copy(tmp.begin(), tmp.end(), perm.begin() + s0);
-        }
+namespace {
+// Storage order and thread blocks of n particles (order_particles_device,
+// dmpm_kernels.h) from device positions x: caller (n,3) layout (soa = 0) of
+// dtype f64/f32, or three planes of stride ld of the sim's precision.
+// Returns the permutation (device, n ints: source index of each stored
+// particle) and the block table (device, nblk + 1 starts; null when blocks
+// are not cut), both allocated on the sim.
+struct Order {
+    int* perm = nullptr;
+    int* bst = nullptr;
+    int nblk = 0;
+};
+Order order_on_device(dmpm_sim* s, const void* x, int x_f64, int soa, long long ld, int n) {
+    Order o;
+    const size_t sb = order_scratch_bytes(n);
+    void* scr = s->dalloc(sb);
+    int* starts = (int*)s->dalloc(((size_t)n + 1) * sizeof(int));
+    o.perm = (int*)s->dalloc((size_t)n * sizeof(int));
+    try {
+        o.nblk = order_particles_device(s->cfg.order, s->cfg.block, s->cfg.split_nodes, s->cfg.grid_min, s->cfg.dx, x,
+                                        x_f64, soa, ld, n, o.perm, starts, scr, sb, s->stream);
+    } catch (const std::exception& e) {
+        s->dfree(scr);
+        s->dfree((void*&)starts);
+        throw Error(DMPM_ERR_HIP, e.what());
     }
+    if (o.nblk > 0) {
+        o.bst = (int*)s->dalloc(((size_t)o.nblk + 1) * sizeof(int));
+        HIPCHK(hipMemcpyAsync(o.bst, starts, ((size_t)o.nblk + 1) * sizeof(int), hipMemcpyDeviceToDevice,
+                              s->stream));
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->dfree(scr);
+    s->dfree((void*&)starts);
+    return o;
 }
 
 }  // namespace
+
+extern "C" {
 
 int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const void* F, const void* C,
                        const void* dFc, const void* m, const void* vol, const void* lam, const void* mu, int dtype,
@@ -1185,12 +1078,10 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
         if (n <= 0) throw Error(DMPM_ERR_ARG, "n must be positive");
         if (!x || !v || !F || !C || !m || !vol || !lam || !mu) throw Error(DMPM_ERR_ARG, "null particle array");
         const size_t es = dsize(dtype);
-        std::vector<char> xh((size_t)n * 3 * es);
-        if (on_device) HIPCHK(hipMemcpy(xh.data(), x, xh.size(), hipMemcpyDeviceToHost));
-        else std::memcpy(xh.data(), x, xh.size());
-        std::vector<int> perm, starts;
-        order_particles(s->cfg, n, [&](int i, int a) { return host_get(xh.data(), dtype, 3 * (size_t)i + a); }, perm,
-                        starts);
+        // storage order from the positions on the device (host positions
+        // are uploaded first; device ones are never copied to the host)
+        const void* xd = s->stage(x, (size_t)n * 3 * es, on_device);
+        const Order ord = order_on_device(s, xd, dtype == DMPM_DTYPE_F64, 0, 0, n);
 
         // (re)allocate particle storage
         for (auto& p : s->layer) s->dfree(p);
@@ -1225,13 +1116,9 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
         s->avmax.assign(1, nullptr);
         s->gctl.assign(1, nullptr);
         s->want_gctl.assign(1, 0);
-        if (!starts.empty()) {
-            s->nblk = (int)starts.size() - 1;
-            s->bst = (int*)s->dalloc(starts.size() * sizeof(int));
-            HIPCHK(hipMemcpy(s->bst, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice));
-        }
-        s->perm = (int*)s->dalloc(n * sizeof(int));
-        HIPCHK(hipMemcpy(s->perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        s->nblk = ord.nblk;
+        s->bst = ord.bst;
+        s->perm = ord.perm;
         s->layer[0] = s->dalloc(s->state_bytes());
         s->stat = s->dalloc((size_t)NSTAT * s->ld * s->esz);
         for (auto& p : s->carry) p = s->dalloc(s->state_bytes());
@@ -1584,19 +1471,10 @@ int dmpm_sort_particles(dmpm_sim* s, int layer) {
             throw Error(DMPM_ERR_STATE, "captured graphs or batched episodes hold this sim's buffers; destroy them "
                                         "before sorting its particles");
         const void* st = s->state(layer);
-        std::vector<char> xh((size_t)3 * s->ld * s->esz);
-        HIPCHK(hipMemcpyAsync(xh.data(), st, xh.size(), hipMemcpyDeviceToHost, s->stream));
-        HIPCHK(hipStreamSynchronize(s->stream));
         const long long ld = s->ld;
-        const bool f64 = s->pr == PREC_F64;
-        std::vector<int> q, starts;
-        order_particles(s->cfg, s->n,
-                        [&](int i, int a) {
-                            return f64 ? ((const double*)xh.data())[a * ld + i] : (double)((const float*)xh.data())[a * ld + i];
-                        },
-                        q, starts);
-        int* dq = (int*)s->dalloc((size_t)s->n * sizeof(int));
-        HIPCHK(hipMemcpy(dq, q.data(), (size_t)s->n * sizeof(int), hipMemcpyHostToDevice));
+        // q[j]: the stored index of the particle that moves to slot j
+        const Order ord = order_on_device(s, (const char*)st + PX * ld * s->esz, s->pr == PREC_F64, 1, ld, s->n);
+        int* dq = ord.perm;
         void* tmp = s->dalloc(s->state_bytes());
         // every per-particle array in the internal order: into tmp, then
         // swapped in (whole states) or copied back
@@ -1622,12 +1500,8 @@ int dmpm_sort_particles(dmpm_sim* s, int layer) {
         s->dfree(tmp);
         s->dfree((void*&)dq);
         s->dfree((void*&)s->bst);
-        s->nblk = 0;
-        if (!starts.empty()) {
-            s->nblk = (int)starts.size() - 1;
-            s->bst = (int*)s->dalloc(starts.size() * sizeof(int));
-            HIPCHK(hipMemcpy(s->bst, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice));
-        }
+        s->nblk = ord.nblk;
+        s->bst = ord.bst;
         // recomputed segments were in the old order
         s->seg_base = -1;
     });

@@ -14,7 +14,8 @@ Differences a caller can observe:
     the device on each call (they are live views, as in the reference, but
     carry no host copy).
   * Extra keyword arguments (a superset): CompGraph(..., precision="f32",
-    physics=None, checkpoint_interval=1, recompute_grids=False);
+    physics=None, checkpoint_interval=1, recompute_grids=False,
+    deterministic=False);
     PointCloud.from_arrays(...);
     device=... on the *_torch getters; get_velocities/get_volumes.
 """
@@ -35,6 +36,17 @@ from dmpm.sim import Physics, Simulator, calculate_volumes as _volumes, lame_par
 
 __all__ = ["OptInput", "PointCloud", "Grid", "CompGraph", "load_point_cloud_from_obj", "calculate_lame_parameters",
            "p2g", "calculate_point_cloud_volumes", "get_positions_from_pc"]
+
+
+def _is_cuda(a) -> bool:
+    return type(a).__module__.startswith("torch") and a.is_cuda
+
+
+def _host(a):
+    """numpy view of a host array or a CPU/CUDA tensor (the latter copied)."""
+    if type(a).__module__.startswith("torch"):
+        return a.detach().cpu().numpy()
+    return np.asarray(a)
 
 
 def _vec3(v) -> np.ndarray:
@@ -237,13 +249,14 @@ class CompGraph:
     like the reference) or "f64"."""
 
     def __init__(self, pc: PointCloud, grid: Grid, target_grid: Grid, precision: str = "f32",
-                 physics: Optional[Physics] = None, checkpoint_interval: int = 1, recompute_grids: bool = False):
+                 physics: Optional[Physics] = None, checkpoint_interval: int = 1, recompute_grids: bool = False,
+                 deterministic: bool = False):
         if pc is None or grid is None or target_grid is None:
             raise TypeError("CompGraph(pc, grid, target_grid): arguments must not be None")
         ph = physics or Physics.reference()
         self._sim = Simulator(grid.dims, grid.dx, tuple(float(a) for a in grid.min_point), physics=ph,
                               precision=precision, checkpoint_interval=checkpoint_interval,
-                              recompute_grids=recompute_grids)
+                              recompute_grids=recompute_grids, deterministic=deterministic)
         self._sim.set_state(pc._state())
         self._sim.set_target_grid_mass(target_grid.m)
         self._grid = grid
@@ -278,13 +291,21 @@ class CompGraph:
 
     # -- render gradients (bind.cpp:189-279, 359-528) ------------------------
     def _check_grads(self, dLdF, dLdx, n_expected=None, prefix="dLdF must be (N,3,3), got"):
-        F = np.asarray(dLdF, dtype=np.float32)
-        X = np.asarray(dLdx, dtype=np.float32)
+        """float32, C-contiguous views of the two gradients: numpy arrays for
+        host data, CUDA tensors left on their device (passed to the library
+        as device pointers, never copied to the host)."""
+        if _is_cuda(dLdF) and _is_cuda(dLdx):
+            import torch
+            F = dLdF.detach().to(torch.float32).contiguous()
+            X = dLdx.detach().to(device=F.device, dtype=torch.float32).contiguous()
+        else:
+            F = np.ascontiguousarray(_host(dLdF), dtype=np.float32)
+            X = np.ascontiguousarray(_host(dLdx), dtype=np.float32)
         if F.ndim != 3 or F.shape[1] != 3 or F.shape[2] != 3:
             raise RuntimeError(f"{prefix} ({','.join(str(s) for s in F.shape)})")
         if X.ndim != 2 or X.shape[1] != 3:
             raise RuntimeError(f"dLdx must be (N,3), got ({','.join(str(s) for s in X.shape)})")
-        return np.ascontiguousarray(F), np.ascontiguousarray(X)
+        return F, X
 
     def accumulate_render_grads(self, dLdF_render, dLdx_render):
         if self.get_num_layers() <= 0:
@@ -297,14 +318,22 @@ class CompGraph:
         self._sim.accumulate_last_grad(F=F, x=X)
 
     def set_render_gradients(self, dLdF_render, dLdx_render):
+        """Stored and added to the last layer on every backward pass
+        (bind.cpp:359-451).  CUDA tensors stay on the device: the library
+        reads them in place and sums their norms there."""
         if self.get_num_layers() <= 0:
             raise RuntimeError("CompGraph has no layers!")
         F, X = self._check_grads(dLdF_render, dLdx_render, prefix="dLdF must be (N,3,3), got shape")
         if X.shape[0] != F.shape[0]:
             raise RuntimeError(f"Shape mismatch: dLdF has {F.shape[0]} points, but dLdx has {X.shape[0]}")
         import ctypes as C
-        _L.check(_L.lib().dmpm_set_render_gradients(self._sim.handle, F.shape[0], F.ctypes.data_as(C.c_void_p),
-                                                     X.ctypes.data_as(C.c_void_p), _L.DTYPE_F32, 0))
+        if isinstance(F, np.ndarray):
+            pF, pX, dev = F.ctypes.data_as(C.c_void_p), X.ctypes.data_as(C.c_void_p), 0
+        else:
+            import torch
+            torch.cuda.current_stream().synchronize()
+            pF, pX, dev = C.c_void_p(F.data_ptr()), C.c_void_p(X.data_ptr()), 1
+        _L.check(_L.lib().dmpm_set_render_gradients(self._sim.handle, int(F.shape[0]), pF, pX, _L.DTYPE_F32, dev))
         info = self.get_render_gradient_info()
         print(f"[C++] Stored render gradients for {F.shape[0]} points")
         print(f"[C++]   ||dL_render/dF|| = {info['grad_F_norm']}")
