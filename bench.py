@@ -355,6 +355,54 @@ def full_scale_episode(T: int = 500, K: int = 25, controls: int = 20) -> dict:
     return out
 
 
+def rollout_overhead(sc, T: int, lds_bytes: int, block: int) -> dict:
+    """What the torch.autograd.Function front end (dmpm.rollout) costs on
+    top of the episode it runs, at the metric size: a forward + backward of
+    a T-step rollout from CUDA tensors (particle upload and ordering on the
+    device, the layer copies of set-up, the final-layer getters, the seed and
+    the layer-0 gradient getters) against the same episode on a sim whose
+    particles are already set."""
+    import torch
+    from dmpm import Physics, Simulator
+    from dmpm.rollout import rollout
+    dev = torch.device("cuda:0")
+    st = {k: torch.tensor(np.asarray(v), dtype=torch.float32, device=dev) for k, v in sc.state().items()}
+    for k in ("x", "v", "F", "C"):
+        st[k].requires_grad_(True)
+    sim = Simulator(sc.dims, sc.dx, sc.grid_min, physics=Physics.plain(dt=1e-4), precision="f32",
+                    lds_bytes=lds_bytes, block=block)
+    w = torch.randn(sc.n, 3, device=dev)
+
+    def once():
+        xT, vT, FT, CT = rollout(sim, st, T)
+        (xT * w).sum().backward()
+        torch.cuda.synchronize()
+
+    once()
+    t0 = time.perf_counter()
+    once()
+    w_roll = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sim.set_particles(st["x"], st["v"], st["F"], st["C"], st["m"], st["vol"], st["lam"], st["mu"])
+    sim.synchronize()
+    w_set = time.perf_counter() - t0
+    sim.set_up(T + 1)
+    sim.set_last_grad(x=w)
+    run_episodes(sim, 1)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    run_episodes(sim, 1)
+    sim.synchronize()
+    w_ep = time.perf_counter() - t0
+    sim.close()
+    out = {"particles": sc.n, "episode_steps": T, "rollout_fwd_bwd_ms": 1e3 * w_roll, "episode_ms": 1e3 * w_ep,
+           "overhead_ms": 1e3 * (w_roll - w_ep), "set_particles_ms": 1e3 * w_set,
+           "note": "episodes in the upload order (no re-sort), so both sides run the same kernels"}
+    log(f"[bench] rollout front end: {1e3 * w_roll:.1f} ms per fwd+bwd call against {1e3 * w_ep:.1f} ms for the "
+        f"episode ({1e3 * (w_roll - w_ep):.1f} ms overhead; set_particles {1e3 * w_set:.1f} ms)")
+    return out
+
+
 def fp32_errors(o32, g32, o64, g64, n) -> dict:
     """Max and RMS of |x32 - x64| (the box is the unit cube) and |F32 - F64|_F
     at the last layer; relative L2 error of each layer-0 gradient."""
@@ -431,6 +479,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="use the default LDS budget and block size")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-instance batched workload")
     ap.add_argument("--no-full-scale", action="store_true", help="skip the 4M-particle, 256^3, 500-step episode")
+    ap.add_argument("--no-rollout", action="store_true",
+                    help="skip the timing of the torch.autograd front end (dmpm.rollout) at the metric size")
     ap.add_argument("--no-implosion", action="store_true",
                     help="skip the strongly compressed companion scene (FP32 vs FP64 at det F ~ 0.3)")
     ap.add_argument("--no-sort", action="store_true",
@@ -608,6 +658,8 @@ def main():
                                                  "trust region 0.6, reference adjoint"}
         log(f"[bench] reference physics: {1e3 * wr:.1f} ms/episode")
 
+    if not args.no_rollout and default_workload:
+        extra["rollout"] = rollout_overhead(sc, T, lds_bytes, block)
     if not args.no_batched:
         extra["batched"] = batched_workload(lds_bytes=lds_bytes, block=block)
     if not args.no_full_scale:

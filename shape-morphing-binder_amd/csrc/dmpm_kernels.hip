@@ -88,6 +88,7 @@ struct KP {
     long long zero16;  // its 16-byte chunks
     int zblk;          // logical blocks sharing the clear
     int cap;
+    int park;          // adjoint gather: byte offset in LDS of the values parked through its node loop
 };
 
 static inline unsigned grid_blocks(int n, int block) {
@@ -147,6 +148,7 @@ static KP<T> make_kp(const Launch& L, int narrays) {
     k.zero16 = L.zero ? L.zero_bytes / 16 : 0;
     k.zblk = (int)launch_blocks(L);
     k.cap = narrays > 0 ? (int)(L.lds_bytes / narrays) : 0;  // narrays = LDS bytes per window node
+    k.park = 0;
     return k;
 }
 
@@ -334,12 +336,46 @@ DMPM_DI long long gindex(const KP<T>& a, int i, int j, int k) {
     return ((long long)i * a.ny + j) * a.nz + k;
 }
 
+// Particle planes are read and written with raw buffer instructions: one
+// buffer resource per array (SGPRs), the particle's byte offset i * sizeof(T)
+// in one VGPR shared by every plane, and the plane's byte offset
+// plane * ld * sizeof(T) in an SGPR.  A 64-bit vector address per plane
+// access (the global form) costs VALU instructions and two VGPRs each.
+// Offsets are 32-bit: an array holds fewer than 2^32 bytes (the host checks).
+DMPM_DI __amdgpu_buffer_rsrc_t plane_rsrc(const void* base) {
+    // stride 0, num_records 0xffffffff, dword 3: 32-bit data format (gfx9)
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, -1, 0x00020000);
+}
+template <typename T>
+DMPM_DI T load_s(const T* base, long long ld, int plane, int i) {
+    const __amdgpu_buffer_rsrc_t r = plane_rsrc(base);
+    const int vo = i * (int)sizeof(T), so = (int)(plane * ld * (long long)sizeof(T));
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+    }
+}
+template <typename T>
+DMPM_DI void store_s(T* base, long long ld, int plane, int i, T v) {
+    const __amdgpu_buffer_rsrc_t r = plane_rsrc(base);
+    const int vo = i * (int)sizeof(T), so = (int)(plane * ld * (long long)sizeof(T));
+    if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, 0);
+    } else {
+        typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vo, so, 0);
+    }
+}
+
 // Grid velocity update (ForwardSimulation.cpp:82-87) plus sticky walls.
 template <typename T>
 DMPM_DI void node_velocity(const KP<T>& a, long long g, int i, int j, int k, T& v0, T& v1, T& v2) {
-    // all four loads issued before the branch (one memory round trip)
-    const T m = a.gm[g];
-    const T p0 = a.gp[g], p1 = a.gp[a.gld + g], p2 = a.gp[2 * a.gld + g];
+    // all four loads issued before the branch (one memory round trip); the
+    // momentum planes follow the mass plane (gp = gm + gld)
+    const T m = load_s(a.gm, a.gld, 0, (int)g);
+    const T p0 = load_s(a.gm, a.gld, 1, (int)g), p1 = load_s(a.gm, a.gld, 2, (int)g);
+    const T p2 = load_s(a.gm, a.gld, 3, (int)g);
     if (m > T(1e-12) && !is_wall(a, i, j, k)) {
         const T inv = T(1) / m;
         v0 = p0 * inv + a.dt * a.f0;
@@ -354,25 +390,25 @@ template <typename T>
 DMPM_DI m3<T> load_m3(const T* base, long long ld, int plane, int i) {
     m3<T> M;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) M.a[q] = base[(plane + q) * ld + i];
+    for (int q = 0; q < 9; ++q) M.a[q] = load_s(base, ld, plane + q, i);
     return M;
 }
 template <typename T>
 DMPM_DI v3<T> load_v3(const T* base, long long ld, int plane, int i) {
     v3<T> v;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) v.a[q] = base[(plane + q) * ld + i];
+    for (int q = 0; q < 3; ++q) v.a[q] = load_s(base, ld, plane + q, i);
     return v;
 }
 template <typename T>
 DMPM_DI void store_m3(T* base, long long ld, int plane, int i, const m3<T>& M) {
 #pragma unroll
-    for (int q = 0; q < 9; ++q) base[(plane + q) * ld + i] = M.a[q];
+    for (int q = 0; q < 9; ++q) store_s(base, ld, plane + q, i, M.a[q]);
 }
 template <typename T>
 DMPM_DI void store_v3(T* base, long long ld, int plane, int i, const v3<T>& v) {
 #pragma unroll
-    for (int q = 0; q < 3; ++q) base[(plane + q) * ld + i] = v.a[q];
+    for (int q = 0; q < 3; ++q) store_s(base, ld, plane + q, i, v.a[q]);
 }
 
 // Per-particle stencil: base node and per-axis weights.
@@ -461,7 +497,7 @@ DMPM_DI void p2g_terms(const KP<T>& a, T vol, T lam, T mu, const v3<T>& v, const
 template <typename T, int ORDER, int MODEL>
 DMPM_DI void p2g_terms(const KP<T>& a, int i, const v3<T>& v, const m3<T>& Ft, const m3<T>& C, T m, m3<T>& A,
                        v3<T>& mv) {
-    p2g_terms<T, ORDER, MODEL>(a, a.stat[SVOL * a.ld + i], a.stat[SLAM * a.ld + i], a.stat[SMU * a.ld + i], v, Ft,
+    p2g_terms<T, ORDER, MODEL>(a, load_s(a.stat, a.ld, SVOL, i), load_s(a.stat, a.ld, SLAM, i), load_s(a.stat, a.ld, SMU, i), v, Ft,
                                C, m, A, mv);
 }
 
@@ -725,7 +761,7 @@ DMPM_DI void p2g_block(const KP<T>& a, int lblock, double* smem_d) {
     m3<T> A = mzero<T>();
     v3<T> mv = vzero<T>();
     if (valid) {
-        m = a.stat[SM * a.ld + i];
+        m = load_s(a.stat, a.ld, SM, i);
         if (!MASS_ONLY) {
             m3<T> F = load_m3(a.st, a.ld, PF, i);
             if (a.ctl) F = madd(F, load_m3(a.ctl, a.ld, 0, i));
@@ -942,10 +978,10 @@ DMPM_DI void step_block(const KP<T>& a, int lblock, double* smem_d) {
         F = load_m3(a.st, a.ld, PF, i);
         Ft = a.ctl ? madd(F, load_m3(a.ctl, a.ld, 0, i)) : F;
         if (a.ctl1) ctl1 = load_m3(a.ctl1, a.ld, 0, i);
-        m = a.stat[SM * a.ld + i];
-        vol = a.stat[SVOL * a.ld + i];
-        lam = a.stat[SLAM * a.ld + i];
-        mu = a.stat[SMU * a.ld + i];
+        m = load_s(a.stat, a.ld, SM, i);
+        vol = load_s(a.stat, a.ld, SVOL, i);
+        lam = load_s(a.stat, a.ld, SLAM, i);
+        mu = load_s(a.stat, a.ld, SMU, i);
     }
     S.init(a, x);  // lanes without a particle get a finite stencil at x = 0
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
@@ -1200,9 +1236,11 @@ __global__ void __launch_bounds__(256) k_adj_scatter_batch(const KP<T>* tab, con
 template <typename T, bool REF>
 DMPM_DI void node_adjoint(const KP<T>& a, long long g, int i, int j, int k, T* v, T* h, T& gmass) {
     // all seven loads issued before the branch (one memory round trip)
-    const T m = a.gm[g];
-    const T q0 = a.gp[g], q1 = a.gp[a.gld + g], q2 = a.gp[2 * a.gld + g];
-    const T g0 = a.ggv[g], g1 = a.ggv[a.gld + g], g2 = a.ggv[2 * a.gld + g];
+    const T m = load_s(a.gm, a.gld, 0, (int)g);
+    const T q0 = load_s(a.gm, a.gld, 1, (int)g), q1 = load_s(a.gm, a.gld, 2, (int)g);
+    const T q2 = load_s(a.gm, a.gld, 3, (int)g);
+    const T g0 = load_s(a.ggv, a.gld, 0, (int)g), g1 = load_s(a.ggv, a.gld, 1, (int)g);
+    const T g2 = load_s(a.ggv, a.gld, 2, (int)g);
     if (m > T(1e-12) && !is_wall(a, i, j, k)) {
         const T inv = T(1) / m;
         const T p0 = q0 * inv, p1 = q1 * inv, p2 = q2 * inv;
@@ -1285,8 +1323,8 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
         }
         const Phase1<T, REF> p1 = phase1<T, REF>(a, cy, Ft, Craw, tr);
 
-        const T m = a.stat[SM * a.ld + i], vol = a.stat[SVOL * a.ld + i];
-        const T lam = a.stat[SLAM * a.ld + i], mu = a.stat[SMU * a.ld + i];
+        const T m = load_s(a.stat, a.ld, SM, i), vol = load_s(a.stat, a.ld, SVOL, i);
+        const T lam = load_s(a.stat, a.ld, SLAM, i), mu = load_s(a.stat, a.ld, SMU, i);
         const v3<T> vp = load_v3(a.st, a.ld, PV, i);
         const m3<T> C = load_m3(a.st, a.ld, PC, i);
         const T dinv = Spline<ORDER>::dinv(a.dx);
@@ -1304,6 +1342,35 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
         const m3<T> Gc = mscale(dinv, p1.gCraw);
         const v3<T> mvp = {{mom * vp.a[0], mom * vp.a[1], mom * vp.a[2]}};
         const v3<T> gvtot = p1.gvtot;
+        // What the code after the node loop needs from phase 1 is formed now
+        // and parked in LDS for the loop's duration (12 values per thread,
+        // lane-consecutive): the dL/dF terms other than the stress's,
+        //   Q = (1-s) A'^T dL/dF' + s dL/dF'            (exact)
+        //   Q = (1-s) A'^T dL/dF' + s dL/dF' + A'^T dL/dF'  (reference: added
+        //       twice, BackPropagation.cpp:27-28 and :140),
+        // and the dL/dx terms other than the gather's, dL/dx' - Gc^T v'.  The
+        // carry, v' and C' are then not reloaded after the loop.
+        {
+            m3<T> Q = p1.gFt;
+            if (REF) {
+                const m3<T> Cn = tr.state ? mscale(tr.s, Craw) : Craw;
+                m3<T> A1 = mscale(a.dt, Cn);
+                A1.a[0] += T(1); A1.a[4] += T(1); A1.a[8] += T(1);
+                Q = madd(Q, mtmul(A1, cy.gF));
+            } else {
+                Q = madd(Q, p1.gFdir);
+            }
+            const v3<T> Cv = mtv(Gc, vn);
+            T* park = reinterpret_cast<T*>(reinterpret_cast<char*>(smem_d) + a.park);
+            const int pt = threadIdx.x, pb = blockDim.x;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) park[q * pb + pt] = Q.a[q];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) park[(9 + q) * pb + pt] = cy.gx.a[q] - Cv.a[q];
+            // dL/d dFc of the layer: the same terms less the smoothing
+            // blend's in the exact adjoint; the stress term is added below
+            if (a.gctl) store_m3(a.gctl, a.ld, 0, i, REF ? Q : p1.gFt);
+        }
 
         // accumulators
         m3<T> gA = mzero<T>();   // sum w h dgp^T
@@ -1414,60 +1481,48 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
                         add(ia, ib, ic, v, h, gmass);
                     }
         }
-        // What only the code below needs (the carry, F + dFc, v' and C', the
-        // phase-1 terms of F and P) is loaded or computed again here instead of
-        // being held in registers through the node loop: the kernel then fits
-        // three waves per SIMD.  The index goes through an empty asm statement so
-        // that the compiler reloads instead of keeping the earlier values.
+        // F + dFc is loaded again here rather than held through the loop, and
+        // the parked phase-1 terms are read back.  The index goes through an
+        // empty asm statement so that the compiler reloads instead of keeping
+        // the earlier values.
         int j = i;
         asm volatile("" : "+v"(j));
-        const Carry<T> cy2 = load_carry(a, j);
         m3<T> Ft2 = load_m3(a.st, a.ld, PF, j);
         if (a.ctl) Ft2 = madd(Ft2, load_m3(a.ctl, a.ld, 0, j));
-        v3<T> vn2;
-        m3<T> Craw2;
-        Trust<T> tr2;
-        if (nxt) {
-            vn2 = load_v3(a.nxt, a.ld, PV, j);
-            Craw2 = load_m3(a.nxt, a.ld, PC, j);
-            tr2 = trust_none<T>();
-        } else {
-            g2p_gather<T, ORDER>(a, win, sv, S, vn2, Craw2);
-            tr2 = trust_region(a, Ft2, Craw2);
+        m3<T> Q;
+        v3<T> gxp;
+        {
+            const T* park = reinterpret_cast<const T*>(reinterpret_cast<const char*>(smem_d) + a.park);
+            const int pt = threadIdx.x, pb = blockDim.x;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) Q.a[q] = park[q * pb + pt];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) gxp.a[q] = park[(9 + q) * pb + pt];
         }
-        const Phase1<T, REF> p1b = phase1<T, REF>(a, cy2, Ft2, Craw2, tr2);
         // dL/dx_p: sum s_i grad w - Gt^T sum w h - D^{-1} gCraw^T v' + dL/dx'
-        const v3<T> gx1 = cy2.gx;
         const v3<T> Gsh = mtv(Gt, sh);
-        const v3<T> Cv = mtv(Gc, vn2);
         v3<T> gx;
     #pragma unroll
-        for (int q = 0; q < 3; ++q) gx.a[q] = dxa.a[q] - Gsh.a[q] - Cv.a[q] + gx1.a[q];
+        for (int q = 0; q < 3; ++q) gx.a[q] = dxa.a[q] - Gsh.a[q] + gxp.a[q];
         const T dragf = m * (T(1) - a.dt * a.drag);
         const v3<T> gv = {{dragf * sh.a[0], dragf * sh.a[1], dragf * sh.a[2]}};
         // A_p = -k0 (P Ft^T + zeta tr C I) + m C: the stress term's gradient
         // in Ft is that of <-k0 gA, P(Ft) Ft^T>
-        m3<T> gFt = madd(p1b.gFt, Model<MODEL>::kirchhoff_vjp(Ft2, lam, mu, mscale(-k0, gA)));
+        const m3<T> gS = Model<MODEL>::kirchhoff_vjp(Ft2, lam, mu, mscale(-k0, gA));
         m3<T> gC = mscale(m, gA);
         if (REF) {
-            // bulk-viscosity adjoint on the diagonal entries only (:117-123), and
-            // (I + dt C')^T dL/dF' added a second time (:140)
+            // bulk-viscosity adjoint on the diagonal entries only (:117-123)
             const T factor = -k0 * a.zeta;
             gC.a[0] += factor * gA.a[0];
             gC.a[4] += factor * gA.a[4];
             gC.a[8] += factor * gA.a[8];
-            const m3<T> gF1 = cy2.gF;
-            const m3<T> Cn = tr2.state ? mscale(tr2.s, Craw2) : Craw2;
-            m3<T> A1 = mscale(a.dt, Cn);
-            A1.a[0] += T(1); A1.a[4] += T(1); A1.a[8] += T(1);
-            gFt = madd(gFt, mtmul(A1, gF1));
         } else {
             const T tb = -k0 * a.zeta * mtrace(gA);
             gC.a[0] += tb; gC.a[4] += tb; gC.a[8] += tb;
         }
+        const m3<T> gF = madd(Q, gS);
         store_v3(a.adj0, a.ld, PX, i, gx);
         store_v3(a.adj0, a.ld, PV, i, gv);
-        const m3<T> gF = madd(gFt, p1b.gFdir);
         store_m3(a.adj0, a.ld, PF, i, gF);
         store_m3(a.adj0, a.ld, PC, i, gC);
         if (FUSED) {
@@ -1476,7 +1531,7 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
             cyt.gF = gF;
             cyt.gC = gC;
         }
-        if (a.gctl) store_m3(a.gctl, a.ld, 0, i, REF ? gF : gFt);
+        if (a.gctl) store_m3(a.gctl, a.ld, 0, i, madd(load_m3(a.gctl, a.ld, 0, j), gS));
     }
     if constexpr (FUSED) {
         v3<T> x1 = vzero<T>();
@@ -1572,7 +1627,7 @@ __global__ void __launch_bounds__(256) k_det_prep_p2g(KP<T> a, unsigned* keys, i
     S.init(a, x);
     keys[i] = det_key(a, S.b);
     idx[i] = i;
-    const T m = a.stat[SM * a.ld + i];
+    const T m = load_s(a.stat, a.ld, SM, i);
     pay[i] = m;
     if (!MASS_ONLY) {
         m3<T> A;
@@ -1901,7 +1956,7 @@ __global__ void k_mass_loss_particles(KP<T> a, const T* dldm, const T* tm) {
     const v3<T> x = load_v3(a.st, a.ld, PX, i);
     Sten<T, ORDER> S;
     S.init(a, x);
-    const T m = a.stat[SM * a.ld + i];
+    const T m = load_s(a.stat, a.ld, SM, i);
     v3<T> g = vzero<T>();
     const T invdx = T(-1) / a.dx;
     for (int ia = 0; ia < KS; ++ia)
@@ -1940,7 +1995,7 @@ __global__ void k_volumes(KP<T> a, T* vol) {
     const T cell = a.dx * a.dx * a.dx;
     const T rho = mg / fmax(cell, T(1e-12));
     const T eps = sizeof(T) == 4 ? T(1.1920929e-07) : T(2.220446049250313e-16);
-    vol[i] = rho > eps ? a.stat[SM * a.ld + i] / rho : T(0);
+    vol[i] = rho > eps ? load_s(a.stat, a.ld, SM, i) / rho : T(0);
 }
 
 template <typename T>
@@ -2015,6 +2070,29 @@ __global__ void k_adam(const T* dLdF, T* dFc, T* mom, T* vec, T* vmax, int n, lo
 
 // ---------------------------------------------------------------------------
 // Launchers.
+
+// LDS of the adjoint gather: its window (budget bytes), then 12 values per
+// thread parked through the node loop.  With FP32 the window budget is held
+// to 27 KiB so that four 256-thread blocks (window + 12 KiB parked) share a
+// CU's 160 KiB.
+constexpr int kParkValues = 12;
+template <typename T>
+static int adj_window_budget(int lds_bytes) {
+    return sizeof(T) == 4 ? std::min(lds_bytes, 27 * 1024) : lds_bytes;
+}
+template <typename T>
+static size_t adj_gather_lds(int budget, int block) {
+    return (size_t)((budget + 15) & ~15) + (size_t)kParkValues * block * sizeof(T);
+}
+template <typename T>
+static KP<T> adj_gather_kp(const Launch& L) {
+    Launch Lw = L;
+    Lw.lds_bytes = adj_window_budget<T>(L.lds_bytes);
+    KP<T> k = make_kp<T>(Lw, 7 * (int)sizeof(T));
+    k.park = (Lw.lds_bytes + 15) & ~15;
+    return k;
+}
+
 #if DMPM_IN(1) || DMPM_IN(2)
 template <typename T>
 static void launch_p2g(const Launch& L, bool mass_only, hipStream_t s) {
@@ -2117,9 +2195,9 @@ static void launch_adj_scatter(const Launch& L, hipStream_t s) {
 
 template <typename T>
 static void launch_adj_gather(const Launch& L, hipStream_t s) {
-    KP<T> k = make_kp<T>(L, 7 * (int)sizeof(T));
+    KP<T> k = adj_gather_kp<T>(L);
     const unsigned nb = launch_blocks(L);
-    const size_t sh = (size_t)k.cap * 7 * sizeof(T);
+    const size_t sh = adj_gather_lds<T>(adj_window_budget<T>(L.lds_bytes), L.block);
     const bool ref = L.ph.adjoint == 1;
     const int o = L.ph.order, md = L.ph.model;
 #define AG_CASE(O, M, R) hipLaunchKernelGGL((k_adj_gather<T, O, M, R>), dim3(nb), dim3(L.block), sh, s, k)
@@ -2177,10 +2255,13 @@ template void batch_adj<double>(int, const Phys&, int, size_t, unsigned, const v
 #if DMPM_IN(5) || DMPM_IN(6)
 template <typename T>
 static void launch_adj_fused(const Launch& L, const Launch& L2, hipStream_t s) {
-    KP<T> k = make_kp<T>(L, 7 * (int)sizeof(T));
-    KP<T> k2 = make_kp<T>(L2, 3 * (int)sizeof(Acc));
+    KP<T> k = adj_gather_kp<T>(L);
+    Launch Lw2 = L2;
+    Lw2.lds_bytes = adj_window_budget<T>(L2.lds_bytes);
+    KP<T> k2 = make_kp<T>(Lw2, 3 * (int)sizeof(Acc));
     const unsigned nb = launch_blocks(L);
-    const size_t sh = std::max((size_t)k.cap * 7 * sizeof(T), (size_t)k2.cap * 3 * sizeof(Acc));
+    const size_t sh = std::max(adj_gather_lds<T>(adj_window_budget<T>(L.lds_bytes), L.block),
+                               (size_t)k2.cap * 3 * sizeof(Acc));
     const bool ref = L.ph.adjoint == 1;
     const int o = L.ph.order, md = L.ph.model;
 #define AF_CASE(O, M, R) hipLaunchKernelGGL((k_adj_fused<T, O, M, R>), dim3(nb), dim3(L.block), sh, s, k, k2)
@@ -2253,12 +2334,15 @@ size_t kp_bytes(Prec pr) { return pr == PREC_F32 ? sizeof(KP<float>) : sizeof(KP
 
 template <typename T>
 static void build_kp_t(int kind, const Launch& L, const Launch* L2, void* dst) {
-    KP<T> k = make_kp<T>(L, node_bytes<T>(kind));
+    const bool adj = kind == OP_ADJ_GATHER || kind == OP_ADJ_FUSED;
+    KP<T> k = adj ? adj_gather_kp<T>(L) : make_kp<T>(L, node_bytes<T>(kind));
     k.zblk = instance_blocks(L);  // the instance's blocks of a batched launch
     if (k.zero && k.zblk <= 0) throw std::runtime_error("a clear over an instance without blocks");
     std::memcpy(dst, &k, sizeof(k));
     if (kind == OP_ADJ_FUSED) {
-        const KP<T> k2 = make_kp<T>(*L2, 3 * (int)sizeof(Acc));
+        Launch Lw2 = *L2;
+        Lw2.lds_bytes = adj_window_budget<T>(L2->lds_bytes);
+        const KP<T> k2 = make_kp<T>(Lw2, 3 * (int)sizeof(Acc));
         std::memcpy((char*)dst + sizeof(k), &k2, sizeof(k2));
     }
 }
@@ -2279,7 +2363,11 @@ static void launch_batch_t(int kind, const Phys& ph, int lds_bytes, int block, c
     const unsigned nb = (unsigned)std::max(8, (total + 7) & ~7);
     const int cap = lds_bytes / node_bytes<T>(kind);
     size_t sh = block_lds<T>(kind, cap);
-    if (kind == OP_ADJ_FUSED) sh = std::max(sh, (size_t)(lds_bytes / (3 * (int)sizeof(Acc))) * 3 * sizeof(Acc));
+    if (kind == OP_ADJ_GATHER || kind == OP_ADJ_FUSED) {
+        const int budget = adj_window_budget<T>(lds_bytes);
+        sh = adj_gather_lds<T>(budget, block);
+        if (kind == OP_ADJ_FUSED) sh = std::max(sh, (size_t)(budget / (3 * (int)sizeof(Acc))) * 3 * sizeof(Acc));
+    }
     switch (kind) {
         case OP_P2G:
         case OP_P2G_MASS:

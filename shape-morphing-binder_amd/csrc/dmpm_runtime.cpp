@@ -986,6 +986,9 @@ int dmpm_create(const dmpm_config* cfg, dmpm_sim** out) {
         for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
         s->G = (long long)cfg->dims[0] * cfg->dims[1] * cfg->dims[2];
         s->gld = round_up(s->G, 64);
+        // the kernels address a grid with 32-bit byte offsets
+        if ((unsigned long long)4 * s->gld * s->esz >= (1ull << 32))
+            throw Error(DMPM_ERR_ARG, "grid too large: its four planes must stay below 4 GiB");
         s->grid = s->dalloc(4 * s->gld * s->esz);
         s->grid2 = s->dalloc(4 * s->gld * s->esz);
         s->ggv = s->dalloc(3 * s->gld * s->esz);
@@ -1077,6 +1080,10 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
         s->refuse_if_captured("dmpm_set_particles");
         if (n <= 0) throw Error(DMPM_ERR_ARG, "n must be positive");
         if (!x || !v || !F || !C || !m || !vol || !lam || !mu) throw Error(DMPM_ERR_ARG, "null particle array");
+        // the kernels address a particle array with 32-bit byte offsets
+        if ((unsigned long long)NSTATE * round_up(n, 64) * s->esz >= (1ull << 32))
+            throw Error(DMPM_ERR_ARG, "too many particles for one sim: a layer must stay below 4 GiB (" +
+                                          std::to_string((1ull << 32) / (NSTATE * s->esz)) + " particles)");
         const size_t es = dsize(dtype);
         // storage order from the positions on the device (host positions
         // are uploaded first; device ones are never copied to the host)
