@@ -386,6 +386,17 @@ DMPM_DI void node_velocity(const KP<T>& a, long long g, int i, int j, int k, T& 
     }
 }
 
+// Grid node g of plane `plane` += v, a device-scope float atomic (buffer form
+// for FP32; FP64 keeps the global form, which has no buffer counterpart).
+template <typename T>
+DMPM_DI void atomic_add_s(T* base, long long ld, int plane, long long g, T v) {
+    if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_ptr_buffer_atomic_fadd_f32(v, plane_rsrc(base), (int)g * 4, (int)(plane * ld * 4), 0);
+    } else {
+        unsafeAtomicAdd(base + plane * ld + g, v);
+    }
+}
+
 template <typename T>
 DMPM_DI m3<T> load_m3(const T* base, long long ld, int plane, int i) {
     m3<T> M;
@@ -647,7 +658,7 @@ DMPM_DI int window_base(const Win& w, const int* b) {
 // the flush drops, or are skipped on the global path.
 template <typename T, int ORDER, int NV, typename Val>
 DMPM_DI void scatter_nodes(const KP<T>& a, const Win& win, Acc* sm, const Sten<T, ORDER>& S, bool valid,
-                           T* const* dst, Val&& val) {
+                           T* dst, Val&& val) {
     constexpr int KS = Spline<ORDER>::K;
     const SegRun<T> run = seg_run<T>(valid, S.b);
     const bool issue = valid && run.head;
@@ -685,7 +696,7 @@ DMPM_DI void scatter_nodes(const KP<T>& a, const Win& win, Acc* sm, const Sten<T
                     if (!issue || !in_grid(a, gi, gj, gk)) continue;
                     const long long g = gindex(a, gi, gj, gk);
 #pragma unroll
-                    for (int q = 0; q < NV; ++q) unsafeAtomicAdd(dst[q] + g, c[q]);
+                    for (int q = 0; q < NV; ++q) atomic_add_s(dst, a.gld, q, g, c[q]);
                 }
             }
         }
@@ -698,7 +709,7 @@ DMPM_DI void scatter_nodes(const KP<T>& a, const Win& win, Acc* sm, const Sten<T
 template <typename T, int ORDER, bool MASS_ONLY>
 DMPM_DI void p2g_scatter(const KP<T>& a, const Win& win, Acc* sm, const Sten<T, ORDER>& S, bool valid, T m,
                          const m3<T>& A, const v3<T>& mv, T* gm, T* gp) {
-    T* const dst[4] = {gm, gp, gp + a.gld, gp + 2 * a.gld};
+    T* const dst = gm;  // planes m, p0, p1, p2 (gp = gm + gld)
     scatter_nodes<T, ORDER, MASS_ONLY ? 1 : 4>(a, win, sm, S, valid, dst, [&](int ia, int ib, int ic, T* c) {
         const T d0 = S.u[0][ia] * a.dx, d1 = S.u[1][ib] * a.dx, d2 = S.u[2][ic] * a.dx;
         const T wt = S.w[0][ia] * S.w[1][ib] * S.w[2][ic];
@@ -727,14 +738,14 @@ DMPM_DI void p2g_flush(const KP<T>& a, const Win& win, const Acc* sm, T* gm, T* 
         const T vm = (T)sm[t];
         const long long g = gindex(a, gi, gj, gk);
         if (MASS_ONLY) {
-            if (vm != T(0)) unsafeAtomicAdd(gm + g, vm);
+            if (vm != T(0)) atomic_add_s(gm, a.gld, 0, g, vm);
         } else {
             const T p0 = (T)sm[a.cap + t], p1 = (T)sm[2 * a.cap + t], p2 = (T)sm[3 * a.cap + t];
             if (vm != T(0) || p0 != T(0) || p1 != T(0) || p2 != T(0)) {
-                unsafeAtomicAdd(gm + g, vm);
-                unsafeAtomicAdd(gp + g, p0);
-                unsafeAtomicAdd(gp + a.gld + g, p1);
-                unsafeAtomicAdd(gp + 2 * a.gld + g, p2);
+                atomic_add_s(gm, a.gld, 0, g, vm);  // the momentum planes follow: gp = gm + gld
+                atomic_add_s(gm, a.gld, 1, g, p0);
+                atomic_add_s(gm, a.gld, 2, g, p1);
+                atomic_add_s(gm, a.gld, 3, g, p2);
             }
         }
     }
@@ -1192,7 +1203,7 @@ DMPM_DI void adj_scatter_core(const KP<T>& a, bool valid, const v3<T>& x, const 
     }
     // dL/dv_i += w (dL/dv' + D^{-1} dL/dC' (x_i - x_p)); lanes without a
     // particle scatter zeros
-    T* const dst[3] = {a.ggv, a.ggv + a.gld, a.ggv + 2 * a.gld};
+    T* const dst = a.ggv;
     scatter_nodes<T, ORDER, 3>(a, win, sg, S, valid, dst, [&](int ia, int ib, int ic, T* c) {
         const T d0 = S.u[0][ia] * a.dx, d1 = S.u[1][ib] * a.dx, d2 = S.u[2][ic] * a.dx;
         const T wt = S.w[0][ia] * S.w[1][ib] * S.w[2][ic];
@@ -1208,9 +1219,9 @@ DMPM_DI void adj_scatter_core(const KP<T>& a, bool valid, const v3<T>& x, const 
             const T g0 = (T)sg[t], g1 = (T)sg[a.cap + t], g2 = (T)sg[2 * a.cap + t];
             if (g0 != T(0) || g1 != T(0) || g2 != T(0)) {
                 const long long g = gindex(a, gi, gj, gk);
-                unsafeAtomicAdd(a.ggv + g, g0);
-                unsafeAtomicAdd(a.ggv + a.gld + g, g1);
-                unsafeAtomicAdd(a.ggv + 2 * a.gld + g, g2);
+                atomic_add_s(a.ggv, a.gld, 0, g, g0);
+                atomic_add_s(a.ggv, a.gld, 1, g, g1);
+                atomic_add_s(a.ggv, a.gld, 2, g, g2);
             }
         }
     }
