@@ -224,6 +224,7 @@ DMPM_DI void block_range(const KP<T>& a, int lblock, int& i0, int& i1) {
 // landing on one every few rows; vol counts the padded slots.
 struct Win {
     int lo0, lo1, lo2, e1, e2, s1, s2, vol;
+    unsigned r1, r2;  // ceil(2^32 / s1), ceil(2^32 / s2): slot -> node without a division
     bool lds;
     bool any;
 };
@@ -288,6 +289,7 @@ DMPM_DI Win block_window(bool valid, int b0, int b1, int b2, int cap) {
     w.lo0 = l0; w.lo1 = l1; w.lo2 = l2;
     if (!w.any) {
         w.e1 = w.e2 = w.s1 = w.s2 = 0; w.vol = 0; w.lds = true;
+        w.r1 = w.r2 = 0u;
         return w;
     }
     const long long e0 = (long long)h0 - l0 + KS, e1 = (long long)h1 - l1 + KS, e2 = (long long)h2 - l2 + KS;
@@ -301,13 +303,20 @@ DMPM_DI Win block_window(bool valid, int b0, int b1, int b2, int cap) {
     w.s1 = w.lds ? (int)s1 : 0;
     w.s2 = w.lds ? (int)s2 : 0;
     w.vol = w.lds ? (int)vol : 0;
+    // 0xffffffff / s + 1 = ceil(2^32 / s) for s >= 2 (s2 >= KS >= 3 here)
+    w.r1 = w.lds ? 0xffffffffu / (unsigned)w.s1 + 1u : 0u;
+    w.r2 = w.lds ? 0xffffffffu / (unsigned)w.s2 + 1u : 0u;
     return w;
 }
 
 // Grid node of window slot t; false for the padding slots of the strides.
+// The quotients come from the multipliers r = ceil(2^32 / s): with t = q s + e
+// (0 <= e < s), t r / 2^32 = q + e / s + t (r - 2^32 / s) / 2^32, and the last
+// two terms stay below 1 while t s < 2^32 (t and s are below 2^16: windows
+// hold at most 160 KiB / 8 B nodes and callers pass t < vol + 4 * blockDim.x).
 DMPM_DI bool window_node(const Win& w, int t, int& gi, int& gj, int& gk) {
-    const int i = t / w.s1, r = t - i * w.s1;
-    const int j = r / w.s2, k = r - j * w.s2;
+    const int i = (int)__umulhi((unsigned)t, w.r1), r = t - i * w.s1;
+    const int j = (int)__umulhi((unsigned)r, w.r2), k = r - j * w.s2;
     gi = w.lo0 + i;
     gj = w.lo1 + j;
     gk = w.lo2 + k;
@@ -366,6 +375,18 @@ DMPM_DI void store_s(T* base, long long ld, int plane, int i, T v) {
         typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vo, so, 0);
     }
+}
+
+// Grid velocity update (ForwardSimulation.cpp:82-87) of a node whose mass
+// and momentum are loaded; `live` is false for walls and nodes off the grid.
+// Selects rather than branches, so that the loads stay ahead of any use.
+template <typename T>
+DMPM_DI void velocity_of(const KP<T>& a, bool live, T m, T p0, T p1, T p2, T& v0, T& v1, T& v2) {
+    const T inv = T(1) / m;
+    live = live && m > T(1e-12);
+    v0 = live ? p0 * inv + a.dt * a.f0 : T(0);
+    v1 = live ? p1 * inv + a.dt * a.f1 : T(0);
+    v2 = live ? p2 * inv + a.dt * a.f2 : T(0);
 }
 
 // Grid velocity update (ForwardSimulation.cpp:82-87) plus sticky walls.
@@ -802,18 +823,43 @@ __global__ void __launch_bounds__(256) k_p2g_batch(const KP<T>* tab, const int* 
     p2g_block<T, ORDER, MODEL, MASS_ONLY>(a, lblock, smem_d);
 }
 
+// Window slots staged per thread and pass: a thread takes slots t,
+// t + B, ..., t + (kStage - 1) B (B = blockDim.x) and issues every load of
+// them before it uses any, so a pass costs one memory round trip.  Slots
+// past the window or off the grid load node 0 and are zeroed.
+constexpr int kStage = 4;
+constexpr int kStageAdj = 2;  // the adjoint stages seven values per slot
+
 // Stage the window's grid velocities in LDS (3 arrays of cap).
 template <typename T>
 DMPM_DI void stage_velocity(const KP<T>& a, const Win& win, T* sv) {
-#pragma unroll 2
-    for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
-        int gi, gj, gk;
-        const bool real = window_node(win, t, gi, gj, gk);
-        T v0 = T(0), v1 = T(0), v2 = T(0);
-        if (real && in_grid(a, gi, gj, gk)) node_velocity(a, gindex(a, gi, gj, gk), gi, gj, gk, v0, v1, v2);
-        sv[t] = v0;
-        sv[a.cap + t] = v1;
-        sv[2 * a.cap + t] = v2;
+    const int B = blockDim.x;
+    for (int t0 = threadIdx.x; t0 < win.vol; t0 += kStage * B) {
+        T m[kStage], p0[kStage], p1[kStage], p2[kStage];
+        bool live[kStage];
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+            int gi, gj, gk;
+            const int t = t0 + u * B;
+            const bool node = t < win.vol && window_node(win, t, gi, gj, gk) && in_grid(a, gi, gj, gk);
+            const int g = node ? (int)gindex(a, gi, gj, gk) : 0;
+            m[u] = load_s(a.gm, a.gld, 0, g);  // the momentum planes follow: gp = gm + gld
+            p0[u] = load_s(a.gm, a.gld, 1, g);
+            p1[u] = load_s(a.gm, a.gld, 2, g);
+            p2[u] = load_s(a.gm, a.gld, 3, g);
+            live[u] = node && !is_wall(a, gi, gj, gk);
+        }
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+            const int t = t0 + u * B;
+            T v0, v1, v2;
+            velocity_of(a, live[u], m[u], p0[u], p1[u], p2[u], v0, v1, v2);
+            if (t < win.vol) {
+                sv[t] = v0;
+                sv[a.cap + t] = v1;
+                sv[2 * a.cap + t] = v2;
+            }
+        }
     }
 }
 
@@ -1242,6 +1288,26 @@ __global__ void __launch_bounds__(256) k_adj_scatter_batch(const KP<T>* tab, con
     adj_scatter_block<T, ORDER, REF>(a, lblock, smem_d);
 }
 
+// Grid adjoint (BackPropagation.cpp:59-72) of a node whose mass, momentum
+// q and dL/dv_i g are loaded: h = dL/dp_i, gm = dL/dm_i.  Exact: dm uses
+// v_i - dt f_ext; REF uses v_i (as written).  `live` is false for walls and
+// nodes off the grid; selects rather than branches keep the loads ahead.
+template <typename T, bool REF>
+DMPM_DI void adjoint_of(const KP<T>& a, bool live, T m, const T* q, const T* gv, T* v, T* h, T& gmass) {
+    const T inv = T(1) / m;
+    live = live && m > T(1e-12);
+    const T p0 = q[0] * inv, p1 = q[1] * inv, p2 = q[2] * inv;
+    const T u0 = p0 + a.dt * a.f0, u1 = p1 + a.dt * a.f1, u2 = p2 + a.dt * a.f2;
+    const T vd = REF ? (u0 * gv[0] + u1 * gv[1] + u2 * gv[2]) : (p0 * gv[0] + p1 * gv[1] + p2 * gv[2]);
+    v[0] = live ? u0 : T(0);
+    v[1] = live ? u1 : T(0);
+    v[2] = live ? u2 : T(0);
+    h[0] = live ? gv[0] * inv : T(0);
+    h[1] = live ? gv[1] * inv : T(0);
+    h[2] = live ? gv[2] * inv : T(0);
+    gmass = live ? -inv * vd : T(0);
+}
+
 // Grid adjoint (BackPropagation.cpp:59-72) for one node: h = dL/dp_i,
 // gm = dL/dm_i.  Exact: dm uses v_i - dt f_ext; REF uses v_i (as written).
 template <typename T, bool REF>
@@ -1306,18 +1372,42 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
     S.init(a, x);  // lanes without a particle get a finite stencil at x = 0
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
     if (win.lds) {
-#pragma unroll 2
-        for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
-            int gi, gj, gk;
-            const bool real = window_node(win, t, gi, gj, gk);
-            T v[3] = {T(0), T(0), T(0)}, h[3] = {T(0), T(0), T(0)}, gmass = T(0);
-            if (real && in_grid(a, gi, gj, gk)) node_adjoint<T, REF>(a, gindex(a, gi, gj, gk), gi, gj, gk, v, h, gmass);
+        // kStageAdj slots per thread and pass, all seven loads of each
+        // issued before any is used (slots past the window or off the grid
+        // load node 0 and are zeroed)
+        constexpr int U = kStageAdj;
+        const int B = blockDim.x;
+        for (int t0 = threadIdx.x; t0 < win.vol; t0 += U * B) {
+            T m[U], q[U][3], gv[U][3];
+            bool live[U];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                sv[q * a.cap + t] = v[q];
-                sv[(3 + q) * a.cap + t] = h[q];
+            for (int u = 0; u < U; ++u) {
+                int gi, gj, gk;
+                const int t = t0 + u * B;
+                const bool node = t < win.vol && window_node(win, t, gi, gj, gk) && in_grid(a, gi, gj, gk);
+                const int g = node ? (int)gindex(a, gi, gj, gk) : 0;
+                m[u] = load_s(a.gm, a.gld, 0, g);  // the momentum planes follow: gp = gm + gld
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    q[u][c] = load_s(a.gm, a.gld, 1 + c, g);
+                    gv[u][c] = load_s(a.ggv, a.gld, c, g);
+                }
+                live[u] = node && !is_wall(a, gi, gj, gk);
             }
-            sv[6 * a.cap + t] = gmass;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = t0 + u * B;
+                T v[3], h[3], gmass;
+                adjoint_of<T, REF>(a, live[u], m[u], q[u], gv[u], v, h, gmass);
+                if (t < win.vol) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        sv[c * a.cap + t] = v[c];
+                        sv[(3 + c) * a.cap + t] = h[c];
+                    }
+                    sv[6 * a.cap + t] = gmass;
+                }
+            }
         }
         __syncthreads();
     }
