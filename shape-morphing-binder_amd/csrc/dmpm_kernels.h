@@ -5,9 +5,12 @@
 //   statics      : 4 planes: m, vol, lam, mu
 //   control dFc  : 9 planes
 //   adjoint carry: 24 planes, same order as the state (dL/dx, dL/dv, dL/dF, dL/dC)
-//   grid         : node (i,j,k) at i*ny*nz + j*nz + k (the reference's order,
-//                  Grid.h:89-94); arrays of `gld` elements: m, p0..2 (forward),
-//                  gv0..2 (adjoint dL/dv_i).
+//   grid         : node (i,j,k) at g = i*ny*nz + j*nz + k (the reference's
+//                  order, Grid.h:89-94), its values interleaved at [4g, 4g+4):
+//                  m, p0..2 (forward), dL/dv_i 0..2 and an unused slot
+//                  (adjoint); `gld` >= nx*ny*nz nodes.  A mass-only P2G (the
+//                  end-layer loss, p2g, point-cloud volumes) writes a plane of
+//                  one mass per node.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -36,18 +39,18 @@ struct Launch {
     Phys ph;
     int n;          // particles
     long long ld;   // particle plane stride (elements)
-    long long gld;  // grid plane stride (elements)
+    long long gld;  // grid nodes allocated (>= nx*ny*nz, a multiple of 64)
     const void* st;    // layer state (24 planes)
     const void* stat;  // statics (4 planes)
     const void* ctl;   // dFc (9 planes) or null
     void* out;         // next layer state (24 planes)
-    void* grid;        // m, p0, p1, p2 (4 grid planes)
-    void* ggv;         // dL/dv_i (3 grid planes)
+    void* grid;        // m, p0, p1, p2 per node (4 values each); mass-only P2G: m per node
+    void* ggv;         // dL/dv_i per node (4 values each, the last unused)
     const void* adj1;  // adjoint carry of layer t+1
     void* adj0;        // adjoint carry of layer t (output)
     void* gctl;        // dL/d dFc of layer t (output, 9 planes) or null
     const void* ctl1;  // fused step: dFc of layer t+1 or null
-    void* grid1;       // fused step: grid t+1 (4 planes, zeroed by the caller)
+    void* grid1;       // fused step: grid t+1 (as grid, zeroed by the caller)
     // adjoint: state of layer t+1 when it is known to be the forward step of
     // `st` under the current control and parameters (its v and C are then
     // read instead of recomputed by a G2P from `grid`), else null

@@ -71,15 +71,13 @@ struct KP {
     const T* stat;
     const T* ctl;
     T* out;
-    T* gm;
-    T* gp;   // 3 planes after gm: gp + a*gld
-    T* ggv;  // 3 planes
+    T* gm;   // grid: 4 values per node (m, p0, p1, p2); a mass-only P2G's grid: 1 (m)
+    T* ggv;  // adjoint grid dL/dv_i: 4 values per node (the last unused)
     const T* adj1;
     T* adj0;
     T* gctl;
     const T* ctl1;  // control of layer t+1 (fused step)
-    T* gm1;         // grid t+1 (fused step): m, then 3 p planes
-    T* gp1;
+    T* gm1;         // grid t+1 (fused step), laid out as gm
     const T* nxt;   // adjoint: layer t+1 state (v, C read instead of a G2P) or null
     const int* bst; // particle block table (nblk + 1 starts) or null
     int nblk;
@@ -132,14 +130,12 @@ static KP<T> make_kp(const Launch& L, int narrays) {
     k.ctl = (const T*)L.ctl;
     k.out = (T*)L.out;
     k.gm = (T*)L.grid;
-    k.gp = L.grid ? (T*)L.grid + L.gld : nullptr;
     k.ggv = (T*)L.ggv;
     k.adj1 = (const T*)L.adj1;
     k.adj0 = (T*)L.adj0;
     k.gctl = (T*)L.gctl;
     k.ctl1 = (const T*)L.ctl1;
     k.gm1 = (T*)L.grid1;
-    k.gp1 = L.grid1 ? (T*)L.grid1 + L.gld : nullptr;
     k.nxt = (const T*)L.nxt;
     k.bst = L.bst;
     k.nblk = L.nblk;
@@ -377,6 +373,42 @@ DMPM_DI void store_s(T* base, long long ld, int plane, int i, T v) {
     }
 }
 
+// Grid arrays hold kNode values per node, node g at [kNode g, kNode g +
+// kNode): m, p0, p1, p2 in the forward grid, dL/dv_i 0..2 and an unused slot
+// in the adjoint grid.  The L2 carries out float atomics per 64-byte request,
+// so the window flushes, whose lanes walk a window row node by node and value
+// by value, cover whole lines with one request; and a node is one 16-byte
+// load.  A mass-only P2G writes one value per node.
+constexpr int kNode = 4;
+
+// The node's values as kNode scalar buffer loads, which the compiler merges
+// into one 16-byte (FP32) or two (FP64) loads.  (Element extraction from the
+// vector the 64- and 128-bit buffer-load builtins return is narrowed to a
+// single dword load by this compiler's optimiser, and is not used.)
+template <typename T>
+DMPM_DI void load_node(const T* base, int g, T* v) {
+    const __amdgpu_buffer_rsrc_t r = plane_rsrc(base);
+#pragma unroll
+    for (int q = 0; q < kNode; ++q) {
+        if constexpr (sizeof(T) == 4) {
+            v[q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, g * 16 + 4 * q, 0, 0));
+        } else {
+            v[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, g * 32 + 8 * q, 0, 0));
+        }
+    }
+}
+
+// Grid element e += v, a device-scope float atomic (buffer form for FP32;
+// FP64 keeps the global form, which has no buffer counterpart).
+template <typename T>
+DMPM_DI void atomic_add_at(T* base, int e, T v) {
+    if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_ptr_buffer_atomic_fadd_f32(v, plane_rsrc(base), e * 4, 0, 0);
+    } else {
+        unsafeAtomicAdd(base + e, v);
+    }
+}
+
 // Grid velocity update (ForwardSimulation.cpp:82-87) of a node whose mass
 // and momentum are loaded; `live` is false for walls and nodes off the grid.
 // Selects rather than branches, so that the loads stay ahead of any use.
@@ -392,30 +424,9 @@ DMPM_DI void velocity_of(const KP<T>& a, bool live, T m, T p0, T p1, T p2, T& v0
 // Grid velocity update (ForwardSimulation.cpp:82-87) plus sticky walls.
 template <typename T>
 DMPM_DI void node_velocity(const KP<T>& a, long long g, int i, int j, int k, T& v0, T& v1, T& v2) {
-    // all four loads issued before the branch (one memory round trip); the
-    // momentum planes follow the mass plane (gp = gm + gld)
-    const T m = load_s(a.gm, a.gld, 0, (int)g);
-    const T p0 = load_s(a.gm, a.gld, 1, (int)g), p1 = load_s(a.gm, a.gld, 2, (int)g);
-    const T p2 = load_s(a.gm, a.gld, 3, (int)g);
-    if (m > T(1e-12) && !is_wall(a, i, j, k)) {
-        const T inv = T(1) / m;
-        v0 = p0 * inv + a.dt * a.f0;
-        v1 = p1 * inv + a.dt * a.f1;
-        v2 = p2 * inv + a.dt * a.f2;
-    } else {
-        v0 = v1 = v2 = T(0);
-    }
-}
-
-// Grid node g of plane `plane` += v, a device-scope float atomic (buffer form
-// for FP32; FP64 keeps the global form, which has no buffer counterpart).
-template <typename T>
-DMPM_DI void atomic_add_s(T* base, long long ld, int plane, long long g, T v) {
-    if constexpr (sizeof(T) == 4) {
-        __builtin_amdgcn_raw_ptr_buffer_atomic_fadd_f32(v, plane_rsrc(base), (int)g * 4, (int)(plane * ld * 4), 0);
-    } else {
-        unsafeAtomicAdd(base + plane * ld + g, v);
-    }
+    T n[kNode];
+    load_node(a.gm, (int)g, n);
+    velocity_of(a, !is_wall(a, i, j, k), n[0], n[1], n[2], n[3], v0, v1, v2);
 }
 
 template <typename T>
@@ -672,7 +683,7 @@ DMPM_DI int window_base(const Win& w, const int* b) {
 
 // Scatter NV values per stencil node: into the block's LDS window (NV Acc
 // arrays of a.cap) or, when the window did not fit, straight into the grid
-// planes dst[q] with global atomics.  val(ia, ib, ic, c) writes the lane's
+// dst (kNode values per node; 1 when NV = 1) with global atomics.  val(ia, ib, ic, c) writes the lane's
 // contributions to node b + (ia, ib, ic); lanes without a particle must
 // produce finite values (they never issue).  Run sums merge the lanes that
 // share a base cell first.  Nodes outside the grid go to window slots that
@@ -715,9 +726,9 @@ DMPM_DI void scatter_nodes(const KP<T>& a, const Win& win, Acc* sm, const Sten<T
                     seg_sums<T, NV>(c, run);
                     const int gi = S.b[0] + ia, gj = S.b[1] + ib, gk = S.b[2] + ic;
                     if (!issue || !in_grid(a, gi, gj, gk)) continue;
-                    const long long g = gindex(a, gi, gj, gk);
+                    const int e = (int)gindex(a, gi, gj, gk) * (NV == 1 ? 1 : kNode);
 #pragma unroll
-                    for (int q = 0; q < NV; ++q) atomic_add_s(dst, a.gld, q, g, c[q]);
+                    for (int q = 0; q < NV; ++q) atomic_add_at(dst, e + q, c[q]);
                 }
             }
         }
@@ -729,8 +740,7 @@ DMPM_DI void scatter_nodes(const KP<T>& a, const Win& win, Acc* sm, const Sten<T
 // A = 0, mv = 0.
 template <typename T, int ORDER, bool MASS_ONLY>
 DMPM_DI void p2g_scatter(const KP<T>& a, const Win& win, Acc* sm, const Sten<T, ORDER>& S, bool valid, T m,
-                         const m3<T>& A, const v3<T>& mv, T* gm, T* gp) {
-    T* const dst = gm;  // planes m, p0, p1, p2 (gp = gm + gld)
+                         const m3<T>& A, const v3<T>& mv, T* dst) {
     scatter_nodes<T, ORDER, MASS_ONLY ? 1 : 4>(a, win, sm, S, valid, dst, [&](int ia, int ib, int ic, T* c) {
         const T d0 = S.u[0][ia] * a.dx, d1 = S.u[1][ib] * a.dx, d2 = S.u[2][ic] * a.dx;
         const T wt = S.w[0][ia] * S.w[1][ib] * S.w[2][ic];
@@ -750,25 +760,20 @@ DMPM_DI void window_zero(const KP<T>& a, const Win& win, U* sm) {
         for (int q = 0; q < NA; ++q) sm[q * a.cap + t] = U(0);
 }
 
-// Flush the window to the grid: one atomic per touched node value.
-template <typename T, bool MASS_ONLY>
-DMPM_DI void p2g_flush(const KP<T>& a, const Win& win, const Acc* sm, T* gm, T* gp) {
-    for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
+// Flush a scatter window (NV Acc arrays of a.cap) to the grid dst (kNode
+// values per node; 1 when NV = 1): one atomic per nonzero node value.  Lane u
+// takes value u % NV of window slot u / NV, so that the lanes of an atomic
+// instruction cover a run of whole nodes, consecutive in memory along a
+// window row.
+template <typename T, int NV>
+DMPM_DI void flush_window(const KP<T>& a, const Win& win, const Acc* sm, T* dst) {
+    constexpr int NS = NV == 1 ? 1 : kNode;
+    for (int u = threadIdx.x; u < NV * win.vol; u += blockDim.x) {
+        const int t = u / NV, q = u - t * NV;
         int gi, gj, gk;
         if (!window_node(win, t, gi, gj, gk) || !in_grid(a, gi, gj, gk)) continue;
-        const T vm = (T)sm[t];
-        const long long g = gindex(a, gi, gj, gk);
-        if (MASS_ONLY) {
-            if (vm != T(0)) atomic_add_s(gm, a.gld, 0, g, vm);
-        } else {
-            const T p0 = (T)sm[a.cap + t], p1 = (T)sm[2 * a.cap + t], p2 = (T)sm[3 * a.cap + t];
-            if (vm != T(0) || p0 != T(0) || p1 != T(0) || p2 != T(0)) {
-                atomic_add_s(gm, a.gld, 0, g, vm);  // the momentum planes follow: gp = gm + gld
-                atomic_add_s(gm, a.gld, 1, g, p0);
-                atomic_add_s(gm, a.gld, 2, g, p1);
-                atomic_add_s(gm, a.gld, 3, g, p2);
-            }
-        }
+        const T v = (T)sm[q * a.cap + t];
+        if (v != T(0)) atomic_add_at(dst, (int)gindex(a, gi, gj, gk) * NS + q, v);
     }
 }
 
@@ -801,10 +806,10 @@ DMPM_DI void p2g_block(const KP<T>& a, int lblock, double* smem_d) {
         }
     }
     __syncthreads();
-    p2g_scatter<T, ORDER, MASS_ONLY>(a, win, sm, S, valid, m, A, mv, a.gm, a.gp);
+    p2g_scatter<T, ORDER, MASS_ONLY>(a, win, sm, S, valid, m, A, mv, a.gm);
     if (win.lds && win.any) {
         __syncthreads();
-        p2g_flush<T, MASS_ONLY>(a, win, sm, a.gm, a.gp);
+        flush_window<T, NA>(a, win, sm, a.gm);
     }
 }
 
@@ -835,25 +840,21 @@ template <typename T>
 DMPM_DI void stage_velocity(const KP<T>& a, const Win& win, T* sv) {
     const int B = blockDim.x;
     for (int t0 = threadIdx.x; t0 < win.vol; t0 += kStage * B) {
-        T m[kStage], p0[kStage], p1[kStage], p2[kStage];
+        T n[kStage][kNode];
         bool live[kStage];
 #pragma unroll
         for (int u = 0; u < kStage; ++u) {
             int gi, gj, gk;
             const int t = t0 + u * B;
             const bool node = t < win.vol && window_node(win, t, gi, gj, gk) && in_grid(a, gi, gj, gk);
-            const int g = node ? (int)gindex(a, gi, gj, gk) : 0;
-            m[u] = load_s(a.gm, a.gld, 0, g);  // the momentum planes follow: gp = gm + gld
-            p0[u] = load_s(a.gm, a.gld, 1, g);
-            p1[u] = load_s(a.gm, a.gld, 2, g);
-            p2[u] = load_s(a.gm, a.gld, 3, g);
+            load_node(a.gm, node ? (int)gindex(a, gi, gj, gk) : 0, n[u]);
             live[u] = node && !is_wall(a, gi, gj, gk);
         }
 #pragma unroll
         for (int u = 0; u < kStage; ++u) {
             const int t = t0 + u * B;
             T v0, v1, v2;
-            velocity_of(a, live[u], m[u], p0[u], p1[u], p2[u], v0, v1, v2);
+            velocity_of(a, live[u], n[u][0], n[u][1], n[u][2], n[u][3], v0, v1, v2);
             if (t < win.vol) {
                 sv[t] = v0;
                 sv[a.cap + t] = v1;
@@ -1010,9 +1011,9 @@ __global__ void __launch_bounds__(256) k_g2p_batch(const KP<T>* tab, const int* 
     g2p_block<T, ORDER>(a, lblock, smem_d);
 }
 
-// One fused forward step: G2P of step t from grid t (a.gm/a.gp), the
+// One fused forward step: G2P of step t from grid t (a.gm), the
 // particle update writing layer t+1, then the stress of layer t+1 (with its
-// control a.ctl1) and its P2G into grid t+1 (a.gm1/a.gp1, zeroed by the
+// control a.ctl1) and its P2G into grid t+1 (a.gm1, zeroed by the
 // caller).  Layer t+1 is never re-read from HBM for its own P2G.
 template <typename T, int ORDER, int MODEL>
 DMPM_DI void step_block(const KP<T>& a, int lblock, double* smem_d) {
@@ -1066,10 +1067,10 @@ DMPM_DI void step_block(const KP<T>& a, int lblock, double* smem_d) {
         p2g_terms<T, ORDER, MODEL>(a, vol, lam, mu, vn, Ft1, Cn, m, A, mv);
     }
     __syncthreads();
-    p2g_scatter<T, ORDER, false>(a, win1, sacc, S1, valid, m, A, mv, a.gm1, a.gp1);
+    p2g_scatter<T, ORDER, false>(a, win1, sacc, S1, valid, m, A, mv, a.gm1);
     if (win1.lds && win1.any) {
         __syncthreads();
-        p2g_flush<T, false>(a, win1, sacc, a.gm1, a.gp1);
+        flush_window<T, 4>(a, win1, sacc, a.gm1);
     }
 }
 
@@ -1259,17 +1260,7 @@ DMPM_DI void adj_scatter_core(const KP<T>& a, bool valid, const v3<T>& x, const 
     });
     if (win.lds && win.any) {
         __syncthreads();
-        for (int t = threadIdx.x; t < win.vol; t += blockDim.x) {
-            int gi, gj, gk;
-            if (!window_node(win, t, gi, gj, gk) || !in_grid(a, gi, gj, gk)) continue;
-            const T g0 = (T)sg[t], g1 = (T)sg[a.cap + t], g2 = (T)sg[2 * a.cap + t];
-            if (g0 != T(0) || g1 != T(0) || g2 != T(0)) {
-                const long long g = gindex(a, gi, gj, gk);
-                atomic_add_s(a.ggv, a.gld, 0, g, g0);
-                atomic_add_s(a.ggv, a.gld, 1, g, g1);
-                atomic_add_s(a.ggv, a.gld, 2, g, g2);
-            }
-        }
+        flush_window<T, 3>(a, win, sg, a.ggv);
     }
 }
 
@@ -1312,28 +1303,10 @@ DMPM_DI void adjoint_of(const KP<T>& a, bool live, T m, const T* q, const T* gv,
 // gm = dL/dm_i.  Exact: dm uses v_i - dt f_ext; REF uses v_i (as written).
 template <typename T, bool REF>
 DMPM_DI void node_adjoint(const KP<T>& a, long long g, int i, int j, int k, T* v, T* h, T& gmass) {
-    // all seven loads issued before the branch (one memory round trip)
-    const T m = load_s(a.gm, a.gld, 0, (int)g);
-    const T q0 = load_s(a.gm, a.gld, 1, (int)g), q1 = load_s(a.gm, a.gld, 2, (int)g);
-    const T q2 = load_s(a.gm, a.gld, 3, (int)g);
-    const T g0 = load_s(a.ggv, a.gld, 0, (int)g), g1 = load_s(a.ggv, a.gld, 1, (int)g);
-    const T g2 = load_s(a.ggv, a.gld, 2, (int)g);
-    if (m > T(1e-12) && !is_wall(a, i, j, k)) {
-        const T inv = T(1) / m;
-        const T p0 = q0 * inv, p1 = q1 * inv, p2 = q2 * inv;
-        v[0] = p0 + a.dt * a.f0;
-        v[1] = p1 + a.dt * a.f1;
-        v[2] = p2 + a.dt * a.f2;
-        h[0] = g0 * inv;
-        h[1] = g1 * inv;
-        h[2] = g2 * inv;
-        const T vd = REF ? (v[0] * g0 + v[1] * g1 + v[2] * g2) : (p0 * g0 + p1 * g1 + p2 * g2);
-        gmass = -inv * vd;
-    } else {
-        v[0] = v[1] = v[2] = T(0);
-        h[0] = h[1] = h[2] = T(0);
-        gmass = T(0);
-    }
+    T n[kNode], gv[kNode];
+    load_node(a.gm, (int)g, n);
+    load_node(a.ggv, (int)g, gv);
+    adjoint_of<T, REF>(a, !is_wall(a, i, j, k), n[0], n + 1, gv, v, h, gmass);
 }
 
 // Adjoint phases 3 + 4 and the stress VJP (BackPropagation.cpp:59-143).
@@ -1378,7 +1351,7 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
         constexpr int U = kStageAdj;
         const int B = blockDim.x;
         for (int t0 = threadIdx.x; t0 < win.vol; t0 += U * B) {
-            T m[U], q[U][3], gv[U][3];
+            T n[U][kNode], gv[U][kNode];
             bool live[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -1386,19 +1359,15 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
                 const int t = t0 + u * B;
                 const bool node = t < win.vol && window_node(win, t, gi, gj, gk) && in_grid(a, gi, gj, gk);
                 const int g = node ? (int)gindex(a, gi, gj, gk) : 0;
-                m[u] = load_s(a.gm, a.gld, 0, g);  // the momentum planes follow: gp = gm + gld
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    q[u][c] = load_s(a.gm, a.gld, 1 + c, g);
-                    gv[u][c] = load_s(a.ggv, a.gld, c, g);
-                }
+                load_node(a.gm, g, n[u]);
+                load_node(a.ggv, g, gv[u]);
                 live[u] = node && !is_wall(a, gi, gj, gk);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int t = t0 + u * B;
                 T v[3], h[3], gmass;
-                adjoint_of<T, REF>(a, live[u], m[u], q[u], gv[u], v, h, gmass);
+                adjoint_of<T, REF>(a, live[u], n[u][0], n[u] + 1, gv[u], v, h, gmass);
                 if (t < win.vol) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
@@ -1811,11 +1780,14 @@ __global__ void __launch_bounds__(256) k_det_gather_p2g(KP<T> a, const int* csta
             p2 += w * (pay[3 * pld + q] + A[6 * pld] * d0 + A[7 * pld] * d1 + A[8 * pld] * d2);
         }
     });
-    a.gm[g] = m;
-    if (!MASS_ONLY) {
-        a.gp[g] = p0;
-        a.gp[a.gld + g] = p1;
-        a.gp[2 * a.gld + g] = p2;
+    if (MASS_ONLY) {
+        a.gm[g] = m;  // a mass-only grid: one value per node
+    } else {
+        T* n = a.gm + kNode * g;
+        n[0] = m;
+        n[1] = p0;
+        n[2] = p1;
+        n[3] = p2;
     }
 }
 
@@ -1860,9 +1832,11 @@ __global__ void __launch_bounds__(256) k_det_gather_adj(KP<T> a, const int* csta
         g1 += w * (pay[pld + q] + Gc[3 * pld] * d0 + Gc[4 * pld] * d1 + Gc[5 * pld] * d2);
         g2 += w * (pay[2 * pld + q] + Gc[6 * pld] * d0 + Gc[7 * pld] * d1 + Gc[8 * pld] * d2);
     });
-    a.ggv[g] = g0;
-    a.ggv[a.gld + g] = g1;
-    a.ggv[2 * a.gld + g] = g2;
+    T* n = a.ggv + kNode * g;
+    n[0] = g0;
+    n[1] = g1;
+    n[2] = g2;
+    n[3] = T(0);
 }
 
 // ---------------------------------------------------------------------------

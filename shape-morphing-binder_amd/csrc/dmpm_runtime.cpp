@@ -123,7 +123,7 @@ struct dmpm_sim {
         }
     }
     void* scratch[2] = {nullptr, nullptr};
-    void* grid = nullptr;  // m, p (4 planes)
+    void* grid = nullptr;  // m, p0, p1, p2 per node (gld nodes)
     void* grid2 = nullptr; // second grid for the fused forward step
     unsigned long long* wstat = nullptr;  // scatter window counters (device, 2 entries)
     void* det = nullptr;   // deterministic-mode scratch
@@ -134,16 +134,17 @@ struct dmpm_sim {
         }
         return det;
     }
-    void* ggv = nullptr;   // 3 planes
+    void* ggv = nullptr;   // dL/dv_i per node (4 values, the last unused)
     bool count_windows = false;  // window counters (wstat) updated by the scatter launches
-    void* ggv2 = nullptr;  // 3 planes: dL/dv of the previous step, written by the fused adjoint
-    void* ggv3 = nullptr;  // 3 planes: the dL/dv grid a fused adjoint launch clears for the next one
+    void* ggv2 = nullptr;  // as ggv: dL/dv of the previous step, written by the fused adjoint
+    void* ggv3 = nullptr;  // as ggv: the dL/dv grid a fused adjoint launch clears for the next one
     void* carry[2] = {nullptr, nullptr};
     int carry_cur = 0;
     int carry_layer = -1;   // layer whose adjoint the current carry holds
     void* adj_last = nullptr;  // adjoint of the last layer (loss / render / seed)
     void* gctl_stop = nullptr; // dL/d dFc of the control layer of the last backward
     void* target_m = nullptr;
+    void* mgrid = nullptr;  // node masses of the end-layer loss (one per node)
     void* dldm = nullptr;
     double* partial = nullptr;  // reduction partials
     double* red = nullptr;      // reduction outputs (8 doubles)
@@ -640,7 +641,7 @@ struct dmpm_sim {
                 if (rec) throw Error(DMPM_ERR_STATE, "deterministic mode cannot be batched");
                 timed(2, [&] { det_adj_scatter(pr, Lc, det_buf(), stream); });
             } else if (!scattered) {
-                op_zero(gcur, 3 * gld);
+                op_zero(gcur, 4 * gld);
                 op_kernel(OP_ADJ_SCATTER, Lc);
             }
             // Fuse this step's gather with the scatter of step t-1 when layer
@@ -657,12 +658,12 @@ struct dmpm_sim {
                 Lp.ctl = ctl[t - 1];
                 Lp.ggv = gnext;
                 Lp.nxt = st;
-                if (!gnext_clear) op_zero(gnext, 3 * gld);
+                if (!gnext_clear) op_zero(gnext, 4 * gld);
                 // the next step fuses only if it has a previous step
                 gnext_clear = t - 2 >= stop;
                 if (gnext_clear) {
                     Lc.zero = gfree;
-                    Lc.zero_bytes = 3 * gld * (long long)esz;
+                    Lc.zero_bytes = 4 * gld * (long long)esz;
                 }
                 op_adj_fused(Lc, Lp);
                 void* read = gcur;
@@ -779,17 +780,17 @@ double end_layer_mass_loss(dmpm_sim* s) {
     Launch Lc = s->launch();
     Lc.st = st;
     Lc.stat = s->stat;
-    Lc.grid = s->grid;
+    Lc.grid = s->mgrid;
     if (s->cfg.deterministic) {
         det_p2g(s->pr, Lc, true, s->det_buf(), s->stream);
     } else {
-        s->zero(s->grid, s->gld);
+        s->zero(s->mgrid, s->gld);
         p2g(s->pr, Lc, true, s->stream);
     }
     const int nb = red_blocks_for(s->G);
-    mass_sums(s->pr, s->grid, s->target_m, s->G, s->partial, nb, s->stream);
+    mass_sums(s->pr, s->mgrid, s->target_m, s->G, s->partial, nb, s->stream);
     reduce_partials(s->partial, nb, 2, s->red, s->stream);
-    mass_loss_nodes(s->pr, s->grid, s->target_m, s->G, s->red, nb, s->dldm, s->partial + 2 * 1024, nb, s->stream);
+    mass_loss_nodes(s->pr, s->mgrid, s->target_m, s->G, s->red, nb, s->dldm, s->partial + 2 * 1024, nb, s->stream);
     reduce_partials(s->partial + 2 * 1024, nb, 1, s->red + 2, s->stream);
     Lc.adj0 = s->adj_last;
     mass_loss_particles(s->pr, Lc, s->dldm, s->target_m, s->stream);
@@ -988,13 +989,14 @@ int dmpm_create(const dmpm_config* cfg, dmpm_sim** out) {
         s->gld = round_up(s->G, 64);
         // the kernels address a grid with 32-bit byte offsets
         if ((unsigned long long)4 * s->gld * s->esz >= (1ull << 32))
-            throw Error(DMPM_ERR_ARG, "grid too large: its four planes must stay below 4 GiB");
+            throw Error(DMPM_ERR_ARG, "grid too large: its four values per node must stay below 4 GiB");
         s->grid = s->dalloc(4 * s->gld * s->esz);
         s->grid2 = s->dalloc(4 * s->gld * s->esz);
-        s->ggv = s->dalloc(3 * s->gld * s->esz);
-        s->ggv2 = s->dalloc(3 * s->gld * s->esz);
-        s->ggv3 = s->dalloc(3 * s->gld * s->esz);
+        s->ggv = s->dalloc(4 * s->gld * s->esz);
+        s->ggv2 = s->dalloc(4 * s->gld * s->esz);
+        s->ggv3 = s->dalloc(4 * s->gld * s->esz);
         s->target_m = s->dalloc(s->gld * s->esz);
+        s->mgrid = s->dalloc(s->gld * s->esz);
         s->dldm = s->dalloc(s->gld * s->esz);
         s->zero(s->target_m, s->gld);
         s->partial = (double*)s->dalloc(4 * 1024 * sizeof(double));
