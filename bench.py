@@ -57,7 +57,9 @@ ALGO_BYTES = {"step": (200, 56), "adj_fused": (332, 72)}
 
 # The kernels' own byte models, FP32 (4-byte elements): every word each kernel
 # reads or writes once, as built (the adjoint kernels read G2P's v' and C' of
-# step t from layer t+1 instead of recomputing them).
+# step t from layer t+1 instead of recomputing them).  Grid nodes hold four
+# values each: m, p (16 B) in the forward grid, dL/dv and an unused slot
+# (16 B) in the adjoint grid.
 #   p2g         reads x, v, F, C, m, vol, lam, mu (28 floats); writes m, p per node
 #   g2p         reads x, F (12 floats) and m, p per node; writes x, v, F, C (24 floats)
 #   adj_scatter reads x, F (12), the adjoint carry (24) and C of layer t+1 (9); writes dL/dv per node
@@ -72,9 +74,9 @@ BYTES = {
     "step": (40 * 4, 32),
     "p2g": (28 * 4, 16),
     "g2p": (36 * 4, 16),
-    "adj_scatter": (45 * 4, 12),
-    "adj_gather": (88 * 4, 28),
-    "adj_fused": (100 * 4, 40),
+    "adj_scatter": (45 * 4, 16),
+    "adj_gather": (88 * 4, 32),
+    "adj_fused": (100 * 4, 48),
 }
 
 

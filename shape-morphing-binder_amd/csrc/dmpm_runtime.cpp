@@ -19,6 +19,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/dmpm.h"
@@ -80,9 +81,15 @@ struct dmpm_sim {
     int L = 1;
     int K = 1;
 
-    // device memory bookkeeping
-    size_t mem_cur = 0, mem_peak = 0;
+    // device memory bookkeeping: allocations in use, and freed ones kept for
+    // reuse (size -> blocks).  hipMalloc / hipFree of the hundreds of layer
+    // buffers a set-up allocates cost milliseconds each (hipFree also waits
+    // for the device); a sim that is set up again with the same particle
+    // count -- every call of a differentiable rollout -- takes them from the
+    // pool instead.  mem_cur counts both (device memory the sim holds).
+    size_t mem_cur = 0, mem_peak = 0, pooled = 0;
     std::vector<std::pair<void*, size_t>> allocs;
+    std::multimap<size_t, void*> pool;
 
     int* perm = nullptr;   // internal index -> caller index
     int* bst = nullptr;    // particle block table of the window kernels (nblk + 1 starts)
@@ -222,6 +229,7 @@ struct dmpm_sim {
         *alive = false;
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto& a : allocs) (void)hipFree(a.first);
+        for (auto& a : pool) (void)hipFree(a.second);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : kpool) (void)hipEventDestroy(e);
@@ -229,30 +237,52 @@ struct dmpm_sim {
     }
 
     // ---- memory ----
+    // A freed block is only handed out again for work queued on the sim's
+    // stream after the free, which runs after every earlier use of it.
     void* dalloc(size_t bytes) {
         if (bytes == 0) bytes = 16;
         void* p = nullptr;
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            throw Error(DMPM_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
-                                            hipGetErrorString(e));
+        auto it = pool.find(bytes);
+        if (it != pool.end()) {
+            p = it->second;
+            pool.erase(it);
+            pooled -= bytes;
+        } else {
+            hipError_t e = hipMalloc(&p, bytes);
+            if (e != hipSuccess && !pool.empty()) {
+                (void)hipGetLastError();
+                release_pool();
+                e = hipMalloc(&p, bytes);
+            }
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                throw Error(DMPM_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                                hipGetErrorString(e));
+            }
+            mem_cur += bytes;
+            mem_peak = std::max(mem_peak, mem_cur);
         }
         allocs.emplace_back(p, bytes);
-        mem_cur += bytes;
-        mem_peak = std::max(mem_peak, mem_cur);
         return p;
     }
     void dfree(void*& p) {
         if (!p) return;
-        for (size_t i = 0; i < allocs.size(); ++i)
+        for (size_t i = allocs.size(); i-- > 0;)
             if (allocs[i].first == p) {
-                mem_cur -= allocs[i].second;
+                pool.emplace(allocs[i].second, p);
+                pooled += allocs[i].second;
                 allocs.erase(allocs.begin() + i);
                 break;
             }
-        (void)hipFree(p);
         p = nullptr;
+    }
+    // Give the pooled blocks back to the device.
+    void release_pool() {
+        if (pool.empty()) return;
+        for (auto& a : pool) (void)hipFree(a.second);
+        mem_cur -= pooled;
+        pooled = 0;
+        pool.clear();
     }
     size_t state_bytes() const { return (size_t)NSTATE * ld * esz; }
     size_t ctl_bytes() const { return (size_t)9 * ld * esz; }
@@ -1113,6 +1143,7 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
         s->dfree(s->det);
         s->has_rg = false;
 
+        if (round_up(n, 64) != s->ld) s->release_pool();  // blocks of the old size fit nothing now
         s->n = n;
         s->ld = round_up(n, 64);
         s->L = 1;
