@@ -1189,13 +1189,15 @@ DMPM_DI void adj_scatter_core(const KP<T>& a, bool valid, const v3<T>& x, const 
 // (BackPropagation.cpp:36-55), accumulated in an LDS window.
 template <typename T, int ORDER, bool REF>
 DMPM_DI void adj_scatter_block(const KP<T>& a, int lblock, double* smem_d) {
-    T* sv = reinterpret_cast<T*>(smem_d);  // 3 velocity arrays, then 3 accumulators
-    Acc* sg = reinterpret_cast<Acc*>(smem_d + (3 * a.cap * sizeof(T) + 7) / 8);
+    // 3 velocity arrays, then 3 accumulators; reading layer t+1 (no G2P
+    // recompute, so no velocities staged), the accumulators alone
+    const bool nxt = reads_next<T, REF>(a);
+    T* sv = reinterpret_cast<T*>(smem_d);
+    Acc* sg = reinterpret_cast<Acc*>(smem_d + (nxt ? 0 : (3 * a.cap * sizeof(T) + 7) / 8));
     int i0, i1;
     block_range(a, lblock, i0, i1);
     const int i = i0 + (int)threadIdx.x;
     const bool valid = i < i1;
-    const bool nxt = reads_next<T, REF>(a);
     // Every per-particle input is loaded up front, so that the memory latency
     // overlaps the window reduction and the clearing of the accumulators
     // (loads are not moved across the barriers by the compiler).
@@ -2258,10 +2260,14 @@ template void batch_fwd<double>(int, const Phys&, int, size_t, unsigned, const v
 #if DMPM_IN(3) || DMPM_IN(4)
 template <typename T>
 static void launch_adj_scatter(const Launch& L, hipStream_t s) {
-    KP<T> k = make_kp<T>(L, 3 * (int)sizeof(T) + 3 * (int)sizeof(Acc) + 8);
-    const unsigned nb = launch_blocks(L);
-    const size_t sh = (size_t)k.cap * (3 * sizeof(T) + 3 * sizeof(Acc)) + 8;
     const bool ref = L.ph.adjoint == 1;
+    // the kernel reads layer t+1 (reads_next): its window holds the 3
+    // accumulators only, so it may span more nodes in the same LDS
+    const bool nxt = L.nxt != nullptr && (ref || !L.ph.trust_region);
+    const int per_node = nxt ? 3 * (int)sizeof(Acc) : 3 * (int)sizeof(T) + 3 * (int)sizeof(Acc) + 8;
+    KP<T> k = make_kp<T>(L, per_node);
+    const unsigned nb = launch_blocks(L);
+    const size_t sh = nxt ? (size_t)k.cap * 3 * sizeof(Acc) : (size_t)k.cap * (3 * sizeof(T) + 3 * sizeof(Acc)) + 8;
 #define AS_CASE(O, R) hipLaunchKernelGGL((k_adj_scatter<T, O, R>), dim3(nb), dim3(L.block), sh, s, k)
     if (L.ph.order == 3) { if (ref) AS_CASE(3, true); else AS_CASE(3, false); }
     else { if (ref) AS_CASE(2, true); else AS_CASE(2, false); }
