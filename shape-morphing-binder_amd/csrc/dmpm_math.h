@@ -348,29 +348,49 @@ template <> struct Model<1> {
 };
 
 template <> struct Model<0> {
+    // The factors of F both the stress and its Hessian action use: the polar
+    // decomposition F = R S, det F (raw and clamped to 1e-6) and F^{-T}, in
+    // FP64.  The adjoint's stress gradient needs both, and forms them once.
+    template <typename T>
+    struct Parts {
+        m3<T> R, S;
+        m3<double> Fit;
+        double Jr, J;
+    };
+    template <typename T>
+    static DMPM_DI Parts<T> parts(const m3<T>& F) {
+        Parts<T> p;
+        polar3(F, p.R, p.S);
+        const m3<double> Fd = mcast<double>(F);
+        p.Jr = mdet(Fd);
+        p.J = fmax(p.Jr, 1e-6);
+        p.Fit = mscale(1.0 / p.Jr, mcof(Fd));
+        return p;
+    }
     // Kirchhoff stress P F^T, formed from P as the reference does.
     template <typename T>
     static DMPM_DI m3<T> kirchhoff(const m3<T>& F, T lam, T mu) {
         return mmult(pk1(F, lam, mu), F);
     }
-    // Gradient of <G, P(F) F^T> in F: G^T P + dP(F; G F).
+    // Gradient of <G, P(F) F^T> in F: G^T P + dP(F; G F), with one polar
+    // decomposition for both terms.
     template <typename T>
     static DMPM_DI m3<T> kirchhoff_vjp(const m3<T>& F, T lam, T mu, const m3<T>& G) {
-        return madd(mtmul(G, pk1(F, lam, mu)), hvp(F, lam, mu, mmul(G, F)));
+        const Parts<T> p = parts(F);
+        return madd(mtmul(G, pk1(p, F, lam, mu)), hvp(p, lam, mu, mmul(G, F)));
     }
     // P = 2 mu (F - R) + lam (J-1) J F^{-T}, J = max(det F, 1e-6)
     template <typename T>
     static DMPM_DI m3<T> pk1(const m3<T>& F, T lam, T mu) {
-        m3<T> R, S;
-        polar3(F, R, S);
-        m3<double> Fd = mcast<double>(F);
-        double Jr = mdet(Fd);
-        double J = fmax(Jr, 1e-6);
-        m3<double> Fit = mscale(1.0 / Jr, mcof(Fd));
-        double k = (double)lam * (J - 1.0) * J;
+        return pk1(parts(F), F, lam, mu);
+    }
+    template <typename T>
+    static DMPM_DI m3<T> pk1(const Parts<T>& p, const m3<T>& F, T lam, T mu) {
+        double k = (double)lam * (p.J - 1.0) * p.J;
         m3<T> P;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) P.a[i] = (T)(2.0 * (double)mu * ((double)F.a[i] - (double)R.a[i]) + k * Fit.a[i]);
+        for (int i = 0; i < 9; ++i)
+            P.a[i] = (T)(2.0 * (double)mu * ((double)F.a[i] - (double)p.R.a[i]) + k * p.Fit.a[i]);
         return P;
     }
     // Analytic Hessian action of the reference (Elasticity.cpp:57-117):
@@ -379,12 +399,15 @@ template <> struct Model<0> {
     // (+1e-4 I when |det| < 1e-6).
     template <typename T>
     static DMPM_DI m3<T> hvp(const m3<T>& F, T lam, T mu, const m3<T>& dF) {
-        m3<T> R, S;
-        polar3(F, R, S);
-        m3<double> Fd = mcast<double>(F), dFd = mcast<double>(dF);
-        double Jr = mdet(Fd);
-        double J = fmax(Jr, 1e-6);
-        m3<double> Fit = mscale(1.0 / Jr, mcof(Fd));
+        return hvp(parts(F), lam, mu, dF);
+    }
+    template <typename T>
+    static DMPM_DI m3<T> hvp(const Parts<T>& p, T lam, T mu, const m3<T>& dF) {
+        const m3<T>& R = p.R;
+        const m3<T>& S = p.S;
+        const m3<double>& Fit = p.Fit;
+        const double Jr = p.Jr, J = p.J;
+        m3<double> dFd = mcast<double>(dF);
         double fd = mddot(Fit, dFd);
         m3<double> X = mmul(mmult(Fit, dFd), Fit);
         double c1 = (double)lam * J * J * fd;
