@@ -1092,6 +1092,16 @@ __global__ void __launch_bounds__(256) k_step_batch(const KP<T>* tab, const int*
     clear_share(a, lblock);
 }
 
+// The fixed-corotated FP32 adjoint parks the polar factors R (9 values) and
+// the upper triangle of S (6) of F + dFc in LDS through its node loop, after
+// the 12 phase-1 terms: its blocks hold three waves per SIMD, so the LDS has
+// room for them.  (FP64 would need twice the room; it recomputes instead.)
+template <typename T, int MODEL>
+__host__ __device__ constexpr bool park_polar() { return MODEL == 0 && sizeof(T) == 4; }
+template <typename T>
+__host__ __device__ constexpr int park_values(int model) { return model == 0 && sizeof(T) == 4 ? 27 : 12; }
+__device__ constexpr int kSymUpper[6] = {0, 1, 2, 4, 5, 8};
+
 // Phase 1 of the adjoint (BackPropagation.cpp:22-32).  REF reproduces the
 // reference's formulas; otherwise the exact VJP of P_op_2 including the
 // trust region and the smoothing blend.
@@ -1404,7 +1414,22 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
         // Gt = A_p = -k0 (P Ft^T + zeta tr(C) I) + m C
         m3<T> Gt;
         {
-            const m3<T> PFt = Model<MODEL>::kirchhoff(Ft, lam, mu);
+            m3<T> PFt;
+            if constexpr (park_polar<T, MODEL>()) {
+                // the polar factors of F + dFc are parked with the phase-1
+                // terms below and used again by the stress gradient after
+                // the node loop (one polar decomposition per particle, not two)
+                const typename Model<0>::template Parts<T> fp = Model<0>::parts(Ft);
+                PFt = mmult(Model<0>::pk1(fp, Ft, lam, mu), Ft);
+                T* park = reinterpret_cast<T*>(reinterpret_cast<char*>(smem_d) + a.park);
+                const int pt = threadIdx.x, pb = blockDim.x;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) park[(12 + q) * pb + pt] = fp.R.a[q];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) park[(21 + q) * pb + pt] = fp.S.a[kSymUpper[q]];
+            } else {
+                PFt = Model<MODEL>::kirchhoff(Ft, lam, mu);
+            }
     #pragma unroll
             for (int q = 0; q < 9; ++q) Gt.a[q] = -k0 * PFt.a[q] + m * C.a[q];
             const T bulk = -k0 * a.zeta * mtrace(C);
@@ -1580,7 +1605,22 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
         const v3<T> gv = {{dragf * sh.a[0], dragf * sh.a[1], dragf * sh.a[2]}};
         // A_p = -k0 (P Ft^T + zeta tr C I) + m C: the stress term's gradient
         // in Ft is that of <-k0 gA, P(Ft) Ft^T>
-        const m3<T> gS = Model<MODEL>::kirchhoff_vjp(Ft2, lam, mu, mscale(-k0, gA));
+        m3<T> gS;
+        if constexpr (park_polar<T, MODEL>()) {
+            const T* park = reinterpret_cast<const T*>(reinterpret_cast<const char*>(smem_d) + a.park);
+            const int pt = threadIdx.x, pb = blockDim.x;
+            m3<T> R, S;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) R.a[q] = park[(12 + q) * pb + pt];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) S.a[kSymUpper[q]] = park[(21 + q) * pb + pt];
+            S.a[3] = S.a[1];  // (the stress gradient reads the upper triangle only)
+            S.a[6] = S.a[2];
+            S.a[7] = S.a[5];
+            gS = Model<0>::kirchhoff_vjp(Model<0>::parts(Ft2, R, S), Ft2, lam, mu, mscale(-k0, gA));
+        } else {
+            gS = Model<MODEL>::kirchhoff_vjp(Ft2, lam, mu, mscale(-k0, gA));
+        }
         m3<T> gC = mscale(m, gA);
         if (REF) {
             // bulk-viscosity adjoint on the diagonal entries only (:117-123)
@@ -2152,19 +2192,22 @@ __global__ void k_adam(const T* dLdF, T* dFc, T* mom, T* vec, T* vmax, int n, lo
 // thread parked through the node loop.  With FP32 the window budget is held
 // to 27 KiB so that four 256-thread blocks (window + 12 KiB parked) share a
 // CU's 160 KiB.
-constexpr int kParkValues = 12;
+
 template <typename T>
-static int adj_window_budget(int lds_bytes) {
-    return sizeof(T) == 4 ? std::min(lds_bytes, 27 * 1024) : lds_bytes;
+static int adj_window_budget(int lds_bytes, int model) {
+    // FP32: the window plus the parked values fit four blocks per CU (three
+    // for the fixed-corotated model, which parks 27 values and holds three
+    // waves per SIMD)
+    return sizeof(T) == 4 ? std::min(lds_bytes, (model == 0 ? 24 : 27) * 1024) : lds_bytes;
 }
 template <typename T>
-static size_t adj_gather_lds(int budget, int block) {
-    return (size_t)((budget + 15) & ~15) + (size_t)kParkValues * block * sizeof(T);
+static size_t adj_gather_lds(int budget, int block, int model) {
+    return (size_t)((budget + 15) & ~15) + (size_t)park_values<T>(model) * block * sizeof(T);
 }
 template <typename T>
 static KP<T> adj_gather_kp(const Launch& L) {
     Launch Lw = L;
-    Lw.lds_bytes = adj_window_budget<T>(L.lds_bytes);
+    Lw.lds_bytes = adj_window_budget<T>(L.lds_bytes, L.ph.model);
     KP<T> k = make_kp<T>(Lw, 7 * (int)sizeof(T));
     k.park = (Lw.lds_bytes + 15) & ~15;
     return k;
@@ -2278,7 +2321,7 @@ template <typename T>
 static void launch_adj_gather(const Launch& L, hipStream_t s) {
     KP<T> k = adj_gather_kp<T>(L);
     const unsigned nb = launch_blocks(L);
-    const size_t sh = adj_gather_lds<T>(adj_window_budget<T>(L.lds_bytes), L.block);
+    const size_t sh = adj_gather_lds<T>(adj_window_budget<T>(L.lds_bytes, L.ph.model), L.block, L.ph.model);
     const bool ref = L.ph.adjoint == 1;
     const int o = L.ph.order, md = L.ph.model;
 #define AG_CASE(O, M, R) hipLaunchKernelGGL((k_adj_gather<T, O, M, R>), dim3(nb), dim3(L.block), sh, s, k)
@@ -2338,10 +2381,10 @@ template <typename T>
 static void launch_adj_fused(const Launch& L, const Launch& L2, hipStream_t s) {
     KP<T> k = adj_gather_kp<T>(L);
     Launch Lw2 = L2;
-    Lw2.lds_bytes = adj_window_budget<T>(L2.lds_bytes);
+    Lw2.lds_bytes = adj_window_budget<T>(L2.lds_bytes, L2.ph.model);
     KP<T> k2 = make_kp<T>(Lw2, 3 * (int)sizeof(Acc));
     const unsigned nb = launch_blocks(L);
-    const size_t sh = std::max(adj_gather_lds<T>(adj_window_budget<T>(L.lds_bytes), L.block),
+    const size_t sh = std::max(adj_gather_lds<T>(adj_window_budget<T>(L.lds_bytes, L.ph.model), L.block, L.ph.model),
                                (size_t)k2.cap * 3 * sizeof(Acc));
     const bool ref = L.ph.adjoint == 1;
     const int o = L.ph.order, md = L.ph.model;
@@ -2422,7 +2465,7 @@ static void build_kp_t(int kind, const Launch& L, const Launch* L2, void* dst) {
     std::memcpy(dst, &k, sizeof(k));
     if (kind == OP_ADJ_FUSED) {
         Launch Lw2 = *L2;
-        Lw2.lds_bytes = adj_window_budget<T>(L2->lds_bytes);
+        Lw2.lds_bytes = adj_window_budget<T>(L2->lds_bytes, L2->ph.model);
         const KP<T> k2 = make_kp<T>(Lw2, 3 * (int)sizeof(Acc));
         std::memcpy((char*)dst + sizeof(k), &k2, sizeof(k2));
     }
@@ -2445,8 +2488,8 @@ static void launch_batch_t(int kind, const Phys& ph, int lds_bytes, int block, c
     const int cap = lds_bytes / node_bytes<T>(kind);
     size_t sh = block_lds<T>(kind, cap);
     if (kind == OP_ADJ_GATHER || kind == OP_ADJ_FUSED) {
-        const int budget = adj_window_budget<T>(lds_bytes);
-        sh = adj_gather_lds<T>(budget, block);
+        const int budget = adj_window_budget<T>(lds_bytes, ph.model);
+        sh = adj_gather_lds<T>(budget, block, ph.model);
         if (kind == OP_ADJ_FUSED) sh = std::max(sh, (size_t)(budget / (3 * (int)sizeof(Acc))) * 3 * sizeof(Acc));
     }
     switch (kind) {

@@ -359,13 +359,26 @@ template <> struct Model<0> {
     };
     template <typename T>
     static DMPM_DI Parts<T> parts(const m3<T>& F) {
+        m3<T> R, S;
+        polar3(F, R, S);
+        return parts(F, R, S);
+    }
+    // The same with the polar factors of F already known.
+    template <typename T>
+    static DMPM_DI Parts<T> parts(const m3<T>& F, const m3<T>& R, const m3<T>& S) {
         Parts<T> p;
-        polar3(F, p.R, p.S);
+        p.R = R;
+        p.S = S;
         const m3<double> Fd = mcast<double>(F);
         p.Jr = mdet(Fd);
         p.J = fmax(p.Jr, 1e-6);
         p.Fit = mscale(1.0 / p.Jr, mcof(Fd));
         return p;
+    }
+    // Gradient of <G, P(F) F^T> in F from the parts of F.
+    template <typename T>
+    static DMPM_DI m3<T> kirchhoff_vjp(const Parts<T>& p, const m3<T>& F, T lam, T mu, const m3<T>& G) {
+        return madd(mtmul(G, pk1(p, F, lam, mu)), hvp(p, lam, mu, mmul(G, F)));
     }
     // Kirchhoff stress P F^T, formed from P as the reference does.
     template <typename T>
@@ -376,8 +389,7 @@ template <> struct Model<0> {
     // decomposition for both terms.
     template <typename T>
     static DMPM_DI m3<T> kirchhoff_vjp(const m3<T>& F, T lam, T mu, const m3<T>& G) {
-        const Parts<T> p = parts(F);
-        return madd(mtmul(G, pk1(p, F, lam, mu)), hvp(p, lam, mu, mmul(G, F)));
+        return kirchhoff_vjp(parts(F), F, lam, mu, G);
     }
     // P = 2 mu (F - R) + lam (J-1) J F^{-T}, J = max(det F, 1e-6)
     template <typename T>
