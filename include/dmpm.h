@@ -373,8 +373,8 @@ int dmpm_window_stats(dmpm_sim* sim, long long* lds_blocks, long long* fallback_
  * reference (a diagnostic of this library's LDS windows). */
 int dmpm_window_counters(dmpm_sim* sim, int on);
 /* Device bytes the sim holds (buffers in use plus freed buffers it keeps
- * for reuse: setting up again with the same particle count allocates nothing
- * new), and the peak.  Buffers of an old particle count are given back when
+ * for reuse: setting up again with the same particle count takes its
+ * buffers from them), and the peak.  Buffers of an old particle count are given back when
  * dmpm_set_particles changes it; dmpm_destroy gives back everything. */
 int dmpm_memory(const dmpm_sim* sim, size_t* current, size_t* peak);
 
