@@ -1890,7 +1890,7 @@ __global__ void k_fill_zero(T* p, long long n) {
     for (; i < n; i += stride) p[i] = T(0);
 }
 
-// Zero n16 16-byte words (grid planes: 16-byte aligned, sizes a multiple of
+// Zero n16 16-byte words (grids: 16-byte aligned, sizes a multiple of
 // 64 elements) with one dwordx4 store per word.
 __global__ void k_fill_zero16(uint4* p, long long n16) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
