@@ -381,6 +381,13 @@ DMPM_DI void store_s(T* base, long long ld, int plane, int i, T v) {
 // load.  A mass-only P2G writes one value per node.
 constexpr int kNode = 4;
 
+// Byte offset i * size + add of an array below 4 GiB (the host checks), in
+// unsigned arithmetic: the buffer instructions take it as an unsigned 32-bit
+// value, and it may exceed INT_MAX.
+DMPM_DI int byte_offset(int i, unsigned size, unsigned add) {
+    return (int)((unsigned)i * size + add);
+}
+
 // The node's values as kNode scalar buffer loads, which the compiler merges
 // into one 16-byte (FP32) or two (FP64) loads.  (Element extraction from the
 // vector the 64- and 128-bit buffer-load builtins return is narrowed to a
@@ -391,9 +398,9 @@ DMPM_DI void load_node(const T* base, int g, T* v) {
 #pragma unroll
     for (int q = 0; q < kNode; ++q) {
         if constexpr (sizeof(T) == 4) {
-            v[q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, g * 16 + 4 * q, 0, 0));
+            v[q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_offset(g, 16, 4 * q), 0, 0));
         } else {
-            v[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, g * 32 + 8 * q, 0, 0));
+            v[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte_offset(g, 32, 8 * q), 0, 0));
         }
     }
 }
@@ -403,7 +410,7 @@ DMPM_DI void load_node(const T* base, int g, T* v) {
 template <typename T>
 DMPM_DI void atomic_add_at(T* base, int e, T v) {
     if constexpr (sizeof(T) == 4) {
-        __builtin_amdgcn_raw_ptr_buffer_atomic_fadd_f32(v, plane_rsrc(base), e * 4, 0, 0);
+        __builtin_amdgcn_raw_ptr_buffer_atomic_fadd_f32(v, plane_rsrc(base), byte_offset(e, 4, 0), 0, 0);
     } else {
         unsafeAtomicAdd(base + e, v);
     }
