@@ -93,3 +93,53 @@ def test_empty_particle_set_is_refused():
         sim.set_state(dict(x=z, v=z, F=np.zeros((0, 3, 3)), C=np.zeros((0, 3, 3)), m=np.zeros(0), vol=np.zeros(0),
                            lam=np.zeros(0), mu=np.zeros(0)))
     sim.close()
+
+
+@pytest.mark.parametrize("prec,nodes", [("f32", 520), ("f64", 412)])
+def test_grid_byte_offsets_beyond_2gib(prec, nodes):
+    """A body near the far corner of a grid whose values reach past 2 GiB
+    (520³ nodes × 16 B in FP32, 412³ × 32 B in FP64): the kernels' 32-bit
+    byte offsets into the grid exceed INT_MAX there.  The same particles on a
+    small grid placed around them must give the same episode."""
+    from dmpm import Physics, Simulator
+    from dmpm.scenes import Body, make_scene
+    dx = 1.0 / (nodes - 1)
+    c = (nodes - 12) * dx
+    base = make_scene(Body((0.5, 0.5, 0.5), (0.2, 0.2, 0.2)), 600, 20, 29, 1e4, 0.3, velocity="sine",
+                      amplitude=0.4)
+    s = _subset(base, base.n)
+    shrink = 4 * dx / 0.2
+    s["x"] = c + (s["x"] - 0.5) * shrink  # radius 4 cells, centred 11 cells from the far faces
+    s["v"] = s["v"] * shrink              # the same velocity gradients as the unshrunk body
+    s["m"] = s["m"] * shrink ** 3         # and the same density
+    s["vol"] = s["vol"] * shrink ** 3
+    big = ((nodes,) * 3, (0.0, 0.0, 0.0))
+    small = ((24,) * 3, tuple(c - 12 * dx for _ in range(3)))
+    assert (nodes ** 3) * 4 * (4 if prec == "f32" else 8) > 2 ** 31
+    g_last = nodes ** 3 - 1
+    assert (np.floor(s["x"] / dx).max() * nodes * nodes) * 4 * (4 if prec == "f32" else 8) > 2 ** 31
+    assert g_last * 4 * (4 if prec == "f32" else 8) < 2 ** 32
+    T = 3
+    adj = random_adjoint(s["x"].shape[0], seed=4)
+    dt = np.float64 if prec == "f64" else np.float32
+    out = []
+    for dims, mn in (big, small):
+        print(f"[edges] byte offsets prec={prec} dims={dims}", flush=True)
+        # dt within the CFL bound of the fine grid (sqrt(E / rho) dt / dx = 0.16)
+        sim = Simulator(dims, dx, mn, physics=Physics(**physics_kwargs("plain", 1e-4)), precision=prec)
+        sim.set_state(dict(s, dFc=None))
+        sim.set_up(T + 1)
+        sim.forward(0)
+        sim.set_last_grad(x=adj["x"], v=adj["v"], F=adj["F"], C_=adj["C"])
+        sim.backward(0)
+        r = {k: sim.get(T, k, dtype=dt) for k in ("x", "v", "F", "C")}
+        r.update({"g" + k: sim.get_grad(0, k, dtype=dt) for k in ("x", "v", "F", "C")})
+        out.append(r)
+        sim.close()
+    # relative to each field's largest magnitude (positions: their absolute
+    # coordinates, about 1); the two grids round the particles' cell
+    # coordinates differently (about 508 against 12 cells), which FP32 shows
+    tol_s, tol_g = (1e-10, 1e-10) if prec == "f64" else (1e-4, 1e-3)
+    for k in out[1]:
+        e = rel_err(out[0][k], out[1][k])
+        assert e < (tol_g if k.startswith("g") else tol_s), f"{k}: rel err {e:.3e}"
