@@ -143,3 +143,51 @@ def test_grid_byte_offsets_beyond_2gib(prec, nodes):
     for k in out[1]:
         e = rel_err(out[0][k], out[1][k])
         assert e < (tol_g if k.startswith("g") else tol_s), f"{k}: rel err {e:.3e}"
+
+
+def test_particle_plane_offsets_beyond_2gib():
+    """12,000,000 FP64 particles: a layer's later planes start past 2 GiB, so
+    the kernels' plane offsets (particle plane p at p * ld * 8 bytes) exceed
+    INT_MAX.  A small cluster far from the bulk of them must step exactly as
+    it does alone."""
+    from dmpm import Physics, Simulator
+    n_bulk, n_small, nodes = 12_000_000, 1_000, 129
+    dx = 1.0 / (nodes - 1)
+    assert 23 * ((n_bulk + n_small + 63) // 64 * 64) * 8 > 2 ** 31
+    rng = np.random.default_rng(41)
+    xb = rng.uniform(0.1, 0.6, size=(n_bulk, 3))
+    xs = 0.8 + 0.05 * rng.normal(size=(n_small, 3)).clip(-1.0, 1.0)
+    def state(x, seed):
+        r = np.random.default_rng(seed)
+        k = x.shape[0]
+        F = np.eye(3) + 0.02 * r.normal(size=(k, 3, 3))
+        return dict(x=x, v=0.1 * r.normal(size=(k, 3)), F=F, C=0.5 * r.normal(size=(k, 3, 3)),
+                    m=np.full(k, 1e3 * dx ** 3 / 8), vol=np.full(k, dx ** 3 / 8),
+                    lam=np.full(k, 5e3), mu=np.full(k, 3e3))
+    bulk, small = state(xb, 1), state(xs, 2)
+    both = {k: np.concatenate([bulk[k], small[k]]) for k in bulk}
+    del bulk, xb
+    T = 2
+    adj = random_adjoint(n_small, seed=6)
+    out = []
+    for name, s in (("bulk + cluster", both), ("cluster alone", small)):
+        print(f"[edges] plane offsets: {name}, n={s['x'].shape[0]}", flush=True)
+        sim = Simulator((nodes,) * 3, dx, (0.0, 0.0, 0.0), physics=Physics(**physics_kwargs("plain", 1e-4)),
+                        precision="f64")
+        sim.set_state(s)
+        sim.set_up(T + 1)
+        sim.forward(0)
+        n = s["x"].shape[0]
+        g = {k: np.zeros((n,) + a.shape[1:]) for k, a in adj.items()}
+        for k in g:
+            g[k][n - n_small:] = adj[k]
+        sim.set_last_grad(x=g["x"], v=g["v"], F=g["F"], C_=g["C"])
+        sim.backward(0)
+        r = {k: sim.get(T, k, dtype=np.float64)[n - n_small:] for k in ("x", "v", "F", "C")}
+        r.update({"g" + k: sim.get_grad(0, k, dtype=np.float64)[n - n_small:] for k in ("x", "v", "F", "C")})
+        out.append(r)
+        sim.close()
+        del g
+    for k in out[1]:
+        e = rel_err(out[0][k], out[1][k])
+        assert e < 1e-10, f"{k}: rel err {e:.3e}"
