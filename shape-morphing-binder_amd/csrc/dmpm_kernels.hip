@@ -840,7 +840,7 @@ __global__ void __launch_bounds__(256) k_p2g_batch(const KP<T>* tab, const int* 
 // them before it uses any, so a pass costs one memory round trip.  Slots
 // past the window or off the grid load node 0 and are zeroed.
 constexpr int kStage = 4;
-constexpr int kStageAdj = 2;  // the adjoint stages seven values per slot
+constexpr int kStageAdj = 2;  // the adjoint loads eight values per slot: m, p and dL/dv of the node
 
 // Stage the window's grid velocities in LDS (3 arrays of cap).
 template <typename T>
@@ -1364,8 +1364,8 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
     S.init(a, x);  // lanes without a particle get a finite stencil at x = 0
     const Win win = block_window<KS>(valid, S.b[0], S.b[1], S.b[2], a.cap);
     if (win.lds) {
-        // kStageAdj slots per thread and pass, all seven loads of each
-        // issued before any is used (slots past the window or off the grid
+        // kStageAdj slots per thread and pass, the eight values of each
+        // loaded before any is used (slots past the window or off the grid
         // load node 0 and are zeroed)
         constexpr int U = kStageAdj;
         const int B = blockDim.x;
