@@ -84,6 +84,7 @@ struct KP {
     unsigned long long* wstat;
     uint4* zero;       // cleared by this launch for a later one (or null)
     long long zero16;  // its 16-byte chunks
+    long long zper;    // chunks per logical block: ceil(zero16 / zblk)
     int zblk;          // logical blocks sharing the clear
     int cap;
     int park;          // adjoint gather: byte offset in LDS of the values parked through its node loop
@@ -143,6 +144,7 @@ static KP<T> make_kp(const Launch& L, int narrays) {
     k.zero = (uint4*)L.zero;
     k.zero16 = L.zero ? L.zero_bytes / 16 : 0;
     k.zblk = (int)launch_blocks(L);
+    k.zper = (k.zero16 + k.zblk - 1) / k.zblk;
     k.cap = narrays > 0 ? (int)(L.lds_bytes / narrays) : 0;  // narrays = LDS bytes per window node
     k.park = 0;
     return k;
@@ -188,7 +190,7 @@ template <typename T>
 DMPM_DI void clear_share(const KP<T>& a, int lblock) {
     if (!a.zero) return;
     typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-    const long long per = (a.zero16 + a.zblk - 1) / a.zblk;
+    const long long per = a.zper;  // (a 64-bit division on the scalar unit per wave otherwise)
     const long long c1 = min(a.zero16, (long long)(lblock + 1) * per);
     u32x4* z = reinterpret_cast<u32x4*>(a.zero);
     for (long long c = (long long)lblock * per + threadIdx.x; c < c1; c += blockDim.x)
@@ -1060,8 +1062,12 @@ DMPM_DI void step_block(const KP<T>& a, int lblock, double* smem_d) {
         g2p_gather<T, ORDER>(a, win, sm, S, vn, Craw);
         particle_update(a, i, x, F, Ft, vn, Craw, xn, Fn, Cn);
     }
-    // every wave is done with the staged velocities and the first window
-    __syncthreads();
+    // The next window's reduction overwrites the first one's table, and its
+    // barrier (inside block_window) is the one after which the accumulators,
+    // which alias the staged velocities, are cleared.  With the staging barrier
+    // above, every wave has read the first table already; without it (the
+    // window did not fit), a barrier here keeps the tables apart.
+    if (!win.lds) __syncthreads();
     Sten<T, ORDER> S1;
     S1.init(a, xn);
     const Win win1 = block_window<KS>(valid, S1.b[0], S1.b[1], S1.b[2], a.cap);
@@ -1661,7 +1667,11 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
             if (b->ctl) Ft1 = madd(Ft1, load_m3(b->ctl, b->ld, 0, i));
             C1 = load_m3(a.st, a.ld, PC, i);
         }
-        __syncthreads();  // the node values staged above are read no more
+        // The scatter window's reduction barrier (inside block_window) also
+        // ends every wave's reads of the node values staged above, which its
+        // accumulators overwrite; a barrier here is needed only when nothing
+        // was staged, to keep the two reductions' tables apart.
+        if (!win.lds) __syncthreads();
         adj_scatter_core<T, ORDER, REF>(*b, valid, x1, Ft1, C1, cyt, true, nullptr, reinterpret_cast<Acc*>(smem_d));
     }
 }
@@ -2469,6 +2479,7 @@ static void build_kp_t(int kind, const Launch& L, const Launch* L2, void* dst) {
     KP<T> k = adj ? adj_gather_kp<T>(L) : make_kp<T>(L, node_bytes<T>(kind));
     k.zblk = instance_blocks(L);  // the instance's blocks of a batched launch
     if (k.zero && k.zblk <= 0) throw std::runtime_error("a clear over an instance without blocks");
+    k.zper = k.zblk > 0 ? (k.zero16 + k.zblk - 1) / k.zblk : 0;
     std::memcpy(dst, &k, sizeof(k));
     if (kind == OP_ADJ_FUSED) {
         Launch Lw2 = *L2;

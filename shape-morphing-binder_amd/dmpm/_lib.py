@@ -10,7 +10,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdmpm.so")
+# DMPM_LIB names another build of the library (a variant built with other
+# compile flags, say); by default the in-tree one
+LIB_PATH = os.environ.get("DMPM_LIB") or os.path.join(HERE, "libdmpm.so")
 
 DMPM_OK = 0
 DTYPE_F32, DTYPE_F64 = 0, 1
