@@ -4,9 +4,10 @@
   gradients w.r.t. initial velocity and F against the oracle (1e-10).
 * The 64 uneven batched instances: one hipGraph holding every instance's
   episode must reproduce the per-instance eager episodes.
-* 4,194,304 particles on 256^3: size-independent properties -- mass and
-  linear momentum conservation, the adjoint against a central difference in
-  FP64, and FP32 gradients against FP64.
+* 4,194,304 particles on 256^3, the full 500-step episode with checkpoint
+  interval 25 and controls at 20 layers: size-independent properties -- mass
+  and linear momentum conservation, the control gradient against a central
+  difference in FP64, and FP32 gradients against FP64.
 """
 import numpy as np
 import pytest
@@ -100,28 +101,45 @@ def test_config4_batched_graph_matches_eager():
 
 
 def test_config5_full_scale_properties():
-    """4,194,304 particles (r = 0.3 sphere), 256^3 cells, dt = 5e-5, 20
-    steps, K = 5.  Properties that hold at any size: total mass and linear
-    momentum are conserved (no wall contact, no external force); the adjoint
-    of L = <w, x_T> along a random direction of v_0 matches a central
-    difference (FP64, h = 1e-4); FP32 gradients match FP64 within 1e-3."""
+    """The full-scale morph episode: 4,194,304 particles (r = 0.3 sphere),
+    256^3 cells, dt = 5e-5, 500 steps with checkpoint interval 25, a
+    per-particle control at 20 control layers (every 25th), and a seeded
+    upstream gradient dL/dx (Gaussian, sigma 1e-3) and dL/dF (sigma 1e-4)
+    injected at the last layer -- L = <gx, x_T> + <gF, F_T>.  Properties that
+    hold at any size: total mass and linear momentum are conserved over the
+    500 steps (no wall contact, no external force); the adjoint's gradient
+    w.r.t. the control of layer 250 along a random direction matches a central
+    difference (FP64, h = 1e-4); FP32 gradients (control and initial
+    velocity) match FP64 within 1e-3."""
     from dmpm import Physics, Simulator, p2g_mass
     from dmpm.scenes import config5
     sc = config5()
     s = sc.state()
-    T, K, dt = 20, 5, 5e-5
-    print(f"[config5] n={sc.n} dims={sc.dims} T={T}", flush=True)
+    T, K, dt = 500, 25, 5e-5
+    layers = list(range(0, T, T // 20))
+    lc = 250
+    print(f"[config5] n={sc.n} dims={sc.dims} T={T} K={K} control layers {len(layers)}", flush=True)
     rng = np.random.default_rng(55)
-    w = rng.normal(size=(sc.n, 3))
-    dv = rng.normal(size=(sc.n, 3))
+    gx = 1e-3 * rng.normal(size=(sc.n, 3))
+    gF = 1e-4 * rng.normal(size=(sc.n, 3, 3))
+    D = rng.normal(size=(sc.n, 3, 3))
 
     def make(prec):
-        return Simulator(sc.dims, sc.dx, physics=Physics.plain(dt=dt), precision=prec, checkpoint_interval=K)
+        sim = Simulator(sc.dims, sc.dx, physics=Physics.plain(dt=dt), precision=prec, checkpoint_interval=K)
+        sim.set_state(s)
+        sim.set_up(T + 1)
+        for t in layers:
+            sim.set_control(t, np.zeros((sc.n, 3, 3)))
+            sim.request_control_grad(t)
+        return sim
+
+    def loss(sim):
+        x = sim.get(T, "x", dtype=np.float64)
+        F = sim.get(T, "F", dtype=np.float64)
+        return float((gx * x).sum() + (gF * F).sum())
 
     sim = make("f64")
-    sim.set_state(s)
-    sim.set_up(T + 1)
-    sim.forward(0)
+    sim.forward(0, keep_controls=True)
     xT, vT = sim.get(T, "x", dtype=np.float64), sim.get(T, "v", dtype=np.float64)
     m = s["m"].astype(np.float64)
     gm = p2g_mass(sc.dims, sc.dx, sc.grid_min, xT, m, order=2, precision="f64")
@@ -129,33 +147,39 @@ def test_config5_full_scale_properties():
     p0, pT = (m[:, None] * s["v"]).sum(0), (m[:, None] * vT).sum(0)
     scale = (m[:, None] * np.abs(s["v"])).sum()
     print(f"[config5] momentum drift {np.abs(pT - p0).max() / scale:.2e}", flush=True)
-    assert np.abs(pT - p0).max() < 1e-10 * scale
+    # exact but for rounding, which accumulates with the steps: 1.6e-10 of
+    # the momentum scale after these 500 steps (the 20-step form of this
+    # test stayed below 1e-10)
+    assert np.abs(pT - p0).max() < 1e-9 * scale
+    del xT, vT, gm
 
-    sim.set_last_grad(x=w)
+    sim.set_last_grad(x=gx, F=gF)
     sim.backward(0)
+    gc64 = sim.get_control_grad(lc, dtype=np.float64)
     gv64 = sim.get_grad(0, "v", dtype=np.float64)
-    an = float((gv64 * dv).sum())
+    an = float((gc64 * D).sum())
     h = 1e-4
     Ls = []
     for sgn in (1, -1):
-        sim.set_state(dict(s, v=s["v"] + sgn * h * dv))
-        sim.set_up(T + 1)
-        sim.forward(0)
-        Ls.append(float((w * sim.get(T, "x", dtype=np.float64)).sum()))
+        sim.set_control(lc, sgn * h * D)
+        sim.forward(0, keep_controls=True)
+        Ls.append(loss(sim))
     fd = (Ls[0] - Ls[1]) / (2 * h)
-    print(f"[config5] adjoint {an:.12e} central difference {fd:.12e}", flush=True)
-    assert abs(an - fd) <= 1e-7 * abs(fd)
+    print(f"[config5] control gradient of layer {lc}: adjoint {an:.12e} central difference {fd:.12e}", flush=True)
+    assert abs(an - fd) <= 1e-6 * abs(fd)
     sim.close()
 
     sim32 = make("f32")
-    sim32.set_state(s)
-    sim32.set_up(T + 1)
-    sim32.forward(0)
-    sim32.set_last_grad(x=w)
+    sim32.forward(0, keep_controls=True)
+    sim32.set_last_grad(x=gx, F=gF)
     sim32.backward(0)
-    e = np.linalg.norm(sim32.get_grad(0, "v", dtype=np.float64) - gv64) / np.linalg.norm(gv64)
-    print(f"[config5] fp32 vs fp64 dL/dv0: {e:.2e}", flush=True)
-    assert e < 1e-3
+    gc32 = sim32.get_control_grad(lc, dtype=np.float64)
+    gv32 = sim32.get_grad(0, "v", dtype=np.float64)
+    ec = np.linalg.norm(gc32 - gc64) / np.linalg.norm(gc64)
+    ev = np.linalg.norm(gv32 - gv64) / np.linalg.norm(gv64)
+    print(f"[config5] fp32 vs fp64 (relative L2): control gradient {ec:.2e}, dL/dv0 {ev:.2e}; "
+          f"max-relative {rel_err(gc32, gc64):.2e}, {rel_err(gv32, gv64):.2e}", flush=True)
+    assert ec < 1e-3 and ev < 1e-3
     sim32.close()
 
 
