@@ -377,10 +377,10 @@ DMPM_DI void store_s(T* base, long long ld, int plane, int i, T v) {
 
 // Grid arrays hold kNode values per node, node g at [kNode g, kNode g +
 // kNode): m, p0, p1, p2 in the forward grid, dL/dv_i 0..2 and an unused slot
-// in the adjoint grid.  The L2 carries out float atomics per 64-byte request,
-// so the window flushes, whose lanes walk a window row node by node and value
-// by value, cover whole lines with one request; and a node is one 16-byte
-// load.  A mass-only P2G writes one value per node.
+// in the adjoint grid.  Device-scope float atomics travel past the L2 to
+// memory in 64-byte requests, so the window flushes, whose lanes walk a
+// window row node by node and value by value, cover whole lines with one
+// request; and a node is one 16-byte load.  A mass-only P2G writes one value per node.
 constexpr int kNode = 4;
 
 // Byte offset i * size + add of an array below 4 GiB (the host checks), in
