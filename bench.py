@@ -228,18 +228,25 @@ def run_episodes(sim, count):
         sim.backward(0)
 
 
-def warm_up(sim, count, T, sort=True):
+def warm_up(sim, count, T, sort=True, segments=0):
     """`count` untimed episodes; after the first, the particle storage is
-    reordered by the middle layer's positions (dmpm_sort_particles): the
-    particles drift from the order they were uploaded in, and the middle
-    layer's order halves the drift over the episode.  No warm-up, no sort."""
+    reordered per episode segment (dmpm_sort_segments): the particles drift
+    from the order they were uploaded in, and each run of layers is stored
+    in the Hilbert order of its middle layer.  `segments` = 0 lets the
+    autotuner pick the count (one more untimed episode per candidate: the
+    whole episode in one order, a segment every 25 or every 10 layers).
+    Returns the autotuner's choice and table (None without a sort).  No
+    warm-up, no sort."""
     if count <= 0:
-        return
+        return None
+    from dmpm.autotune import tune_segments
     run_episodes(sim, 1 if sort else count)
+    info = None
     if sort:
-        sim.sort_particles(T // 2)
+        info = tune_segments(sim, [segments] if segments > 0 else None)
         run_episodes(sim, max(count - 1, 0))
     sim.synchronize()
+    return info
 
 
 def batched_workload(dt: float = 2.5e-4, T: int = 100, K: int = 10, reps: int = 2, lds_bytes: int = 0,
@@ -273,7 +280,8 @@ def batched_workload(dt: float = 2.5e-4, T: int = 100, K: int = 10, reps: int = 
 
     log(f"[bench] batched: {len(sims)} instances set up, {total} particles")
     for s in sims:
-        warm_up(s, 1, T)
+        # one order each: a batched episode runs every instance in one
+        warm_up(s, 1, T, segments=1)
     t0 = time.perf_counter()
     for _ in range(reps):
         sequential()
@@ -323,7 +331,8 @@ def full_scale_episode(T: int = 500, K: int = 25, controls: int = 20) -> dict:
     interval 25, Gaussian dL/dx (sigma 1e-3) and dL/dF (sigma 1e-4) injected
     at the last layer and backpropagated to a per-particle control at 20
     control layers (every 25th).  One warm-up episode, then the storage is
-    re-sorted by the middle layer (a checkpoint), as in the main workload."""
+    re-sorted per run of layers (whole checkpoint segments), as in the main
+    workload."""
     from dmpm import Physics, Simulator
     from dmpm.scenes import config5
     sc = config5()
@@ -339,8 +348,9 @@ def full_scale_episode(T: int = 500, K: int = 25, controls: int = 20) -> dict:
     rng = np.random.default_rng(5)
     sim.set_last_grad(x=1e-3 * rng.normal(size=(sc.n, 3)), F=1e-4 * rng.normal(size=(sc.n, 3, 3)))
     log(f"[bench] full scale: {sc.n} particles set up")
-    warm_up(sim, 1, T)
-    log("[bench] full scale: warm-up episode done")
+    order = warm_up(sim, 1, T)
+    log(f"[bench] full scale: warm-up episode done; storage orders: {order['segments']} segment(s); "
+        + ", ".join(f"{r['segments']}: {r['ms']:.0f}ms" for r in order["table"]))
     t0 = time.perf_counter()
     run_episodes(sim, 1)
     sim.synchronize()
@@ -349,7 +359,7 @@ def full_scale_episode(T: int = 500, K: int = 25, controls: int = 20) -> dict:
     _, peak = sim.memory()
     sim.close()
     out = {"particles": sc.n, "grid_nodes": list(sc.dims), "episode_steps": T, "checkpoint_interval": K,
-           "control_layers": len(layers), "value": sc.n * T / w, "unit": "particle-steps/s",
+           "control_layers": len(layers), "order_segments": order, "value": sc.n * T / w, "unit": "particle-steps/s",
            "ms_per_step": 1e3 * w, "peak_device_bytes": peak,
            "control_grad_finite": bool(np.isfinite(g).all()), "control_grad_rms": float(np.sqrt((g ** 2).mean()))}
     log(f"[bench] full scale: {1e3 * w:.0f} ms/episode, {out['value']:.4g} particle-steps/s, "
@@ -437,7 +447,7 @@ def implosion_workload(T: int, lds_bytes: int, block: int) -> dict:
     def run(prec, K, scene, det_every=0):
         s = episode_sim(scene, prec, K, T, gx, gF, lds_bytes=lds_bytes if prec == "f32" else 0,
                         block=block if prec == "f32" else 0)
-        warm_up(s, 1, T)
+        order = warm_up(s, 1, T)
         t0 = time.perf_counter()
         run_episodes(s, 1)
         s.synchronize()
@@ -447,17 +457,18 @@ def implosion_workload(T: int, lds_bytes: int, block: int) -> dict:
         jmin = [float(np.linalg.det(s.get(t, "F", dtype=np.float64)).min()) for t in range(0, T + 1, det_every)] \
             if det_every else None
         s.close()
-        return w, o, g, jmin
+        return w, o, g, jmin, order["segments"]
 
-    w64, o64, g64, jmin = run("f64", 1, sc, det_every=10)
+    w64, o64, g64, jmin, _ = run("f64", 1, sc, det_every=10)
     scp = config3_implosion()
     scp.x = scp.x * (1 + 1e-7 * np.random.default_rng(9).normal(size=scp.x.shape))
-    _, op, gp, _ = run("f64", 1, scp)
-    w1, o1, g1, _ = run("f32", 1, sc)
-    w25, o25, g25, _ = run("f32", 25, sc)
+    _, op, gp, _, _ = run("f64", 1, scp)
+    w1, o1, g1, _, s1 = run("f32", 1, sc)
+    w25, o25, g25, _, s25 = run("f32", 25, sc)
     out = {"scene": "sphere r=0.3, 1,048,576 particles, 128^3, sine field (0.5) + radial implosion 22/s",
            "detF_min_over_episode": min(jmin), "detF_min_every_10_layers": jmin,
            "fp64_ms": 1e3 * w64, "fp32_ms": 1e3 * w1, "fp32_checkpointed_ms": 1e3 * w25,
+           "fp32_order_segments": s1, "fp32_checkpointed_order_segments": s25,
            "errors_vs_fp64": fp32_errors(o1, g1, o64, g64, sc.n),
            "checkpointed_errors_vs_fp64": fp32_errors(o25, g25, o64, g64, sc.n),
            "fp64_x0_perturbed_1e-7_vs_fp64": fp32_errors(op, gp, o64, g64, sc.n)}
@@ -490,7 +501,9 @@ def main():
     ap.add_argument("--no-implosion", action="store_true",
                     help="skip the strongly compressed companion scene (FP32 vs FP64 at det F ~ 0.3)")
     ap.add_argument("--no-sort", action="store_true",
-                    help="keep the upload order (no dmpm_sort_particles after the first warm-up episode)")
+                    help="keep the upload order (no re-sort after the first warm-up episode)")
+    ap.add_argument("--segments", type=int, default=0,
+                    help="storage-order segments of each episode (dmpm_sort_segments); 0 = chosen by timing")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (3 launches): the known-size kernel "
                          "that scales the FETCH_SIZE/WRITE_SIZE counters of a --pmc run")
@@ -542,7 +555,10 @@ def main():
     # replay of the captured episode graph measures the same), then one more
     # episode with the per-kernel HIP-event timers on.
     sim = episode_sim(sc, "f32", K, T, seed_gx, seed_gF, lds_bytes=lds_bytes, block=block)
-    warm_up(sim, args.warmup, T, sort=not args.no_sort)
+    order = warm_up(sim, args.warmup, T, sort=not args.no_sort, segments=args.segments)
+    if order:
+        log(f"[bench] storage orders: {order['segments']} segment(s); "
+            + ", ".join(f"{r['segments']}: {r['ms']:.1f}ms" for r in order["table"]))
     t0 = time.perf_counter()
     run_episodes(sim, args.steps)
     sim.synchronize()
@@ -611,7 +627,7 @@ def main():
     extra = {}
     if not args.no_checkpointed and K != 25:
         simc = episode_sim(sc, "f32", 25, T, seed_gx, seed_gF, lds_bytes=lds_bytes, block=block)
-        warm_up(simc, args.warmup, T, sort=not args.no_sort)
+        warm_up(simc, args.warmup, T, sort=not args.no_sort, segments=args.segments)
         t0 = time.perf_counter()
         run_episodes(simc, args.steps)
         simc.synchronize()
@@ -627,7 +643,7 @@ def main():
             f"peak {peak_c / 1e9:.2f} GB")
     if not args.no_fp64:
         sim64 = episode_sim(sc, "f64", K, T, seed_gx, seed_gF)
-        warm_up(sim64, 1, T, sort=not args.no_sort)
+        warm_up(sim64, 1, T, sort=not args.no_sort, segments=args.segments)
         t0 = time.perf_counter()
         run_episodes(sim64, 1)
         sim64.synchronize()
@@ -653,7 +669,7 @@ def main():
     if not args.no_reference_physics:
         simr = episode_sim(sc, "f32", K, T, seed_gx, seed_gF,
                            physics=Physics.reference(dt=1e-4, drag=0.0, smoothing=0.0))
-        warm_up(simr, 1, T, sort=not args.no_sort)
+        warm_up(simr, 1, T, sort=not args.no_sort, segments=args.segments)
         t0 = time.perf_counter()
         run_episodes(simr, 1)
         simr.synchronize()
@@ -700,9 +716,10 @@ def main():
             "particles": sc.n, "grid_nodes": list(sc.dims), "episode_steps": T, "checkpoint_interval": K,
             "tape": ("every layer's state and P2G grid kept in HBM (no recomputation)" if K == 1
                      else f"checkpoints every {K} layers, segments recomputed"),
-            "particle_order": ("upload order (Hilbert order of layer 0's cells)" if args.no_sort else
-                               f"Hilbert order of layer {T // 2}'s cells (dmpm_sort_particles after the first "
-                               f"warm-up episode)"),
+            "particle_order": ("upload order (Hilbert order of layer 0's cells)" if not order else
+                               f"{order['segments']} run(s) of layers, each stored in the Hilbert order of its "
+                               f"middle layer's cells (dmpm_sort_segments after the first warm-up episode)"),
+            "order_segments": order,
             "physics": "plain: quadratic B-spline, Neo-Hookean, sticky walls (3 cells)",
             "precision": "FP32 state/arithmetic, FP64 det/log J/F^-T", "E": 5e4, "nu": 0.45, "dt": 1e-4,
             "active_nodes": gact, "peak_device_bytes": mem_peak,

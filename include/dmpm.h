@@ -193,6 +193,27 @@ int dmpm_render_gradient_info(const dmpm_sim* sim, int* has, int* num_points, do
  * sim is alive: those hold its device buffers.  No counterpart in the
  * reference (its particles have no device order). */
 int dmpm_sort_particles(dmpm_sim* sim, int layer);
+/* Reorder the particle storage per episode segment: the layers are split
+ * into `segments` runs of equal length (multiples of the checkpoint
+ * interval), and every per-particle array of a run's layers is stored in the
+ * Hilbert order of the run's middle layer, its thread blocks cut so that
+ * their windows also fit at the stored layers bounding the run.  A body
+ * that is advected or deformed drifts away from any single order within a
+ * few dozen steps (runs of equal cells break up, windows grow); segment
+ * orders keep every layer near an order of its own.  The forward step into
+ * a new segment runs in the previous order and its output is permuted into
+ * the new one; the adjoint permutes its carry (and the boundary layer, where
+ * it reads it) back.  Besides that the sim holds per order a permutation,
+ * its block table and the four static planes, and three index maps.
+ * Results do not change beyond summation order, and getters,
+ * setters and gradients keep the caller's order.  segments = 1 is one order
+ * for the whole episode.  dmpm_sort_particles, dmpm_set_particles,
+ * dmpm_set_up_comp_graph, dmpm_promote_last_as_initial and
+ * dmpm_batch_capture return the sim to one storage order.  Refused
+ * (DMPM_ERR_STATE) while a captured graph or batched episode of the sim is
+ * alive, and in deterministic mode.  No counterpart in the reference (its
+ * particles have no device order). */
+int dmpm_sort_segments(dmpm_sim* sim, int segments);
 /* promote_last_as_initial(carry_grid) (bind.cpp:282-291). */
 int dmpm_promote_last_as_initial(dmpm_sim* sim, int carry_grid);
 

@@ -166,17 +166,22 @@ __host__ __device__ inline uint32_t hilbert3(uint32_t x0, uint32_t x1, uint32_t 
 //    merge their sums across lanes;
 //  * thread blocks of up to `block` consecutive particles, each ended
 //    before the particle that would grow the box of its stencils beyond
-//    split_nodes grid nodes (0: 512; negative: no cuts, consecutive blocks);
+//    split_nodes grid nodes (0: 512; negative: no cuts, consecutive blocks),
+//    at x and at each of the `ncut` (at most kOrderCutSets - 1) further
+//    position sets xcut[] of the same layout -- other layers of an episode,
+//    so that the blocks' windows stay within the bound as the body deforms;
 //  * inside each block, its runs of equal base cells re-ordered so that few
 //    cross a 16-lane row (best fit: each row takes the longest remaining run
 //    that fits its free lanes, else the longest remaining run).
 // perm (n ints) receives the source index of each stored particle and
 // starts (n + 1 ints of room) the blocks' first particles and n; returns the
 // block count (0 without cuts).  Synchronises the stream (the count).
-size_t order_scratch_bytes(int n);
+constexpr int kOrderCutSets = 3;
+size_t order_scratch_bytes(int n, int ncut = 0);
 int order_particles_device(int order, int block, int split_nodes, const double grid_min[3], double dx,
                            const void* x, int x_f64, int soa, long long ld, int n, int* perm, int* starts,
-                           void* scratch, size_t scratch_bytes, hipStream_t s);
+                           void* scratch, size_t scratch_bytes, hipStream_t s, const void* const* xcut = nullptr,
+                           int ncut = 0);
 
 // Record the message dmpm_last_error() returns (for library parts outside
 // the runtime's own error guard).
@@ -230,6 +235,8 @@ void export_F_total(Prec pr, const void* st, const void* ctl, void* dst, int dst
 void permute_planes(Prec pr, const void* src, void* dst, const int* q, int n, int nc, long long ld, hipStream_t s);
 // dst[j] = src[q[j]] for n ints.
 void permute_ints(const int* src, int* dst, const int* q, int n, hipStream_t s);
+// inv[q[j]] = j for a permutation q of 0..n-1
+void invert_perm(const int* q, int* inv, int n, hipStream_t s);
 // Copy between same-layout plane buffers (`count` elements).
 void copy_elems(Prec pr, void* dst, const void* src, long long count, hipStream_t s);
 

@@ -2630,6 +2630,15 @@ void permute_ints(const int* src, int* dst, const int* q, int n, hipStream_t s) 
     hipLaunchKernelGGL(k_permute<int>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, q, n, 1, 0LL);
 }
 
+__global__ void k_invert_perm(const int* q, int* inv, int n) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) inv[q[j]] = j;
+}
+
+void invert_perm(const int* q, int* inv, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_invert_perm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, q, inv, n);
+}
+
 void fill_zero(Prec pr, void* p, long long n, hipStream_t s) {
     if (n <= 0) return;
     const long long bytes = n * (pr == PREC_F32 ? 4 : 8);
@@ -2829,27 +2838,35 @@ __global__ void k_order_cells(const S* x, long long ld, int n, const int* perm, 
 }
 
 // End of the block that would start at particle i: it takes particles while
-// it holds fewer than B and the box of their stencils stays within split nodes.
-__global__ void k_block_next(const int* cell, int n, int B, int KS, long long split, int* next) {
+// it holds fewer than B and the box of their stencils stays within split
+// nodes in every one of the `sets` base-cell sets (cell + q * 3n holds set
+// q: the sort layer's cells, then other layers').
+__global__ void k_block_next(const int* cell, int sets, int n, int B, int KS, long long split, int* next) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int lo[3], hi[3];
-    for (int a = 0; a < 3; ++a) lo[a] = hi[a] = cell[3 * (long long)i + a];
+    int lo[kOrderCutSets][3], hi[kOrderCutSets][3];
+    for (int q = 0; q < sets; ++q)
+        for (int a = 0; a < 3; ++a) lo[q][a] = hi[q][a] = cell[(long long)q * 3 * n + 3 * (long long)i + a];
     int j = i + 1;
     for (; j < n && j - i < B; ++j) {
-        int l2[3], h2[3];
-        long long vol = 1;
-        for (int a = 0; a < 3; ++a) {
-            const int c = cell[3 * (long long)j + a];
-            l2[a] = min(lo[a], c);
-            h2[a] = max(hi[a], c);
-            vol *= (long long)h2[a] - l2[a] + KS;
+        int l2[kOrderCutSets][3], h2[kOrderCutSets][3];
+        bool fits = true;
+        for (int q = 0; q < sets; ++q) {
+            long long vol = 1;
+            for (int a = 0; a < 3; ++a) {
+                const int c = cell[(long long)q * 3 * n + 3 * (long long)j + a];
+                l2[q][a] = min(lo[q][a], c);
+                h2[q][a] = max(hi[q][a], c);
+                vol *= (long long)h2[q][a] - l2[q][a] + KS;
+            }
+            fits = fits && vol <= split;
         }
-        if (vol > split) break;
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = l2[a];
-            hi[a] = h2[a];
-        }
+        if (!fits) break;
+        for (int q = 0; q < sets; ++q)
+            for (int a = 0; a < 3; ++a) {
+                lo[q][a] = l2[q][a];
+                hi[q][a] = h2[q][a];
+            }
     }
     next[i] = j;
 }
@@ -2930,7 +2947,7 @@ size_t order_cub_bytes(int n) {
                                              (const int*)nullptr, (int*)nullptr, n, 0, 30, (hipStream_t)0);
     return temp;
 }
-OrderLayout order_layout(void* base, int n) {
+OrderLayout order_layout(void* base, int n, int sets) {
     OrderLayout L{};
     char* p = (char*)base;
     auto take = [&](size_t bytes) {
@@ -2943,7 +2960,7 @@ OrderLayout order_layout(void* base, int n) {
     L.kout = (unsigned*)take(4 * nn);
     L.vin = (int*)take(4 * nn);
     L.vout = (int*)take(4 * nn);
-    L.cell = (int*)take(12 * nn);
+    L.cell = (int*)take(12 * nn * (size_t)sets);
     L.next = (int*)take(4 * nn);
     L.rlen = (int*)take(4 * nn);
     L.rstart = (int*)take(4 * nn);
@@ -2955,27 +2972,30 @@ OrderLayout order_layout(void* base, int n) {
 }
 }  // namespace
 
-size_t order_scratch_bytes(int n) { return 45 * (size_t)std::max(n, 1) + order_cub_bytes(n) + 12 * 256; }
+size_t order_scratch_bytes(int n, int ncut) {
+    return (45 + 12 * (size_t)ncut) * (size_t)std::max(n, 1) + order_cub_bytes(n) + 12 * 256;
+}
 
 int order_particles_device(int order, int block, int split_nodes, const double grid_min[3], double dx,
                            const void* x, int x_f64, int soa, long long ld, int n, int* perm, int* starts,
-                           void* scratch, size_t scratch_bytes, hipStream_t s) {
+                           void* scratch, size_t scratch_bytes, hipStream_t s, const void* const* xcut, int ncut) {
     if (n <= 0) return 0;
-    if (scratch_bytes < order_scratch_bytes(n)) throw std::runtime_error("particle ordering: scratch too small");
-    OrderLayout L = order_layout(scratch, n);
+    if (ncut < 0 || ncut > kOrderCutSets - 1) throw std::runtime_error("particle ordering: too many cut sets");
+    if (scratch_bytes < order_scratch_bytes(n, ncut)) throw std::runtime_error("particle ordering: scratch too small");
+    OrderLayout L = order_layout(scratch, n, 1 + ncut);
     const double shift = order == 2 ? 0.5 : 0.0;
     const unsigned nb = (unsigned)((n + 255) / 256);
-#define ORDER_LAUNCH(KERNEL, ...)                                                                           \
+#define ORDER_LAUNCH(KERNEL, X, ...)                                                                        \
     do {                                                                                                    \
         if (x_f64) {                                                                                        \
-            if (soa) hipLaunchKernelGGL((KERNEL<double, true>), dim3(nb), dim3(256), 0, s, (const double*)x, __VA_ARGS__); \
-            else hipLaunchKernelGGL((KERNEL<double, false>), dim3(nb), dim3(256), 0, s, (const double*)x, __VA_ARGS__);   \
+            if (soa) hipLaunchKernelGGL((KERNEL<double, true>), dim3(nb), dim3(256), 0, s, (const double*)(X), __VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<double, false>), dim3(nb), dim3(256), 0, s, (const double*)(X), __VA_ARGS__);   \
         } else {                                                                                            \
-            if (soa) hipLaunchKernelGGL((KERNEL<float, true>), dim3(nb), dim3(256), 0, s, (const float*)x, __VA_ARGS__);   \
-            else hipLaunchKernelGGL((KERNEL<float, false>), dim3(nb), dim3(256), 0, s, (const float*)x, __VA_ARGS__);     \
+            if (soa) hipLaunchKernelGGL((KERNEL<float, true>), dim3(nb), dim3(256), 0, s, (const float*)(X), __VA_ARGS__);   \
+            else hipLaunchKernelGGL((KERNEL<float, false>), dim3(nb), dim3(256), 0, s, (const float*)(X), __VA_ARGS__);     \
         }                                                                                                   \
     } while (0)
-    ORDER_LAUNCH(k_order_keys, ld, n, grid_min[0], grid_min[1], grid_min[2], dx, shift, L.kin, L.vin);
+    ORDER_LAUNCH(k_order_keys, x, ld, n, grid_min[0], grid_min[1], grid_min[2], dx, shift, L.kin, L.vin);
     size_t tb = L.temp_bytes;
     if (hipcub::DeviceRadixSort::SortPairs(L.temp, tb, L.kin, L.kout, L.vin, L.vout, n, 0, 30, s) != hipSuccess)
         throw std::runtime_error("particle ordering: radix sort failed");
@@ -2985,10 +3005,16 @@ int order_particles_device(int order, int block, int split_nodes, const double g
             throw std::runtime_error("particle ordering: copy failed");
         return 0;
     }
-    ORDER_LAUNCH(k_order_cells, ld, n, L.vout, grid_min[0], grid_min[1], grid_min[2], dx, shift, L.cell);
+    // base cells in the sorted order: of x (the runs and the cuts), then of
+    // each further set (the cuts only)
+    ORDER_LAUNCH(k_order_cells, x, ld, n, L.vout, grid_min[0], grid_min[1], grid_min[2], dx, shift, L.cell);
+    for (int q = 0; q < ncut; ++q)
+        ORDER_LAUNCH(k_order_cells, xcut[q], ld, n, L.vout, grid_min[0], grid_min[1], grid_min[2], dx, shift,
+                     L.cell + (size_t)(q + 1) * 3 * n);
 #undef ORDER_LAUNCH
     const int B = block > 0 ? block : 256;
-    hipLaunchKernelGGL(k_block_next, dim3(nb), dim3(256), 0, s, L.cell, n, B, order == 2 ? 3 : 4, split, L.next);
+    hipLaunchKernelGGL(k_block_next, dim3(nb), dim3(256), 0, s, L.cell, 1 + ncut, n, B, order == 2 ? 3 : 4, split,
+                       L.next);
     hipLaunchKernelGGL(k_block_chain, dim3(1), dim3(64), 0, s, L.next, n, starts, L.nblk);
     int nblk = 0;
     if (hipMemcpyAsync(&nblk, L.nblk, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||

@@ -95,6 +95,34 @@ struct dmpm_sim {
     int* bst = nullptr;    // particle block table of the window kernels (nblk + 1 starts)
     int nblk = 0;
     void* stat = nullptr;  // m, vol, lam, mu
+
+    // ---- storage orders of episode segments (dmpm_sort_segments) ----
+    // The layers [ob[s], ob[s+1]) of segment s keep every per-particle array
+    // (state, control, Adam state, control gradient) in storage order s, each
+    // segment sorted by its own middle layer; the adjoint carry, the last
+    // layer's seed and render gradients follow the order of the layer they
+    // belong to.  Order 0 is perm / bst / nblk / stat above; orders 1.. are in
+    // oseg.  Without segment orders (the default) ob is empty.
+    struct OrderSeg {
+        int* perm = nullptr;
+        int* bst = nullptr;
+        int nblk = 0;
+        void* stat = nullptr;
+        int* fwdq = nullptr;    // slot j of this order <- index in the previous order
+        int* bwdq = nullptr;    // slot j of the previous order <- index in this order
+        int* zq = nullptr;      // slot j of order 0 <- index in this order
+    };
+    std::vector<OrderSeg> oseg;
+    std::vector<int> ob;
+    void* ctl_x = nullptr;  // a segment's first control staged in the previous order
+    int nord() const { return 1 + (int)oseg.size(); }
+    int ord_of(int t) const {
+        int o = 0;
+        while (o + 1 < nord() && t >= ob[o + 1]) ++o;
+        return o;
+    }
+    const int* perm_of(int o) const { return o == 0 ? perm : oseg[o - 1].perm; }
+    void* stat_of(int o) const { return o == 0 ? stat : oseg[o - 1].stat; }
     std::vector<void*> layer;  // stored layer states (nullptr = not stored)
     std::vector<char> valid;   // layer computed (stored ones only)
     std::vector<void*> ctl;    // dFc per layer (nullptr = zero)
@@ -289,6 +317,15 @@ struct dmpm_sim {
     void zero(void* p, long long elems) { fill_zero(pr, p, elems, stream); }
 
     // ---- launch descriptor ----
+    // of the window kernels over the particles of a layer stored in order o
+    Launch launch(int o) const {
+        Launch Lc = launch();
+        if (o > 0) {
+            Lc.bst = oseg[o - 1].bst;
+            Lc.nblk = oseg[o - 1].nblk;
+        }
+        return Lc;
+    }
     Launch launch() const {
         Launch Lc{};
         Lc.ph = ph;
@@ -471,9 +508,9 @@ struct dmpm_sim {
 
     // Zero `g` and P2G layer t's state `in` into it.
     void p2g_into(const void* in, int t, void* g) {
-        Launch Lc = launch();
+        Launch Lc = launch(ord_of(t));
         Lc.st = in;
-        Lc.stat = stat;
+        Lc.stat = stat_of(ord_of(t));
         Lc.ctl = ctl[t];
         Lc.grid = g;
         if (cfg.deterministic) {
@@ -487,9 +524,9 @@ struct dmpm_sim {
 
     // G2P of layer t (`in`, grid `g`) -> `out`; no P2G of the result.
     void g2p_only(const void* in, int t, void* g, void* out) {
-        Launch Lc = launch();
+        Launch Lc = launch(ord_of(t));
         Lc.st = in;
-        Lc.stat = stat;
+        Lc.stat = stat_of(ord_of(t));
         Lc.ctl = ctl[t];
         Lc.grid = g;
         Lc.out = out;
@@ -500,15 +537,17 @@ struct dmpm_sim {
     // g1_clear: g1 was cleared by the previous launch.  ahead: the grid of
     // the next step, cleared by this launch's blocks when the fused kernel
     // runs (returns whether it did).
+    // `ctl1`, when given, is layer t+1's control in layer t's storage order
+    // (a step into a segment of another order).
     bool fused_step(const void* in, int t, void* g, void* out, void* g1, bool g1_clear = false,
-                    void* ahead = nullptr) {
-        Launch Lc = launch();
+                    void* ahead = nullptr, const void* const* ctl1 = nullptr) {
+        Launch Lc = launch(ord_of(t));
         Lc.st = in;
-        Lc.stat = stat;
+        Lc.stat = stat_of(ord_of(t));
         Lc.ctl = ctl[t];
         Lc.grid = g;
         Lc.out = out;
-        Lc.ctl1 = t + 1 < L ? ctl[t + 1] : nullptr;
+        Lc.ctl1 = ctl1 ? *ctl1 : t + 1 < L ? ctl[t + 1] : nullptr;
         Lc.grid1 = g1;
         if (cfg.unfused || cfg.deterministic) {
             op_kernel(OP_G2P, Lc);
@@ -577,7 +616,48 @@ struct dmpm_sim {
                     out = scratch[sc];
                     sc ^= 1;
                 }
-                if (t + 1 < L - 1) {
+                const int o1 = ord_of(t + 1);
+                if (o1 != ord_of(t) && t + 1 < L - 1 && !cfg.unfused) {
+                    // Into a segment of another storage order: the fused step
+                    // in the old order (G2P of layer t, P2G of layer t+1 with
+                    // its control staged in the old order) writes layer t+1
+                    // into the scratch state that does not hold `cur` (layer
+                    // t+1 is stored, so the scratch states are free or hold
+                    // layer t), and that is permuted into the new order.
+                    OrderSeg& os = oseg[o1 - 1];
+                    void* tmp = scratch[sc];
+                    const void* c1 = nullptr;
+                    if (ctl[t + 1]) {
+                        if (!ctl_x) ctl_x = dalloc(ctl_bytes());
+                        permute_planes(pr, ctl[t + 1], ctl_x, os.bwdq, n, 9, ld, stream);
+                        c1 = ctl_x;
+                    }
+                    void* gb = kg        ? lgrid[t + 1]
+                               : in_last ? seg_grid[t + 1 - c_last]
+                                         : (ga == grid ? grid2 : grid);
+                    void* ahead = t + 2 < L - 1 ? target(t + 1) : nullptr;
+                    gb_clear = fused_step(cur, t, ga, tmp, gb, gb_clear, ahead, &c1);
+                    permute_planes(pr, tmp, out, os.fwdq, n, NSTATE, ld, stream);
+                    if (kg) gvalid[t + 1] = 1;
+                    ga = gb;
+                } else if (o1 != ord_of(t)) {
+                    // the same without the fused step: G2P in the old order
+                    // into the scratch state, permuted into the new order,
+                    // and the layer's P2G in the new order
+                    OrderSeg& os = oseg[o1 - 1];
+                    void* tmp = scratch[sc];
+                    g2p_only(cur, t, ga, tmp);
+                    permute_planes(pr, tmp, out, os.fwdq, n, NSTATE, ld, stream);
+                    if (t + 1 < L - 1) {
+                        void* gb = kg        ? lgrid[t + 1]
+                                   : in_last ? seg_grid[t + 1 - c_last]
+                                             : (ga == grid ? grid2 : grid);
+                        p2g_into(out, t + 1, gb);
+                        if (kg) gvalid[t + 1] = 1;
+                        ga = gb;
+                        gb_clear = false;
+                    }
+                } else if (t + 1 < L - 1) {
                     void* gb = kg        ? lgrid[t + 1]
                                : in_last ? seg_grid[t + 1 - c_last]
                                          : (ga == grid ? grid2 : grid);
@@ -627,6 +707,13 @@ struct dmpm_sim {
         for (int t = L - 2; t >= stop; --t) {
             const void* st;
             void* g;
+            const int o = ord_of(t);
+            // Out of a segment of another storage order: the carry of layer
+            // t+1 into this order
+            if (ord_of(t + 1) != o) {
+                permute_planes(pr, carry[carry_cur], carry[carry_cur ^ 1], oseg[o].bwdq, n, NSTATE, ld, stream);
+                carry_cur ^= 1;
+            }
             // layer t+1 when it is the forward step of layer t as it stands
             // now (recomputed segment layers always are)
             const void* nxt = nullptr;
@@ -651,9 +738,17 @@ struct dmpm_sim {
                 p2g_into(st, t, g);
             }
             if (K <= 1 && fresh[t + 1] && valid[t + 1]) nxt = layer[t + 1];
-            Launch Lc = launch();
+            // Layer t+1 in another order (the first layer of the next
+            // segment), where the adjoint reads it: permuted into this order
+            // in a scratch state (the backward pass uses neither).
+            if (nxt && ord_of(t + 1) != o && !cfg.adjoint_recompute &&
+                (cfg.adjoint == DMPM_ADJOINT_REFERENCE || !cfg.trust_region)) {
+                permute_planes(pr, nxt, scratch[0], oseg[o].bwdq, n, NSTATE, ld, stream);
+                nxt = scratch[0];
+            }
+            Launch Lc = launch(o);
             Lc.st = st;
-            Lc.stat = stat;
+            Lc.stat = stat_of(o);
             Lc.ctl = ctl[t];
             Lc.grid = g;
             Lc.ggv = gcur;
@@ -680,11 +775,11 @@ struct dmpm_sim {
             // the next layer.
             const void* prev = t - 1 >= stop ? resident_state(t - 1) : nullptr;
             const bool fuse = prev && !cfg.deterministic && !cfg.adjoint_recompute && fresh[t] &&
-                              (cfg.adjoint == DMPM_ADJOINT_REFERENCE || !cfg.trust_region);
+                              (cfg.adjoint == DMPM_ADJOINT_REFERENCE || !cfg.trust_region) && ord_of(t - 1) == o;
             if (fuse) {
-                Launch Lp = launch();
+                Launch Lp = launch(o);
                 Lp.st = prev;
-                Lp.stat = stat;
+                Lp.stat = stat_of(o);
                 Lp.ctl = ctl[t - 1];
                 Lp.ggv = gnext;
                 Lp.nxt = st;
@@ -718,6 +813,55 @@ struct dmpm_sim {
             // no step: dL/d dFc of the last layer is dL/dF
             op_copy(gctl_stop, (char*)carry[carry_cur] + PF * ld * esz, 9 * ld);
         }
+    }
+
+    // ---- segment orders ----
+    // buf (nc planes) := buf permuted by q (slot j <- index q[j]), through tmp
+    // (a state-sized buffer): whole states are swapped in, others copied back.
+    void permute_buf(void*& buf, int nc, const int* q, void*& tmp) {
+        if (!buf) return;
+        permute_planes(pr, buf, tmp, q, n, nc, ld, stream);
+        if (nc == NSTATE) std::swap(buf, tmp);
+        else copy_elems(pr, buf, tmp, (long long)nc * ld, stream);
+    }
+    // Every per-particle array that belongs to layer t (its state when
+    // stored, control, Adam state, control gradient), permuted by q.
+    void permute_layer(int t, const int* q, void*& tmp) {
+        if (t < (int)layer.size()) permute_buf(layer[t], NSTATE, q, tmp);
+        for (auto* v : {&ctl, &am, &av, &avmax, &gctl})
+            if (t < (int)v->size()) permute_buf((*v)[t], 9, q, tmp);
+    }
+    // Back to one storage order (order 0) for every layer.
+    void collapse_orders() {
+        if (oseg.empty()) return;
+        void* tmp = dalloc(state_bytes());
+        for (int t = 0; t < L; ++t) {
+            const int o = ord_of(t);
+            if (o > 0) permute_layer(t, oseg[o - 1].zq, tmp);
+        }
+        const int ol = ord_of(L - 1);
+        if (ol > 0) {
+            permute_buf(adj_last, NSTATE, oseg[ol - 1].zq, tmp);
+            permute_buf(rg, 12, oseg[ol - 1].zq, tmp);
+        }
+        const int oc = carry_layer >= 0 ? ord_of(carry_layer) : 0;
+        if (oc > 0) {
+            for (auto& c : carry) permute_buf(c, NSTATE, oseg[oc - 1].zq, tmp);
+            permute_buf(gctl_stop, 9, oseg[oc - 1].zq, tmp);
+        }
+        dfree(tmp);
+        for (auto& os : oseg) {
+            dfree((void*&)os.perm);
+            dfree((void*&)os.bst);
+            dfree(os.stat);
+            dfree((void*&)os.fwdq);
+            dfree((void*&)os.bwdq);
+            dfree((void*&)os.zq);
+        }
+        oseg.clear();
+        ob.clear();
+        dfree(ctl_x);
+        seg_base = -1;
     }
 
     // adj_last[plane_off..] += src (internal-order planes)
@@ -807,9 +951,10 @@ int red_blocks_for(long long count) {
 
 double end_layer_mass_loss(dmpm_sim* s) {
     const void* st = s->state(s->L - 1);
-    Launch Lc = s->launch();
+    const int o = s->ord_of(s->L - 1);
+    Launch Lc = s->launch(o);
     Lc.st = st;
-    Lc.stat = s->stat;
+    Lc.stat = s->stat_of(o);
     Lc.grid = s->mgrid;
     if (s->cfg.deterministic) {
         det_p2g(s->pr, Lc, true, s->det_buf(), s->stream);
@@ -1075,15 +1220,16 @@ struct Order {
     int* bst = nullptr;
     int nblk = 0;
 };
-Order order_on_device(dmpm_sim* s, const void* x, int x_f64, int soa, long long ld, int n) {
+Order order_on_device(dmpm_sim* s, const void* x, int x_f64, int soa, long long ld, int n,
+                      const void* const* xcut = nullptr, int ncut = 0) {
     Order o;
-    const size_t sb = order_scratch_bytes(n);
+    const size_t sb = order_scratch_bytes(n, ncut);
     void* scr = s->dalloc(sb);
     int* starts = (int*)s->dalloc(((size_t)n + 1) * sizeof(int));
     o.perm = (int*)s->dalloc((size_t)n * sizeof(int));
     try {
         o.nblk = order_particles_device(s->cfg.order, s->cfg.block, s->cfg.split_nodes, s->cfg.grid_min, s->cfg.dx, x,
-                                        x_f64, soa, ld, n, o.perm, starts, scr, sb, s->stream);
+                                        x_f64, soa, ld, n, o.perm, starts, scr, sb, s->stream, xcut, ncut);
     } catch (const std::exception& e) {
         s->dfree(scr);
         s->dfree((void*&)starts);
@@ -1117,6 +1263,7 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
             throw Error(DMPM_ERR_ARG, "too many particles for one sim: a layer must stay below 4 GiB (" +
                                           std::to_string((1ull << 32) / (NSTATE * s->esz)) + " particles)");
         const size_t es = dsize(dtype);
+        s->collapse_orders();
         // storage order from the positions on the device (host positions
         // are uploaded first; device ones are never copied to the host)
         const void* xd = s->stage(x, (size_t)n * 3 * es, on_device);
@@ -1230,6 +1377,7 @@ int dmpm_set_up_comp_graph(dmpm_sim* s, int num_layers) {
         s->refuse_if_captured("dmpm_set_up_comp_graph");
         if (s->n <= 0) throw Error(DMPM_ERR_STATE, "no particles");
         if (num_layers <= 0) throw Error(DMPM_ERR_ARG, "num_layers must be positive");
+        s->collapse_orders();
         for (int t = 1; t < (int)s->ctl.size(); ++t) {
             s->dfree(s->ctl[t]);
             s->dfree(s->am[t]);
@@ -1314,6 +1462,9 @@ int dmpm_get_layer(dmpm_sim* s, int layer, int field, void* out, int dtype, int 
         check_sim(s);
         if (layer < 0 || layer >= s->L) throw Error(DMPM_ERR_INDEX, "Layer index out of range.");
         if (!out) throw Error(DMPM_ERR_ARG, "null output");
+        // the layer's storage order (segment orders) and its statics
+        const int* pm = s->perm_of(s->ord_of(layer));
+        void* const sto = s->stat_of(s->ord_of(layer));
         int nc = 0, plane = 0;
         const void* base = nullptr;
         const size_t es = dsize(dtype);
@@ -1324,10 +1475,10 @@ int dmpm_get_layer(dmpm_sim* s, int layer, int field, void* out, int dtype, int 
             case DMPM_FIELD_C: nc = 9; plane = PC; break;
             case DMPM_FIELD_DFC: nc = 9; break;
             case DMPM_FIELD_FTOTAL: nc = 9; break;
-            case DMPM_FIELD_M: nc = 1; plane = SM; base = s->stat; break;
-            case DMPM_FIELD_VOL: nc = 1; plane = SVOL; base = s->stat; break;
-            case DMPM_FIELD_LAM: nc = 1; plane = SLAM; base = s->stat; break;
-            case DMPM_FIELD_MU: nc = 1; plane = SMU; base = s->stat; break;
+            case DMPM_FIELD_M: nc = 1; plane = SM; base = sto; break;
+            case DMPM_FIELD_VOL: nc = 1; plane = SVOL; base = sto; break;
+            case DMPM_FIELD_LAM: nc = 1; plane = SLAM; base = sto; break;
+            case DMPM_FIELD_MU: nc = 1; plane = SMU; base = sto; break;
             case DMPM_FIELD_P: nc = 9; break;
             case DMPM_FIELD_ADAM_M: nc = 9; base = s->am[layer]; break;
             case DMPM_FIELD_ADAM_V: nc = 9; base = s->av[layer]; break;
@@ -1338,19 +1489,19 @@ int dmpm_get_layer(dmpm_sim* s, int layer, int field, void* out, int dtype, int 
         void* dst = on_device ? out : s->dalloc(bytes);
         const int f64 = dtype == DMPM_DTYPE_F64;
         if (field == DMPM_FIELD_ADAM_M || field == DMPM_FIELD_ADAM_V || field == DMPM_FIELD_ADAM_VMAX) {
-            if (base) export_planes(s->pr, base, dst, f64, s->perm, s->n, 9, s->ld, s->stream);
+            if (base) export_planes(s->pr, base, dst, f64, pm, s->n, 9, s->ld, s->stream);
             else fill_zero(f64 ? PREC_F64 : PREC_F32, dst, (long long)s->n * 9, s->stream);
         } else if (field == DMPM_FIELD_P) {
-            export_stress(s->pr, s->cfg.model, s->state(layer), s->ctl[layer], s->stat, dst, f64, s->perm, s->n,
+            export_stress(s->pr, s->cfg.model, s->state(layer), s->ctl[layer], sto, dst, f64, pm, s->n,
                           s->ld, s->stream);
         } else if (field == DMPM_FIELD_DFC) {
-            if (s->ctl[layer]) export_planes(s->pr, s->ctl[layer], dst, f64, s->perm, s->n, 9, s->ld, s->stream);
+            if (s->ctl[layer]) export_planes(s->pr, s->ctl[layer], dst, f64, pm, s->n, 9, s->ld, s->stream);
             else fill_zero(dtype == DMPM_DTYPE_F64 ? PREC_F64 : PREC_F32, dst, (long long)s->n * 9, s->stream);
         } else if (field == DMPM_FIELD_FTOTAL) {
-            export_F_total(s->pr, s->state(layer), s->ctl[layer], dst, f64, s->perm, s->n, s->ld, s->stream);
+            export_F_total(s->pr, s->state(layer), s->ctl[layer], dst, f64, pm, s->n, s->ld, s->stream);
         } else {
             if (!base) base = s->state(layer);
-            export_planes(s->pr, (const char*)base + plane * s->ld * s->esz, dst, f64, s->perm, s->n, nc, s->ld,
+            export_planes(s->pr, (const char*)base + plane * s->ld * s->esz, dst, f64, pm, s->n, nc, s->ld,
                           s->stream);
         }
         if (!on_device) {
@@ -1387,7 +1538,7 @@ int dmpm_get_layer_grad(dmpm_sim* s, int layer, int field, void* out, int dtype,
         }
         const size_t bytes = (size_t)s->n * nc * dsize(dtype);
         void* dst = on_device ? out : s->dalloc(bytes);
-        export_planes(s->pr, src, dst, dtype == DMPM_DTYPE_F64, s->perm, s->n, nc, s->ld, s->stream);
+        export_planes(s->pr, src, dst, dtype == DMPM_DTYPE_F64, s->perm_of(s->ord_of(layer)), s->n, nc, s->ld, s->stream);
         if (!on_device) {
             HIPCHK(hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, s->stream));
             HIPCHK(hipStreamSynchronize(s->stream));
@@ -1405,7 +1556,7 @@ int dmpm_accumulate_last_layer_grad(dmpm_sim* s, const void* dLdF, const void* d
         auto acc = [&](const void* src, int off, int nc) {
             if (!src) return;
             const void* d = s->stage(src, (size_t)s->n * nc * es, on_device);
-            accumulate_carry(s->pr, d, f64, s->adj_last, off, nc, s->perm, s->n, s->ld, s->stream);
+            accumulate_carry(s->pr, d, f64, s->adj_last, off, nc, s->perm_of(s->ord_of(s->L - 1)), s->n, s->ld, s->stream);
         };
         acc(dLdF, PF, 9);
         acc(dLdx, PX, 3);
@@ -1425,7 +1576,7 @@ int dmpm_set_last_layer_grad(dmpm_sim* s, const void* dLdx, const void* dLdv, co
         auto put = [&](const void* src, int off, int nc) {
             if (!src) return;
             const void* d = s->stage(src, (size_t)s->n * nc * es, on_device);
-            import_planes(s->pr, d, f64, (char*)s->adj_last + off * s->ld * s->esz, s->perm, s->n, nc, s->ld,
+            import_planes(s->pr, d, f64, (char*)s->adj_last + off * s->ld * s->esz, s->perm_of(s->ord_of(s->L - 1)), s->n, nc, s->ld,
                           s->stream);
         };
         put(dLdx, PX, 3);
@@ -1460,8 +1611,8 @@ int dmpm_set_render_gradients(dmpm_sim* s, int n, const void* dLdF, const void* 
         s->has_rg = true;
         if (n == s->n) {
             if (!s->rg) s->rg = s->dalloc((size_t)12 * s->ld * s->esz);
-            import_planes(s->pr, dF, f64, s->rg, s->perm, n, 9, s->ld, s->stream);
-            import_planes(s->pr, dX, f64, (char*)s->rg + 9 * s->ld * s->esz, s->perm, n, 3, s->ld, s->stream);
+            import_planes(s->pr, dF, f64, s->rg, s->perm_of(s->ord_of(s->L - 1)), n, 9, s->ld, s->stream);
+            import_planes(s->pr, dX, f64, (char*)s->rg + 9 * s->ld * s->esz, s->perm_of(s->ord_of(s->L - 1)), n, 3, s->ld, s->stream);
         } else {
             // stored for the info call only: the backward pass skips a
             // mismatched set (CompGraph.cpp:148-160)
@@ -1510,10 +1661,21 @@ int dmpm_sort_particles(dmpm_sim* s, int layer) {
         if (*s->captures > 0)
             throw Error(DMPM_ERR_STATE, "captured graphs or batched episodes hold this sim's buffers; destroy them "
                                         "before sorting its particles");
+        s->collapse_orders();
         const void* st = s->state(layer);
         const long long ld = s->ld;
+        // The blocks are cut so that their windows stay within the bound at
+        // the first and the last layer of the episode too (where computed):
+        // a body that deforms strongly spreads a block cut at one layer over
+        // a larger box at another.
+        const void* xcut[kOrderCutSets - 1];
+        int ncut = 0;
+        for (int t : {0, s->L - 1})
+            if (t != layer && s->stored(t) && s->valid[t] && s->layer[t] && ncut < kOrderCutSets - 1)
+                xcut[ncut++] = (const char*)s->layer[t] + PX * ld * s->esz;
         // q[j]: the stored index of the particle that moves to slot j
-        const Order ord = order_on_device(s, (const char*)st + PX * ld * s->esz, s->pr == PREC_F64, 1, ld, s->n);
+        const Order ord = order_on_device(s, (const char*)st + PX * ld * s->esz, s->pr == PREC_F64, 1, ld, s->n, xcut,
+                                          ncut);
         int* dq = ord.perm;
         void* tmp = s->dalloc(s->state_bytes());
         // every per-particle array in the internal order: into tmp, then
@@ -1547,10 +1709,112 @@ int dmpm_sort_particles(dmpm_sim* s, int layer) {
     });
 }
 
+int dmpm_sort_segments(dmpm_sim* s, int segments) {
+    return guard([&] {
+        check_sim(s);
+        if (segments < 1) throw Error(DMPM_ERR_ARG, "segments must be at least 1");
+        if (*s->captures > 0)
+            throw Error(DMPM_ERR_STATE, "captured graphs or batched episodes hold this sim's buffers; destroy them "
+                                        "before sorting its particles");
+        if (s->cfg.deterministic) throw Error(DMPM_ERR_STATE, "segment orders are not used in deterministic mode");
+        if (s->n <= 0) throw Error(DMPM_ERR_STATE, "no particles");
+        s->collapse_orders();
+        const int L = s->L, K = s->K, n = s->n;
+        const long long ld = s->ld;
+        // segment boundaries: equal lengths, multiples of the checkpoint
+        // interval (a recomputed checkpoint segment stays in one order)
+        int P = (L + segments - 1) / segments;
+        if (K > 1) P = (P + K - 1) / K * K;
+        std::vector<int> b;
+        for (int t = 0; t < L; t += P) b.push_back(t);
+        b.push_back(L);
+        const int S = (int)b.size() - 1;
+        // each segment's order: the Hilbert order of its middle layer, its
+        // blocks cut so that their windows also fit at the stored layers
+        // that bound it
+        std::vector<Order> ord(S);
+        try {
+            for (int q = 0; q < S; ++q) {
+                const int mid = (b[q] + b[q + 1] - 1) / 2;
+                const void* st = s->state(mid);
+                const void* xcut[kOrderCutSets - 1];
+                int ncut = 0;
+                for (int t : {b[q], std::min(b[q + 1], L - 1)})
+                    if (t != mid && s->stored(t) && s->valid[t] && s->layer[t] && ncut < kOrderCutSets - 1)
+                        xcut[ncut++] = (const char*)s->layer[t] + PX * ld * s->esz;
+                ord[q] = order_on_device(s, (const char*)st + PX * ld * s->esz, s->pr == PREC_F64, 1, ld, n, xcut,
+                                         ncut);
+            }
+        } catch (...) {
+            for (auto& o : ord) {
+                s->dfree((void*&)o.perm);
+                s->dfree((void*&)o.bst);
+            }
+            throw;
+        }
+        // ord[q].perm = dq_q: slot j of order q <- index in the current order
+        std::vector<int*> inv(S);
+        for (int q = 0; q < S; ++q) {
+            inv[q] = (int*)s->dalloc((size_t)n * sizeof(int));
+            invert_perm(ord[q].perm, inv[q], n, s->stream);
+        }
+        auto seg_of = [&](int t) {
+            int q = 0;
+            while (q + 1 < S && t >= b[q + 1]) ++q;
+            return q;
+        };
+        std::vector<dmpm_sim::OrderSeg> os(S);
+        for (int q = 0; q < S; ++q) {
+            os[q].perm = (int*)s->dalloc((size_t)n * sizeof(int));
+            permute_ints(s->perm, os[q].perm, ord[q].perm, n, s->stream);
+            os[q].bst = ord[q].bst;
+            os[q].nblk = ord[q].nblk;
+            os[q].stat = s->dalloc((size_t)NSTAT * ld * s->esz);
+            permute_planes(s->pr, s->stat, os[q].stat, ord[q].perm, n, NSTAT, ld, s->stream);
+            if (q == 0) continue;
+            os[q].fwdq = (int*)s->dalloc((size_t)n * sizeof(int));
+            permute_ints(inv[q - 1], os[q].fwdq, ord[q].perm, n, s->stream);
+            os[q].bwdq = (int*)s->dalloc((size_t)n * sizeof(int));
+            permute_ints(inv[q], os[q].bwdq, ord[q - 1].perm, n, s->stream);
+            os[q].zq = (int*)s->dalloc((size_t)n * sizeof(int));
+            permute_ints(inv[q], os[q].zq, ord[0].perm, n, s->stream);
+        }
+        // every per-particle array into the order of the layer it belongs to
+        void* tmp = s->dalloc(s->state_bytes());
+        for (int t = 0; t < L; ++t) s->permute_layer(t, ord[seg_of(t)].perm, tmp);
+        s->permute_buf(s->adj_last, NSTATE, ord[seg_of(L - 1)].perm, tmp);
+        s->permute_buf(s->rg, 12, ord[seg_of(L - 1)].perm, tmp);
+        const int qc = s->carry_layer >= 0 ? seg_of(s->carry_layer) : 0;
+        for (auto& c : s->carry) s->permute_buf(c, NSTATE, ord[qc].perm, tmp);
+        s->permute_buf(s->gctl_stop, 9, ord[qc].perm, tmp);
+        s->dfree(tmp);
+        // install: order 0 in place of the single order, the others after it
+        s->dfree((void*&)s->perm);
+        s->dfree((void*&)s->bst);
+        s->dfree(s->stat);
+        s->perm = os[0].perm;
+        s->bst = os[0].bst;
+        s->nblk = os[0].nblk;
+        s->stat = os[0].stat;
+        s->oseg.assign(os.begin() + 1, os.end());
+        s->ob = b;
+        // staging of a boundary layer's control (allocated here, not while a
+        // forward pass is being captured)
+        if (S > 1) s->ctl_x = s->dalloc(s->ctl_bytes());
+        s->seg_base = -1;
+        HIPCHK(hipStreamSynchronize(s->stream));
+        for (int q = 0; q < S; ++q) {
+            s->dfree((void*&)ord[q].perm);
+            s->dfree((void*&)inv[q]);
+        }
+    });
+}
+
 int dmpm_promote_last_as_initial(dmpm_sim* s, int /*carry_grid*/) {
     return guard([&] {
         check_sim(s);
         s->refuse_if_captured("dmpm_promote_last_as_initial");
+        s->collapse_orders();
         if (s->L <= 0) return;
         const int last = s->L - 1;
         if (last > 0) {
@@ -1603,7 +1867,7 @@ int dmpm_set_control(dmpm_sim* s, int layer, const void* dFc, int dtype, int on_
             s->ctl[layer] = s->dalloc(s->ctl_bytes());
         }
         const void* d = s->stage(dFc, (size_t)s->n * 9 * dsize(dtype), on_device);
-        import_planes(s->pr, d, dtype == DMPM_DTYPE_F64, s->ctl[layer], s->perm, s->n, 9, s->ld, s->stream);
+        import_planes(s->pr, d, dtype == DMPM_DTYPE_F64, s->ctl[layer], s->perm_of(s->ord_of(layer)), s->n, 9, s->ld, s->stream);
         s->unstage();
         s->grids_stale(layer);
     });
@@ -1622,7 +1886,7 @@ int dmpm_set_adam_state(dmpm_sim* s, int layer, const void* m, const void* v, co
         void* dst[3] = {s->am[layer], s->av[layer], s->avmax[layer]};
         for (int q = 0; q < 3; ++q) {
             const void* d = s->stage(src[q], bytes, on_device);
-            import_planes(s->pr, d, f64, dst[q], s->perm, s->n, 9, s->ld, s->stream);
+            import_planes(s->pr, d, f64, dst[q], s->perm_of(s->ord_of(layer)), s->n, 9, s->ld, s->stream);
         }
         s->unstage();
     });
@@ -1648,7 +1912,7 @@ int dmpm_get_control_grad(dmpm_sim* s, int layer, void* out, int dtype, int on_d
         }
         const size_t bytes = (size_t)s->n * 9 * dsize(dtype);
         void* dst = on_device ? out : s->dalloc(bytes);
-        export_planes(s->pr, src, dst, dtype == DMPM_DTYPE_F64, s->perm, s->n, 9, s->ld, s->stream);
+        export_planes(s->pr, src, dst, dtype == DMPM_DTYPE_F64, s->perm_of(s->ord_of(layer)), s->n, 9, s->ld, s->stream);
         if (!on_device) {
             HIPCHK(hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, s->stream));
             HIPCHK(hipStreamSynchronize(s->stream));
@@ -1977,6 +2241,12 @@ int dmpm_batch_capture(dmpm_sim** sims, int count, int start_layer, int stop_lay
             check_sim(sims[i]);
             if (sims[i]->n <= 0) throw Error(DMPM_ERR_STATE, "sim without particles");
             if (sims[i]->cfg.deterministic) throw Error(DMPM_ERR_STATE, "deterministic mode cannot be batched");
+        }
+        // a batched episode runs each sim in one storage order (its recorded
+        // operations have no permutations between segments)
+        for (int i = 0; i < count; ++i) {
+            if (!sims[i]->oseg.empty()) sims[i]->refuse_if_captured("dmpm_batch_capture of a sim with segment orders");
+            sims[i]->collapse_orders();
         }
         dmpm_sim* s0 = sims[0];
         const Launch L0 = s0->launch();

@@ -130,6 +130,7 @@ def lib() -> C.CDLL:
         "dmpm_render_gradient_info": (i, [vp, ip, ip, dp, dp]),
         "dmpm_promote_last_as_initial": (i, [vp, i]),
         "dmpm_sort_particles": (i, [vp, i]),
+        "dmpm_sort_segments": (i, [vp, i]),
         "dmpm_set_control": (i, [vp, i, vp, i, i]),
         "dmpm_request_control_grad": (i, [vp, i, i]),
         "dmpm_get_control_grad": (i, [vp, i, vp, i, i]),
@@ -181,7 +182,7 @@ EXPORTED = [
     "dmpm_set_sim_params", "dmpm_set_up_comp_graph", "dmpm_get_num_layers", "dmpm_compute_forward_pass", "dmpm_rollout_forward",
     "dmpm_end_layer_mass_loss", "dmpm_compute_backward_pass", "dmpm_get_layer", "dmpm_get_layer_grad",
     "dmpm_accumulate_last_layer_grad", "dmpm_set_last_layer_grad", "dmpm_set_render_gradients",
-    "dmpm_clear_render_gradients", "dmpm_render_gradient_info", "dmpm_promote_last_as_initial", "dmpm_sort_particles",
+    "dmpm_clear_render_gradients", "dmpm_render_gradient_info", "dmpm_promote_last_as_initial", "dmpm_sort_particles", "dmpm_sort_segments",
     "dmpm_set_control", "dmpm_request_control_grad", "dmpm_get_control_grad", "dmpm_control_grad_norm",
     "dmpm_descend_adam", "dmpm_optimize_single_timestep", "dmpm_run_optimization",
     "dmpm_single_timestep_adam_count", "dmpm_p2g_mass",

@@ -53,3 +53,47 @@ def tune(scene, physics: Physics, precision: str = "f32", steps: int = 20, check
                           "fallback_frac": fb_blocks / max(1, lds_blocks + fb_blocks)})
     best = min(table, key=lambda r: r["ms"])
     return {"lds_bytes": best["lds_bytes"], "block": best["block"], "table": table}
+
+
+def segment_candidates(num_layers: int, checkpoint_interval: int = 1, lengths: Iterable[int] = (0, 25, 10)) -> list:
+    """Segment counts for `tune_segments`: one per target segment length in
+    layers (0 = the whole episode in one order), dropping counts that give
+    the same segments once their length is rounded up to a multiple of the
+    checkpoint interval (the rule of dmpm_sort_segments)."""
+    K = max(1, int(checkpoint_interval))
+    out, seen = [], set()
+    for length in lengths:
+        S = 1 if length <= 0 else max(1, -(-num_layers // length))
+        P = -(-num_layers // S)
+        P = -(-P // K) * K
+        if P not in seen:
+            seen.add(P)
+            out.append(S)
+    return out
+
+
+def tune_segments(sim: Simulator, candidates: Optional[Iterable[int]] = None, reps: int = 1) -> dict:
+    """Storage orders for the sim's episode.  For each candidate segment
+    count S: dmpm_sort_segments(S), then `reps` timed forward + backward
+    episodes from layer 0 (controls kept); the sim is left sorted with the
+    fastest S.  The layers must have been computed (one episode run): each
+    segment is sorted by the positions of its middle layer.  The episodes
+    recompute the tape and the gradients from the same inputs."""
+    if candidates is None:
+        candidates = segment_candidates(sim.num_layers, sim.cfg.checkpoint_interval)
+    table = []
+    for S in candidates:
+        sim.sort_segments(S)
+        times = []
+        for _ in range(max(1, reps)):
+            sim.synchronize()
+            t0 = time.perf_counter()
+            sim.forward(0, keep_controls=True)
+            sim.backward(0)
+            sim.synchronize()
+            times.append(time.perf_counter() - t0)
+        table.append({"segments": int(S), "ms": 1e3 * min(times)})
+    best = min(table, key=lambda r: r["ms"])
+    if best is not table[-1]:
+        sim.sort_segments(best["segments"])
+    return {"segments": best["segments"], "table": table}

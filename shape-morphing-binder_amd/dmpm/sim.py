@@ -192,6 +192,13 @@ class Simulator:
         positions (results unchanged up to summation order)."""
         L.check(L.lib().dmpm_sort_particles(self._h, int(layer)))
 
+    def sort_segments(self, segments: int):
+        """Reorder the device storage per episode segment: the layers split
+        into `segments` equal runs (multiples of the checkpoint interval),
+        each stored in the Hilbert order of its middle layer (results
+        unchanged up to summation order; the caller's order everywhere)."""
+        L.check(L.lib().dmpm_sort_segments(self._h, int(segments)))
+
     # -- inputs ----------------------------------------------------------------
     def set_particles(self, x, v, F, C_, m, vol, lam, mu, dFc=None):
         n = int(np.shape(x)[0]) if not _is_torch(x) else int(x.shape[0])
