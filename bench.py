@@ -627,7 +627,7 @@ def main():
     extra = {}
     if not args.no_checkpointed and K != 25:
         simc = episode_sim(sc, "f32", 25, T, seed_gx, seed_gF, lds_bytes=lds_bytes, block=block)
-        warm_up(simc, args.warmup, T, sort=not args.no_sort, segments=args.segments)
+        order_c = warm_up(simc, args.warmup, T, sort=not args.no_sort, segments=args.segments)
         t0 = time.perf_counter()
         run_episodes(simc, args.steps)
         simc.synchronize()
@@ -636,7 +636,8 @@ def main():
         simc_out = {k: simc.get(T, k, dtype=np.float64) for k in ("x", "F")}
         simc_out.update({"g" + k: simc.get_grad(0, k, dtype=np.float64) for k in ("x", "v", "F", "C")})
         simc.close()
-        extra["checkpointed"] = {"checkpoint_interval": 25, "value": sc.n * T / wc, "ms_per_step": 1e3 * wc,
+        extra["checkpointed"] = {"checkpoint_interval": 25, "order_segments": order_c,
+                                 "value": sc.n * T / wc, "ms_per_step": 1e3 * wc,
                                  "peak_device_bytes": peak_c}
         out25 = {k: simc_out[k] for k in simc_out}
         log(f"[bench] f32 K=25: {1e3 * wc:.1f} ms/episode, {sc.n * T / wc:.4g} particle-steps/s, "
