@@ -164,12 +164,14 @@ __host__ __device__ inline uint32_t hilbert3(uint32_t x0, uint32_t x1, uint32_t 
 //    floor(x/dx) - 1 for the cubic one (clamped to the 1024^3 lattice), so
 //    that particles sharing a stencil are neighbours and the scatters can
 //    merge their sums across lanes;
-//  * thread blocks of up to `block` consecutive particles, each ended
-//    before the particle that would grow the box of its stencils beyond
-//    split_nodes grid nodes (0: 512; negative: no cuts, consecutive blocks),
-//    at x and at each of the `ncut` (at most kOrderCutSets - 1) further
-//    position sets xcut[] of the same layout -- other layers of an episode,
-//    so that the blocks' windows stay within the bound as the body deforms;
+//  * thread blocks of up to `block` (at most 256) consecutive particles:
+//    every chunk of `block` consecutive particles is cut before each
+//    particle that would grow the box of the current block's stencils
+//    beyond split_nodes grid nodes (0: 512; negative: no cuts, consecutive
+//    blocks), at x and at each of the `ncut` (at most kOrderCutSets - 1)
+//    further position sets xcut[] of the same layout -- other layers of an
+//    episode, so that the blocks' windows stay within the bound as the body
+//    deforms;
 //  * inside each block, its runs of equal base cells re-ordered so that few
 //    cross a 16-lane row (best fit: each row takes the longest remaining run
 //    that fits its free lanes, else the longest remaining run).

@@ -2837,115 +2837,193 @@ __global__ void k_order_cells(const S* x, long long ld, int n, const int* perm, 
     }
 }
 
-// End of the block that would start at particle i: it takes particles while
-// it holds fewer than B and the box of their stencils stays within split
-// nodes in every one of the `sets` base-cell sets (cell + q * 3n holds set
-// q: the sort layer's cells, then other layers').
-__global__ void k_block_next(const int* cell, int sets, int n, int B, int KS, long long split, int* next) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int lo[kOrderCutSets][3], hi[kOrderCutSets][3];
-    for (int q = 0; q < sets; ++q)
-        for (int a = 0; a < 3; ++a) lo[q][a] = hi[q][a] = cell[(long long)q * 3 * n + 3 * (long long)i + a];
-    int j = i + 1;
-    for (; j < n && j - i < B; ++j) {
-        int l2[kOrderCutSets][3], h2[kOrderCutSets][3];
-        bool fits = true;
-        for (int q = 0; q < sets; ++q) {
-            long long vol = 1;
-            for (int a = 0; a < 3; ++a) {
-                const int c = cell[(long long)q * 3 * n + 3 * (long long)j + a];
-                l2[q][a] = min(lo[q][a], c);
-                h2[q][a] = max(hi[q][a], c);
-                vol *= (long long)h2[q][a] - l2[q][a] + KS;
-            }
-            fits = fits && vol <= split;
+// Block cuts inside fixed chunks of B (at most 256) particles, one wave per
+// chunk.  From a block's first particle s the wave forms, for every later
+// particle of the chunk, the box of the base cells of s..that particle in
+// each of the SETS base-cell sets (cell + q * 3n holds set q: the sort
+// layer's cells, then other layers') -- lane l holds particles s + 4l ..
+// s + 4l + 3, its own running box and the boxes of the lanes below through
+// a wave scan -- and the block ends before the first particle whose
+// stencil box exceeds `split` nodes in one of the sets (boxes only grow, so
+// every later particle fails too); the next block starts there.  Every block
+// therefore holds at most B particles and fits in every set (a lone
+// particle always makes a block).  The chunk's block starts go to
+// cut[cB ...], their count to cnt[c].
+template <int SETS>
+__global__ void __launch_bounds__(64) k_chunk_cuts(const int* cell, int n, int B, int KS, long long split, int* cut,
+                                                   int* cnt) {
+    const int lane = threadIdx.x;
+    const long long i0 = (long long)blockIdx.x * B;
+    const int m = (int)min((long long)B, (long long)n - i0);
+    int k = 0;
+    for (int s = 0; s < m;) {  // s, k: the same in every lane
+        // running boxes of this lane's particles, and their cells
+        int lo[4][SETS][3], hi[4][SETS][3];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int p = s + 4 * lane + i;
+#pragma unroll
+            for (int q = 0; q < SETS; ++q)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int v = p < m ? cell[(long long)q * 3 * n + 3 * (i0 + p) + a] : 0;
+                    const int l0 = i == 0 ? INT_MAX : lo[i - 1][q][a], h0 = i == 0 ? INT_MIN : hi[i - 1][q][a];
+                    lo[i][q][a] = p < m ? min(l0, v) : l0;
+                    hi[i][q][a] = p < m ? max(h0, v) : h0;
+                }
         }
-        if (!fits) break;
-        for (int q = 0; q < sets; ++q)
+        // boxes of the lanes below (exclusive wave scan of the lane boxes)
+        int elo[SETS][3], ehi[SETS][3];
+#pragma unroll
+        for (int q = 0; q < SETS; ++q)
+#pragma unroll
             for (int a = 0; a < 3; ++a) {
-                lo[q][a] = l2[q][a];
-                hi[q][a] = h2[q][a];
+                int vl = lo[3][q][a], vh = hi[3][q][a];
+#pragma unroll
+                for (int d = 1; d < 64; d *= 2) {
+                    const int ul = __shfl_up(vl, d), uh = __shfl_up(vh, d);
+                    if (lane >= d) {
+                        vl = min(vl, ul);
+                        vh = max(vh, uh);
+                    }
+                }
+                const int xl = __shfl_up(vl, 1), xh = __shfl_up(vh, 1);
+                elo[q][a] = lane == 0 ? INT_MAX : xl;
+                ehi[q][a] = lane == 0 ? INT_MIN : xh;
             }
+        // this lane's first particle whose box from s does not fit (4: none)
+        int first = 4;
+#pragma unroll
+        for (int i = 3; i >= 0; --i) {
+            const int p = s + 4 * lane + i;
+            bool fits = true;
+#pragma unroll
+            for (int q = 0; q < SETS; ++q) {
+                long long vol = 1;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    vol *= (long long)max(ehi[q][a], hi[i][q][a]) - min(elo[q][a], lo[i][q][a]) + KS;
+                fits = fits && vol <= split;
+            }
+            if (p < m && p > s && !fits) first = i;
+        }
+        const unsigned long long bad = __ballot(first < 4);
+        const int end = bad ? s + 4 * (__ffsll((long long)bad) - 1) + __shfl(first, __ffsll((long long)bad) - 1) : m;
+        if (lane == 0) cut[i0 + k] = (int)i0 + s;
+        ++k;
+        s = end;
     }
-    next[i] = j;
+    if (lane == 0) cnt[blockIdx.x] = k;
 }
 
-// The blocks actually used: 0, next(0), next(next(0)), ... (one thread).
-__global__ void k_block_chain(const int* next, int n, int* starts, int* nblk) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    int k = 0, i = 0;
-    starts[0] = 0;
-    while (i < n) {
-        i = next[i];
-        starts[++k] = i;
+// Block starts of all chunks in order (off: exclusive prefix sum of cnt),
+// the end sentinel n and the block count.
+__global__ void k_chunk_starts(const int* cut, const int* cnt, const int* off, int nchunk, int n, int B, int* starts,
+                               int* nblk) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchunk) return;
+    const int o = off[c], k = cnt[c];
+    for (int q = 0; q < k; ++q) starts[o + q] = cut[(long long)c * B + q];
+    if (c == nchunk - 1) {
+        starts[o + k] = n;
+        *nblk = o + k;
     }
-    *nblk = k;
 }
 
-// One thread per block: its runs of equal base cells placed row by row.
-__global__ void k_pack_rows(const int* cell, const int* starts, int nblk, const int* perm_in, int* perm_out,
-                            int* rlen, int* rstart, unsigned char* used) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nblk) return;
-    const int s0 = starts[b], s1 = starts[b + 1];
+// One wave per block: its runs of equal base cells placed row by row (16
+// lanes), the longest runs first, each into the current row's remaining room
+// when one fits (the longest that fits, the earliest of equal length), else
+// the longest left.  The lanes find the runs (ballots of run heads) and rank
+// them by length; one lane places them, taking runs from per-length queues;
+// the lanes move the particles.  Blocks hold at most 256 particles.
+__global__ void __launch_bounds__(64) k_pack_rows(const int* cell, const int* starts, const int* perm_in,
+                                                  int* perm_out) {
+    __shared__ unsigned short s_at[257];     // run starts (block-relative); s_at[R] = block length
+    __shared__ unsigned short s_sorted[256]; // runs, longest first (stable)
+    __shared__ unsigned short s_dest[256];   // block-relative destination of each run
+    __shared__ unsigned short s_next[257];   // per run length: next unplaced entry of s_sorted
+    __shared__ unsigned short s_end[257];    // per run length: end of its entries in s_sorted
+    const int s0 = starts[blockIdx.x], len = starts[blockIdx.x + 1] - s0;
+    const int lane = threadIdx.x;
+    for (int L = lane; L < 257; L += 64) s_next[L] = s_end[L] = 0;
+    int run_of[4];
     int R = 0;
-    for (int r = s0; r < s1;) {
-        int e = r + 1;
-        while (e < s1 && cell[3 * (long long)e] == cell[3 * (long long)r] &&
-               cell[3 * (long long)e + 1] == cell[3 * (long long)r + 1] &&
-               cell[3 * (long long)e + 2] == cell[3 * (long long)r + 2])
-            ++e;
-        rlen[s0 + R] = e - r;
-        rstart[s0 + R] = r;
-        ++R;
-        r = e;
-    }
-    // stable insertion sort, longest runs first
-    for (int q = 1; q < R; ++q) {
-        const int L = rlen[s0 + q], S = rstart[s0 + q];
-        int p = q;
-        for (; p > 0 && rlen[s0 + p - 1] < L; --p) {
-            rlen[s0 + p] = rlen[s0 + p - 1];
-            rstart[s0 + p] = rstart[s0 + p - 1];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int r = k * 64 + lane;
+        bool head = false;
+        if (r < len) {
+            const long long j = s0 + r;
+            head = r == 0 || cell[3 * j] != cell[3 * j - 3] || cell[3 * j + 1] != cell[3 * j - 2] ||
+                   cell[3 * j + 2] != cell[3 * j - 1];
         }
-        rlen[s0 + p] = L;
-        rstart[s0 + p] = S;
+        const unsigned long long m = __ballot(head);
+        const int before = __popcll(m & ((1ull << lane) - 1));
+        if (head) s_at[R + before] = (unsigned short)r;
+        run_of[k] = R + before + (head ? 0 : -1);
+        R += __popcll(m);
     }
-    for (int q = 0; q < R; ++q) used[s0 + q] = 0;
-    int out = s0;
-    for (int placed = 0; placed < R; ++placed) {
-        const int room = 16 - ((out - s0) % 16);
-        int pick = R;
-        for (int q = 0; q < R; ++q)
-            if (!used[s0 + q] && rlen[s0 + q] <= room) {
-                pick = q;
-                break;
-            }
-        if (pick == R)
-            for (int q = 0; q < R; ++q)
-                if (!used[s0 + q]) {
-                    pick = q;
+    if (lane == 0) s_at[R] = (unsigned short)len;
+    __syncthreads();
+    for (int r = lane; r < R; r += 64) {
+        const int L = s_at[r + 1] - s_at[r];
+        int rank = 0;
+        for (int q = 0; q < R; ++q) {
+            const int Lq = s_at[q + 1] - s_at[q];
+            rank += Lq > L || (Lq == L && q < r);
+        }
+        s_sorted[rank] = (unsigned short)r;
+    }
+    __syncthreads();
+    if (lane == 0 && R > 0) {
+        for (int i = 0; i < R; ++i) {
+            const int L = s_at[s_sorted[i] + 1] - s_at[s_sorted[i]];
+            if (s_end[L] == 0) s_next[L] = (unsigned short)i;
+            s_end[L] = (unsigned short)(i + 1);
+        }
+        int maxL = s_at[s_sorted[0] + 1] - s_at[s_sorted[0]];
+        int out = 0;
+        for (int placed = 0; placed < R; ++placed) {
+            const int room = 16 - out % 16;
+            int pick = 0;
+            for (int L = min(room, maxL); L >= 1; --L)
+                if (s_next[L] < s_end[L]) {
+                    pick = L;
                     break;
                 }
-        used[s0 + pick] = 1;
-        for (int k = 0; k < rlen[s0 + pick]; ++k) perm_out[out++] = perm_in[rstart[s0 + pick] + k];
+            if (pick == 0) {
+                while (s_next[maxL] >= s_end[maxL]) --maxL;
+                pick = maxL;
+            }
+            const int r = s_sorted[s_next[pick]++];
+            s_dest[r] = (unsigned short)out;
+            out += pick;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int r = k * 64 + lane;
+        if (r < len) perm_out[s0 + s_dest[run_of[k]] + (r - s_at[run_of[k]])] = perm_in[s0 + r];
     }
 }
 
 namespace {
 struct OrderLayout {
     unsigned *kin, *kout;
-    int *vin, *vout, *cell, *next, *rlen, *rstart, *nblk;
-    unsigned char* used;
+    int *vin, *vout, *cell, *cut, *cnt, *off, *nblk;
     void* temp;
     size_t temp_bytes;
 };
+// temporary storage of the radix sort and of the chunk scan (used one after
+// the other)
 size_t order_cub_bytes(int n) {
-    size_t temp = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned*)nullptr, (unsigned*)nullptr,
+    size_t sort = 0, scan = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sort, (const unsigned*)nullptr, (unsigned*)nullptr,
                                              (const int*)nullptr, (int*)nullptr, n, 0, 30, (hipStream_t)0);
-    return temp;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (const int*)nullptr, (int*)nullptr, std::max(n, 1),
+                                           (hipStream_t)0);
+    return std::max(sort, scan);
 }
 OrderLayout order_layout(void* base, int n, int sets) {
     OrderLayout L{};
@@ -2961,10 +3039,9 @@ OrderLayout order_layout(void* base, int n, int sets) {
     L.vin = (int*)take(4 * nn);
     L.vout = (int*)take(4 * nn);
     L.cell = (int*)take(12 * nn * (size_t)sets);
-    L.next = (int*)take(4 * nn);
-    L.rlen = (int*)take(4 * nn);
-    L.rstart = (int*)take(4 * nn);
-    L.used = (unsigned char*)take(nn);
+    L.cut = (int*)take(4 * nn);
+    L.cnt = (int*)take(4 * nn);
+    L.off = (int*)take(4 * nn);
     L.nblk = (int*)take(4);
     L.temp_bytes = order_cub_bytes(n);
     L.temp = take(L.temp_bytes);
@@ -2973,7 +3050,7 @@ OrderLayout order_layout(void* base, int n, int sets) {
 }  // namespace
 
 size_t order_scratch_bytes(int n, int ncut) {
-    return (45 + 12 * (size_t)ncut) * (size_t)std::max(n, 1) + order_cub_bytes(n) + 12 * 256;
+    return (44 + 12 * (size_t)ncut) * (size_t)std::max(n, 1) + order_cub_bytes(n) + 11 * 256;
 }
 
 int order_particles_device(int order, int block, int split_nodes, const double grid_min[3], double dx,
@@ -3013,15 +3090,33 @@ int order_particles_device(int order, int block, int split_nodes, const double g
                      L.cell + (size_t)(q + 1) * 3 * n);
 #undef ORDER_LAUNCH
     const int B = block > 0 ? block : 256;
-    hipLaunchKernelGGL(k_block_next, dim3(nb), dim3(256), 0, s, L.cell, 1 + ncut, n, B, order == 2 ? 3 : 4, split,
-                       L.next);
-    hipLaunchKernelGGL(k_block_chain, dim3(1), dim3(64), 0, s, L.next, n, starts, L.nblk);
+    if (B > 256) throw std::runtime_error("particle ordering: blocks hold at most 256 particles");
+    const int nchunk = (int)(((long long)n + B - 1) / B);
+    const unsigned nbc = (unsigned)((nchunk + 255) / 256);
+    {
+        const int KS = order == 2 ? 3 : 4;
+        static_assert(kOrderCutSets == 3, "one k_chunk_cuts instance per set count");
+        if (ncut == 0)
+            hipLaunchKernelGGL(k_chunk_cuts<1>, dim3((unsigned)nchunk), dim3(64), 0, s, L.cell, n, B, KS, split, L.cut,
+                               L.cnt);
+        else if (ncut == 1)
+            hipLaunchKernelGGL(k_chunk_cuts<2>, dim3((unsigned)nchunk), dim3(64), 0, s, L.cell, n, B, KS, split, L.cut,
+                               L.cnt);
+        else
+            hipLaunchKernelGGL(k_chunk_cuts<3>, dim3((unsigned)nchunk), dim3(64), 0, s, L.cell, n, B, KS, split, L.cut,
+                               L.cnt);
+    }
+    size_t tsb = L.temp_bytes;
+    if (hipcub::DeviceScan::ExclusiveSum(L.temp, tsb, L.cnt, L.off, nchunk, s) != hipSuccess)
+        throw std::runtime_error("particle ordering: block scan failed");
+    hipLaunchKernelGGL(k_chunk_starts, dim3(nbc), dim3(256), 0, s, L.cut, L.cnt, L.off, nchunk, n, B, starts,
+                       L.nblk);
     int nblk = 0;
     if (hipMemcpyAsync(&nblk, L.nblk, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         throw std::runtime_error("particle ordering: block count read failed");
-    hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)((nblk + 63) / 64)), dim3(64), 0, s, L.cell, starts, nblk, L.vout,
-                       perm, L.rlen, L.rstart, L.used);
+    if (nblk > 0)
+        hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)nblk), dim3(64), 0, s, L.cell, starts, L.vout, perm);
     return nblk;
 }
 
