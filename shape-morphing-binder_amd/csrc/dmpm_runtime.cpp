@@ -125,6 +125,19 @@ struct dmpm_sim {
     void* stat_of(int o) const { return o == 0 ? stat : oseg[o - 1].stat; }
     std::vector<void*> layer;  // stored layer states (nullptr = not stored)
     std::vector<char> valid;   // layer computed (stored ones only)
+    // pend[t]: stored layer t holds a copy of layer 0 that set-up has not
+    // made yet (it is made when the layer is first read; the forward pass
+    // overwrites it first in the usual flow).  Only in one storage order.
+    std::vector<char> pend;
+    void materialize(int t) {
+        if (t < (int)pend.size() && pend[t]) {
+            copy_elems(pr, layer[t], layer[0], NSTATE * ld, stream);
+            pend[t] = 0;
+        }
+    }
+    void materialize_all() {
+        for (int t = 1; t < (int)pend.size(); ++t) materialize(t);
+    }
     std::vector<void*> ctl;    // dFc per layer (nullptr = zero)
     std::vector<void*> am, av, avmax;  // Adam moments per layer (PointCloud momentum/vector/vector_max)
     std::vector<char> want_gctl;
@@ -444,6 +457,7 @@ struct dmpm_sim {
         layer.resize(L, nullptr);
         valid.assign(L, 0);
         valid[0] = 1;
+        pend.assign(L, 0);
         fresh.assign(L, 0);
         for (int t = 1; t < L; ++t)
             if (stored(t)) layer[t] = dalloc(state_bytes());
@@ -470,6 +484,7 @@ struct dmpm_sim {
         const int c = (t / K) * K;
         if (seg_base == c) return;
         if (!valid[c]) throw Error(DMPM_ERR_STATE, "checkpoint layer " + std::to_string(c) + " missing");
+        materialize(c);
         const int top = std::min(c + K - 1, L - 2);
         seg_base = -1;
         seg[0] = layer[c];
@@ -489,6 +504,7 @@ struct dmpm_sim {
     const void* state(int t) {
         if (stored(t)) {
             if (!valid[t]) throw Error(DMPM_ERR_STATE, "layer " + std::to_string(t) + " has not been computed");
+            materialize(t);
             return layer[t];
         }
         ensure_segment(t);
@@ -503,7 +519,7 @@ struct dmpm_sim {
             if (seg_base < 0 || t < seg_base || t >= seg_base + K || t > L - 2) return nullptr;
             return t == seg_base ? layer[t] : seg[t - seg_base];
         }
-        return valid[t] ? layer[t] : nullptr;
+        return valid[t] && !pend[t] ? layer[t] : nullptr;
     }
 
     // Zero `g` and P2G layer t's state `in` into it.
@@ -668,7 +684,10 @@ struct dmpm_sim {
                 } else {
                     g2p_only(cur, t, ga, out);
                 }
-                if (stored(t + 1)) valid[t + 1] = 1;
+                if (stored(t + 1)) {
+                    valid[t + 1] = 1;
+                    pend[t + 1] = 0;
+                }
                 fresh[t + 1] = 1;
                 cur = out;
             }
@@ -1296,6 +1315,7 @@ int dmpm_set_particles(dmpm_sim* s, int n, const void* x, const void* v, const v
         s->L = 1;
         s->layer.assign(1, nullptr);
         s->valid.assign(1, 1);
+        s->pend.assign(1, 0);
         s->fresh.assign(1, 0);
         s->ctl.assign(1, nullptr);
         s->am.assign(1, nullptr);
@@ -1394,9 +1414,11 @@ int dmpm_set_up_comp_graph(dmpm_sim* s, int num_layers) {
         s->want_gctl.resize(s->L, 0);
         s->alloc_layers();
         // every layer is a copy of layer 0, controls and Adam state included
+        // (the state copies are made when a layer is first read, unless the
+        // forward pass overwrites it first)
         for (int t = 1; t < s->L; ++t) {
             if (s->layer[t]) {
-                copy_elems(s->pr, s->layer[t], s->layer[0], NSTATE * s->ld, s->stream);
+                s->pend[t] = 1;
                 s->valid[t] = 1;
             }
             auto dup = [&](std::vector<void*>& v) {
@@ -1671,7 +1693,7 @@ int dmpm_sort_particles(dmpm_sim* s, int layer) {
         const void* xcut[kOrderCutSets - 1];
         int ncut = 0;
         for (int t : {0, s->L - 1})
-            if (t != layer && s->stored(t) && s->valid[t] && s->layer[t] && ncut < kOrderCutSets - 1)
+            if (t != layer && s->stored(t) && s->valid[t] && !s->pend[t] && s->layer[t] && ncut < kOrderCutSets - 1)
                 xcut[ncut++] = (const char*)s->layer[t] + PX * ld * s->esz;
         // q[j]: the stored index of the particle that moves to slot j
         const Order ord = order_on_device(s, (const char*)st + PX * ld * s->esz, s->pr == PREC_F64, 1, ld, s->n, xcut,
@@ -1719,6 +1741,7 @@ int dmpm_sort_segments(dmpm_sim* s, int segments) {
         if (s->cfg.deterministic) throw Error(DMPM_ERR_STATE, "segment orders are not used in deterministic mode");
         if (s->n <= 0) throw Error(DMPM_ERR_STATE, "no particles");
         s->collapse_orders();
+        s->materialize_all();  // a copy of layer 0 in another order would need its own permutation
         const int L = s->L, K = s->K, n = s->n;
         const long long ld = s->ld;
         // segment boundaries: equal lengths, multiples of the checkpoint
@@ -1740,7 +1763,7 @@ int dmpm_sort_segments(dmpm_sim* s, int segments) {
                 const void* xcut[kOrderCutSets - 1];
                 int ncut = 0;
                 for (int t : {b[q], std::min(b[q + 1], L - 1)})
-                    if (t != mid && s->stored(t) && s->valid[t] && s->layer[t] && ncut < kOrderCutSets - 1)
+                    if (t != mid && s->stored(t) && s->valid[t] && !s->pend[t] && s->layer[t] && ncut < kOrderCutSets - 1)
                         xcut[ncut++] = (const char*)s->layer[t] + PX * ld * s->esz;
                 ord[q] = order_on_device(s, (const char*)st + PX * ld * s->esz, s->pr == PREC_F64, 1, ld, n, xcut,
                                          ncut);
@@ -2158,11 +2181,12 @@ int dmpm_graph_capture(dmpm_sim** sims, int count, int start_layer, int stop_lay
             if (s->cfg.deterministic) throw Error(DMPM_ERR_STATE, "deterministic mode cannot be captured");
             for (int j = 0; j < i; ++j)
                 if (sims[j]->stream == s->stream) throw Error(DMPM_ERR_ARG, "sims must have distinct streams");
-            HIPCHK(hipStreamSynchronize(s->stream));
             s->ktimer = false;
-            // everything the passes allocate lazily must exist before capture
+            // everything the passes allocate or copy lazily must exist before capture
             for (int t = 0; t < s->L; ++t)
                 if (s->want_gctl[t] && !s->gctl[t]) s->gctl[t] = s->dalloc(s->ctl_bytes());
+            s->materialize_all();
+            HIPCHK(hipStreamSynchronize(s->stream));
         }
         HIPCHK(hipStreamBeginCapture(g->origin, hipStreamCaptureModeRelaxed));
         hipError_t cap_err = hipSuccess;
@@ -2247,6 +2271,7 @@ int dmpm_batch_capture(dmpm_sim** sims, int count, int start_layer, int stop_lay
         for (int i = 0; i < count; ++i) {
             if (!sims[i]->oseg.empty()) sims[i]->refuse_if_captured("dmpm_batch_capture of a sim with segment orders");
             sims[i]->collapse_orders();
+            sims[i]->materialize_all();
         }
         dmpm_sim* s0 = sims[0];
         const Launch L0 = s0->launch();
