@@ -108,7 +108,7 @@ typedef struct {
     int split_nodes;        /* thread blocks of the particle kernels take up to `block` consecutive
                                particles (Hilbert order of the uploaded positions), cut where the
                                box of their stencils would exceed this many grid nodes, so that
-                               each block's LDS window stays small (0: 512; negative: no cuts) */
+                               each block's LDS window stays small (0: 704; negative: no cuts) */
 } dmpm_config;
 
 /* Fill `cfg` with the reference's physics (cubic, fixed-corotated, zeta

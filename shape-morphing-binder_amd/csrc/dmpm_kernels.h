@@ -167,7 +167,7 @@ __host__ __device__ inline uint32_t hilbert3(uint32_t x0, uint32_t x1, uint32_t 
 //  * thread blocks of up to `block` (at most 256) consecutive particles:
 //    every chunk of `block` consecutive particles is cut before each
 //    particle that would grow the box of the current block's stencils
-//    beyond split_nodes grid nodes (0: 512; negative: no cuts, consecutive
+//    beyond split_nodes grid nodes (0: kDefaultSplitNodes; negative: no cuts, consecutive
 //    blocks), at x and at each of the `ncut` (at most kOrderCutSets - 1)
 //    further position sets xcut[] of the same layout -- other layers of an
 //    episode, so that the blocks' windows stay within the bound as the body
@@ -179,6 +179,16 @@ __host__ __device__ inline uint32_t hilbert3(uint32_t x0, uint32_t x1, uint32_t 
 // starts (n + 1 ints of room) the blocks' first particles and n; returns the
 // block count (0 without cuts).  Synchronises the stream (the count).
 constexpr int kOrderCutSets = 3;
+// Default block cut: the largest stencil box whose window, padded to odd
+// strides, fits every kernel's share of the default LDS budget (the fused
+// adjoint's gather window, 28 B per node in 24 KiB with FP32 state, is the
+// tightest).  Fewer cuts mean fewer part-filled blocks.  On the 1M-particle,
+// 300-step workload, 704 against 512 nodes: 61.0 against 61.4 ms per
+// episode in 13 segment orders, 65.5 against 68.0 ms in one order, and under
+// the reference physics (cubic stencil, whose boxes are larger) 120.5
+// against 155.4 ms; no window falls back to global atomics (at 768 nodes a
+// few do).
+constexpr int kDefaultSplitNodes = 704;
 size_t order_scratch_bytes(int n, int ncut = 0);
 int order_particles_device(int order, int block, int split_nodes, const double grid_min[3], double dx,
                            const void* x, int x_f64, int soa, long long ld, int n, int* perm, int* starts,

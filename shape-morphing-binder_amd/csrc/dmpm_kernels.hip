@@ -3076,7 +3076,7 @@ int order_particles_device(int order, int block, int split_nodes, const double g
     size_t tb = L.temp_bytes;
     if (hipcub::DeviceRadixSort::SortPairs(L.temp, tb, L.kin, L.kout, L.vin, L.vout, n, 0, 30, s) != hipSuccess)
         throw std::runtime_error("particle ordering: radix sort failed");
-    const long long split = split_nodes == 0 ? 512 : split_nodes;
+    const long long split = split_nodes == 0 ? kDefaultSplitNodes : split_nodes;
     if (split <= 0) {
         if (hipMemcpyAsync(perm, L.vout, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, s) != hipSuccess)
             throw std::runtime_error("particle ordering: copy failed");

@@ -348,7 +348,7 @@ struct dmpm_sim {
         Lc.wstat = count_windows ? wstat : nullptr;
         Lc.bst = bst;
         Lc.nblk = nblk;
-        // default LDS budget: the windows of 256-particle blocks cut at 512
+        // default LDS budget: the windows of 256-particle blocks cut at 704
         // stencil nodes fit 30 KiB with FP32 state.  The fused step holds 94
         // VGPRs (five waves per SIMD), so five blocks should share a CU's
         // 160 KiB; with the kernels' few hundred bytes of static LDS, a
