@@ -21,14 +21,14 @@ T = 12
 CTL_LAYERS = (2, 4, 5, 6, 8, 10, 11)
 
 
-def _sim(name, K):
+def _sim(name, K, **kw):
     from dmpm import Physics, Simulator
     sc, s = perturbed_scene(dfc=0.0)
     # a drift of about a cell every few steps, so that the segments' orders
     # differ from each other and from the uploaded one
     s["v"] = s["v"] + np.array([8.0, -5.0, 3.0])
     sim = Simulator(sc.dims, sc.dx, sc.grid_min, physics=Physics(**physics_kwargs(name, DT)), precision="f64",
-                    checkpoint_interval=K)
+                    checkpoint_interval=K, **kw)
     sim.set_state(dict(s, dFc=None))
     sim.set_up(T + 1)
     rng = np.random.default_rng(8)
@@ -61,10 +61,19 @@ def _same(a, b, tol, what):
         assert e < tol, f"{what}: {k}: {e:.3e}"
 
 
-@pytest.mark.parametrize("name,K", [("plain", 1), ("plain", 3), ("reference", 1), ("reference_exact", 3)])
-def test_segment_orders_keep_results(name, K):
-    sim, sc = _sim(name, K)
-    print(f"[segments] {name} K={K} T={T} n={sc.n}", flush=True)
+@pytest.mark.parametrize("name,K,kw", [("plain", 1, {}), ("plain", 3, {}), ("reference", 1, {}),
+                                       ("reference_exact", 3, {}),
+                                       # the unfused kernels (the crossing's G2P, permutation, P2G)
+                                       ("plain", 1, {"unfused": True}),
+                                       # grids recomputed in the backward pass, not kept
+                                       ("plain", 1, {"recompute_grids": True}),
+                                       # the adjoint recomputing layer t+1 (nothing permuted for it)
+                                       ("reference", 1, {"adjoint_recompute": True}),
+                                       # most blocks cut short
+                                       ("plain", 3, {"split_nodes": 40})])
+def test_segment_orders_keep_results(name, K, kw):
+    sim, sc = _sim(name, K, **kw)
+    print(f"[segments] {name} K={K} {kw} T={T} n={sc.n}", flush=True)
     sim.set_target_grid_mass(np.full(int(np.prod(sc.dims)), 1e-4))
     one = _episode(sim, sc)
     sim.sort_segments(3)
@@ -145,4 +154,14 @@ def test_segment_orders_graph_replays_match_eager():
     with pytest.raises(Exception):
         sim.sort_segments(2)
     g.close()
+    sim.close()
+
+
+def test_segment_orders_refused_in_deterministic_mode():
+    from dmpm._lib import DmpmError
+    sim, sc = _sim("plain", 1, deterministic=True)
+    sim.forward(0, keep_controls=True)
+    with pytest.raises(DmpmError):
+        sim.sort_segments(3)
+    sim.sort_particles(T // 2)  # one order is fine
     sim.close()
