@@ -55,6 +55,10 @@ struct Launch {
     // `st` under the current control and parameters (its v and C are then
     // read instead of recomputed by a G2P from `grid`), else null
     const void* nxt;
+    // rows in another storage order: particle i's new state (forward step)
+    // or adjoint carry (unfused adjoint gather) is written to row opos[i]
+    // instead of i (a step between segments of two orders), or null
+    const int* opos;
     // particle blocks of the window kernels: block b holds particles
     // [bst[b], bst[b+1]) (nblk blocks of at most `block` particles, cut where
     // a block's stencil box would grow too large); null: consecutive blocks of

@@ -79,6 +79,7 @@ struct KP {
     const T* ctl1;  // control of layer t+1 (fused step)
     T* gm1;         // grid t+1 (fused step), laid out as gm
     const T* nxt;   // adjoint: layer t+1 state (v, C read instead of a G2P) or null
+    const int* opos;  // output row of each particle in another storage order, or null
     const int* bst; // particle block table (nblk + 1 starts) or null
     int nblk;
     unsigned long long* wstat;
@@ -138,6 +139,7 @@ static KP<T> make_kp(const Launch& L, int narrays) {
     k.ctl1 = (const T*)L.ctl1;
     k.gm1 = (T*)L.grid1;
     k.nxt = (const T*)L.nxt;
+    k.opos = L.opos;
     k.bst = L.bst;
     k.nblk = L.nblk;
     k.wstat = L.wstat;
@@ -966,10 +968,11 @@ DMPM_DI void particle_update(const KP<T>& a, int i, const v3<T>& x, const m3<T>&
 #pragma unroll
     for (int q = 0; q < 9; ++q) Fn.a[q] = (T(1) - a.sm) * Fu.a[q] + a.sm * F.a[q];
     xn = {{x.a[0] + a.dt * vn.a[0], x.a[1] + a.dt * vn.a[1], x.a[2] + a.dt * vn.a[2]}};
-    store_v3(a.out, a.ld, PX, i, xn);
-    store_v3(a.out, a.ld, PV, i, vn);
-    store_m3(a.out, a.ld, PF, i, Fn);
-    store_m3(a.out, a.ld, PC, i, Cn);
+    const int io = a.opos ? a.opos[i] : i;
+    store_v3(a.out, a.ld, PX, io, xn);
+    store_v3(a.out, a.ld, PV, io, vn);
+    store_m3(a.out, a.ld, PF, io, Fn);
+    store_m3(a.out, a.ld, PC, io, Cn);
 }
 
 template <typename T>
@@ -1646,10 +1649,11 @@ DMPM_DI void adj_gather_block(const KP<T>& a, const KP<T>* b, int lblock, double
             gC.a[0] += tb; gC.a[4] += tb; gC.a[8] += tb;
         }
         const m3<T> gF = madd(Q, gS);
-        store_v3(a.adj0, a.ld, PX, i, gx);
-        store_v3(a.adj0, a.ld, PV, i, gv);
-        store_m3(a.adj0, a.ld, PF, i, gF);
-        store_m3(a.adj0, a.ld, PC, i, gC);
+        const int io = !FUSED && a.opos ? a.opos[i] : i;
+        store_v3(a.adj0, a.ld, PX, io, gx);
+        store_v3(a.adj0, a.ld, PV, io, gv);
+        store_m3(a.adj0, a.ld, PF, io, gF);
+        store_m3(a.adj0, a.ld, PC, io, gC);
         if (FUSED) {
             cyt.gx = gx;
             cyt.gv = gv;

@@ -553,10 +553,11 @@ struct dmpm_sim {
     // g1_clear: g1 was cleared by the previous launch.  ahead: the grid of
     // the next step, cleared by this launch's blocks when the fused kernel
     // runs (returns whether it did).
-    // `ctl1`, when given, is layer t+1's control in layer t's storage order
-    // (a step into a segment of another order).
+    // `ctl1`, when given, is layer t+1's control in layer t's storage order,
+    // and `opos` the row of each particle in layer t+1's order (a step into
+    // a segment of another order).
     bool fused_step(const void* in, int t, void* g, void* out, void* g1, bool g1_clear = false,
-                    void* ahead = nullptr, const void* const* ctl1 = nullptr) {
+                    void* ahead = nullptr, const void* const* ctl1 = nullptr, const int* opos = nullptr) {
         Launch Lc = launch(ord_of(t));
         Lc.st = in;
         Lc.stat = stat_of(ord_of(t));
@@ -564,6 +565,7 @@ struct dmpm_sim {
         Lc.grid = g;
         Lc.out = out;
         Lc.ctl1 = ctl1 ? *ctl1 : t + 1 < L ? ctl[t + 1] : nullptr;
+        Lc.opos = opos;
         Lc.grid1 = g1;
         if (cfg.unfused || cfg.deterministic) {
             op_kernel(OP_G2P, Lc);
@@ -637,11 +639,8 @@ struct dmpm_sim {
                     // Into a segment of another storage order: the fused step
                     // in the old order (G2P of layer t, P2G of layer t+1 with
                     // its control staged in the old order) writes layer t+1
-                    // into the scratch state that does not hold `cur` (layer
-                    // t+1 is stored, so the scratch states are free or hold
-                    // layer t), and that is permuted into the new order.
+                    // straight into its rows in the new order.
                     OrderSeg& os = oseg[o1 - 1];
-                    void* tmp = scratch[sc];
                     const void* c1 = nullptr;
                     if (ctl[t + 1]) {
                         if (!ctl_x) ctl_x = dalloc(ctl_bytes());
@@ -652,8 +651,7 @@ struct dmpm_sim {
                                : in_last ? seg_grid[t + 1 - c_last]
                                          : (ga == grid ? grid2 : grid);
                     void* ahead = t + 2 < L - 1 ? target(t + 1) : nullptr;
-                    gb_clear = fused_step(cur, t, ga, tmp, gb, gb_clear, ahead, &c1);
-                    permute_planes(pr, tmp, out, os.fwdq, n, NSTATE, ld, stream);
+                    gb_clear = fused_step(cur, t, ga, out, gb, gb_clear, ahead, &c1, os.bwdq);
                     if (kg) gvalid[t + 1] = 1;
                     ga = gb;
                 } else if (o1 != ord_of(t)) {
@@ -728,8 +726,9 @@ struct dmpm_sim {
             void* g;
             const int o = ord_of(t);
             // Out of a segment of another storage order: the carry of layer
-            // t+1 into this order
-            if (ord_of(t + 1) != o) {
+            // t+1 into this order.  The adjoint step of layer t+1 wrote it
+            // there itself; only the seed of the last layer is permuted.
+            if (ord_of(t + 1) != o && t + 1 == L - 1) {
                 permute_planes(pr, carry[carry_cur], carry[carry_cur ^ 1], oseg[o].bwdq, n, NSTATE, ld, stream);
                 carry_cur ^= 1;
             }
@@ -758,11 +757,14 @@ struct dmpm_sim {
             }
             if (K <= 1 && fresh[t + 1] && valid[t + 1]) nxt = layer[t + 1];
             // Layer t+1 in another order (the first layer of the next
-            // segment), where the adjoint reads it: permuted into this order
-            // in a scratch state (the backward pass uses neither).
+            // segment), where the adjoint reads it: its v and C planes (all
+            // the adjoint reads of it) permuted into this order in a scratch
+            // state (the backward pass uses neither).
             if (nxt && ord_of(t + 1) != o && !cfg.adjoint_recompute &&
                 (cfg.adjoint == DMPM_ADJOINT_REFERENCE || !cfg.trust_region)) {
-                permute_planes(pr, nxt, scratch[0], oseg[o].bwdq, n, NSTATE, ld, stream);
+                for (const int pl : {(int)PV, (int)PC})
+                    permute_planes(pr, (const char*)nxt + pl * ld * esz, (char*)scratch[0] + pl * ld * esz,
+                                   oseg[o].bwdq, n, pl == PV ? 3 : 9, ld, stream);
                 nxt = scratch[0];
             }
             Launch Lc = launch(o);
@@ -774,6 +776,10 @@ struct dmpm_sim {
             Lc.nxt = cfg.adjoint_recompute ? nullptr : nxt;
             Lc.adj1 = carry[carry_cur];
             Lc.adj0 = carry[carry_cur ^ 1];
+            // the first layer of a segment, with the previous segment's last
+            // layer still to come: its carry goes out in that segment's order
+            // (this step's gather runs unfused, see below)
+            if (t > stop && ord_of(t - 1) != o) Lc.opos = oseg[o - 1].fwdq;
             void* gout = nullptr;
             if (t == stop) gout = gctl_stop;
             else if (want_gctl[t]) {
