@@ -201,9 +201,10 @@ int dmpm_sort_particles(dmpm_sim* sim, int layer);
  * that is advected or deformed drifts away from any single order within a
  * few dozen steps (runs of equal cells break up, windows grow); segment
  * orders keep every layer near an order of its own.  The forward step into
- * a new segment runs in the previous order and its output is permuted into
- * the new one; the adjoint permutes its carry (and the boundary layer, where
- * it reads it) back.  Besides that the sim holds per order a permutation,
+ * a new segment runs in the previous order and writes its output into the
+ * new one's rows; the adjoint's carry goes back the same way (and the
+ * boundary layer's v and C are permuted back, where the adjoint reads
+ * them).  Besides that the sim holds per order a permutation,
  * its block table and the four static planes, and three index maps.
  * Results do not change beyond summation order, and getters,
  * setters and gradients keep the caller's order.  segments = 1 is one order
