@@ -165,3 +165,16 @@ def test_segment_orders_refused_in_deterministic_mode():
         sim.sort_segments(3)
     sim.sort_particles(T // 2)  # one order is fine
     sim.close()
+
+
+def test_tune_segments_leaves_the_fastest_order_and_the_results():
+    from dmpm.autotune import segment_candidates, tune_segments
+    sim, sc = _sim("plain", 1)
+    one = _episode(sim, sc)
+    cands = segment_candidates(T + 1, 1, lengths=(0, 6, 3))
+    assert cands == [1, 3, 5]
+    info = tune_segments(sim, cands)
+    assert info["segments"] in cands and len(info["table"]) == 3
+    assert info["segments"] == min(info["table"], key=lambda r: r["ms"])["segments"]
+    _same(_episode(sim, sc), one, 1e-10, f"after tuning ({info['segments']} segments)")
+    sim.close()
